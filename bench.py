@@ -1,0 +1,204 @@
+"""Throughput of the path-tracing hot path on one MI355X.
+
+Workload (one "step"): the 1920x1080 heightfield render -- 1024^2 cells
+(2,097,152 triangles) displaced by seeded 6-octave fBm (amplitude 0.1),
+per-triangle GGX-over-Lambert materials, sky 1.0 plus a 0.5 deg sun of
+radiance 5e4, 64 spp, max depth 16, StandardPathTracer (NEE + BSDF MIS) with
+the sun/sky sampled as a light.  Scene build and upload happen before the
+timed region; the film lives in HBM.
+
+Prints one JSON line: Mpaths/s (the metric), Mrays/s, the roofline of the
+dominant kernel (closest-hit traversal) and the CPU baseline (the C++ CPU
+checker in oracle/, OpenMP over 16x16 tiles as RenderTarget.h:110-127, on a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+NODE_BYTES, TRI_BYTES = 128, 48
+EXT_RAY_IO_BYTES = 32 + 16     # ray in (o, d) + hit out
+SHADOW_RAY_IO_BYTES = 48 + 16  # shadow record in + radiance update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--max-depth", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by the copy-kernel calibration, if a summary for this
+    workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(desc, args, max_depth, flags):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = pyoracle.OracleScene(desc)
+    W, H = args.width, args.height
+    t0 = time.perf_counter()
+    o.render(W, H, 1, seed=1, max_depth=max_depth, flags=flags, nthreads=threads)
+    t1 = time.perf_counter()
+    one = t1 - t0
+    spp = max(1, min(args.spp, int(args.cpu_seconds / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    o.render(W, H, spp, seed=1, max_depth=max_depth, flags=flags, nthreads=threads, spp_offset=1)
+    dt = time.perf_counter() - t0
+    paths = W * H * spp
+    ext, shd = (int(x) for x in o.last_stats)
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": round(paths / dt / 1e6, 4),
+        "unit": "Mpaths/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{W}x{H} at {spp} spp of the same scene/seed (oracle C++ restatement, -O3 -march=x86-64-v3, "
+                  f"OpenMP over 16x16 tiles, reference-shaped binary BVH), {dt:.1f} s wall; "
+                  f"{(ext + shd) / dt / 1e6:.2f} Mrays/s; {cpu_model}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only, "
+              f"see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from xitils_amd import api, scenes
+
+    W, H, spp, max_depth = args.width, args.height, args.spp, args.max_depth
+    flags = scenes.FLAG_ENV_NEE
+    log(f"[bench] building heightfield {args.cells}^2 cells")
+    t0 = time.perf_counter()
+    desc = scenes.heightfield(cells=args.cells, width=W, height=H)
+    gen_s = time.perf_counter() - t0
+    dev = api.Device(0)
+    t0 = time.perf_counter()
+    scene = api.Scene(dev, desc)
+    upload_s = time.perf_counter() - t0
+    info = scene.info()
+    log(f"[bench] scene {info}")
+    film = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    kw = dict(seed=1, max_depth=max_depth, flags=flags)
+
+    # fetch counts of the identical work (same seed, same rays): one untimed counting render
+    scene.set_counting(True)
+    scene.render_device(film.data_ptr(), W, H, spp, **kw)
+    torch.cuda.synchronize()
+    cst = scene.stats()
+    scene.set_counting(False)
+    film.zero_()
+    log(f"[bench] counting pass: {cst}")
+
+    for k in range(args.warmup):
+        scene.render_device(film.data_ptr(), W, H, spp, spp_offset=0, **kw)
+        log(f"[bench] warmup {k} done")
+    torch.cuda.synchronize()
+    agg = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "extension_rays": 0,
+           "shadow_rays": 0, "extend_launches": 0, "ms_total": 0.0}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        scene.render_device(film.data_ptr(), W, H, spp, **kw)
+        st = scene.stats()
+        for key in agg:
+            agg[key] += st[key]
+        log(f"[bench] step {k}: {st['ms_total']:.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    paths = W * H * spp * args.steps
+    mpaths = paths / wall / 1e6
+    rays = agg["extension_rays"] + agg["shadow_rays"]
+    # roofline of the closest-hit traversal (dominant kernel)
+    n_ext = cst["extension_rays"]
+    bytes_ext = n_ext * EXT_RAY_IO_BYTES + cst["ext_node_fetches"] * NODE_BYTES + cst["ext_tri_fetches"] * TRI_BYTES
+    per_step_ext_bytes = bytes_ext
+    launches = agg["extend_launches"] / args.steps
+    achieved = per_step_ext_bytes / (agg["ms_extend"] / args.steps * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles("extend_heightfield1080p"),
+        "kernel": "k_trace<closest> (extend)",
+        "bytes_per_launch": round(per_step_ext_bytes / launches),
+        "avg_launch_ms": round(agg["ms_extend"] / agg["extend_launches"], 4),
+        "per_ray": {"node_fetches": round(cst["ext_node_fetches"] / n_ext, 3),
+                    "tri_fetches": round(cst["ext_tri_fetches"] / n_ext, 3),
+                    "bytes": round(bytes_ext / n_ext, 1)},
+    }
+    out = {
+        "metric": "Mpaths/s at 1080p",
+        "value": round(mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"heightfield{args.cells}_{W}x{H}_{spp}spp_depth{max_depth}_sky+sun_envNEE",
+                   "triangles": info["num_prims"], "width": W, "height": H, "spp": spp, "max_depth": max_depth,
+                   "integrator": "StandardPathTracer (NEE+BSDF MIS)", "parallelism": "1 GPU"},
+        "mrays_per_s": round(rays / wall / 1e6, 2),
+        "stage_ms_per_step": {k: round(agg[k] / args.steps, 2) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")},
+        "build": {"scene_gen_s": round(gen_s, 2), "bvh_build_upload_s": round(upload_s, 2),
+                  "bvh_nodes": info["bvh_nodes"], "bvh_depth": info["bvh_depth"]},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(desc, args, max_depth, flags)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

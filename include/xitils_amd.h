@@ -1,0 +1,197 @@
+/*
+ * xitils_amd — MI355X (gfx950) path-tracing hot path behind a C ABI.
+ *
+ * The reference (mizuooon/Xitils) renders with
+ *     RenderTarget<T>::render(scene, spp, [&](pFilm, sampler, color) {
+ *         color += pathTracer->eval(scene, sampler, camera->generateRay(pFilm, sampler)).color; })
+ * (RenderTarget.h:110-127, PathTracer.h:18-21, Sandbox/SimplePathTracer/Main.cpp:134-138).
+ * A per-sample host std::function cannot cross to the GPU, so the unit this
+ * library replaces is that whole composite: xt_render() below.  The scene
+ * side replaces Scene::addObject / buildAccelerationStructure (Scene.h:17-28)
+ * and the two ray queries Scene::intersect / intersectAny (Scene.h:30-36).
+ *
+ * Conventions: plain C types only; every call returns xt_status and never
+ * throws; on failure xt_last_error() (thread-local) says why.  Host arrays
+ * passed in are copied; the library owns all device memory behind opaque
+ * handles.  A handle is used by one host thread at a time.
+ */
+#ifndef XITILS_AMD_H
+#define XITILS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum xt_status {
+    XT_OK = 0,
+    XT_ERR_INVALID_ARGUMENT = 1,
+    XT_ERR_UNSUPPORTED_MATERIAL = 2,
+    XT_ERR_OUT_OF_MEMORY = 3,
+    XT_ERR_DEVICE = 4,          /* a HIP runtime call failed */
+    XT_ERR_NO_DEVICE = 5,       /* no gfx950 device at that ordinal */
+    XT_ERR_TOO_LARGE = 6        /* scene or image beyond the supported limits */
+} xt_status;
+
+typedef struct xt_device xt_device;
+typedef struct xt_scene xt_scene;
+
+/* Material kinds (Material.h).  Parameters, 7 floats:
+ *   XT_MAT_DIFFUSE       albedo r, g, b                         Diffuse            Material.h:50-86
+ *   XT_MAT_SPEC_REFL     -                                      SpecularReflection Material.h:177-189
+ *   XT_MAT_SPEC_REFR     index                                  SpecularRefraction Material.h:191-218
+ *   XT_MAT_SPEC_FRESNEL  index                                  SpecularFresnel    Material.h:220-273
+ *   XT_MAT_METAL         alpha (= roughness^2), f0 r, g, b      Metal (GGX)        Material.h:275-384
+ *   XT_MAT_EMISSION      power r, g, b                          Emission           Material.h:386-411
+ *   XT_MAT_GGX_LAMBERT   albedo r, g, b, alpha, f0, p_spec      GGX specular over Lambert, one-sample
+ *                                                               lobe mixture as GlossyWithHighLight
+ *                                                               (Material.h:134-175); no reference twin */
+enum {
+    XT_MAT_DIFFUSE = 0,
+    XT_MAT_SPEC_REFL = 1,
+    XT_MAT_SPEC_REFR = 2,
+    XT_MAT_SPEC_FRESNEL = 3,
+    XT_MAT_METAL = 4,
+    XT_MAT_EMISSION = 5,
+    XT_MAT_GGX_LAMBERT = 6,
+    XT_MAT_COUNT_ = 7
+};
+
+typedef struct xt_material {   /* 32 bytes */
+    uint32_t type;
+    float p[7];
+} xt_material;
+
+/* One Object (Object.h:16-40): a triangle mesh (TriangleMesh::setGeometry
+ * raw-array form, TriangleMesh.h:151-161), its objectToWorld Transform with
+ * the explicit inverse the reference keeps (Transform.h:13-30), and one
+ * material; tri_materials optionally overrides it per triangle. */
+typedef struct xt_object_desc {
+    const float* positions;        /* num_vertices * 3, object space */
+    const float* normals;          /* num_vertices * 3, or NULL (face normals) */
+    const int32_t* indices;        /* num_triangles * 3 */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    const float* object_to_world;  /* 16 floats row-major, or NULL = identity */
+    const float* world_to_object;  /* 16 floats row-major (Transform::matInv), or NULL */
+    uint32_t material;             /* index into xt_scene_desc.materials */
+    const uint32_t* tri_materials; /* num_triangles material indices, or NULL */
+} xt_object_desc;
+
+/* SkySphereUniform (SkySphere.h:36-46) plus an optional sun disk.  Without a
+ * sky the scene has Scene::skySphere == nullptr (escapes add nothing). */
+typedef struct xt_env {
+    int32_t has_sky;
+    float sky[3];
+    float sun_dir[3];       /* unit vector toward the sun */
+    float sun_one_minus_cos;/* 1 - cos(half angle); 0 = no sun */
+    float sun_sin2;         /* sin^2(half angle) */
+    float sun_radiance[3];
+} xt_env;
+
+typedef struct xt_scene_desc {
+    const xt_object_desc* objects;
+    uint32_t num_objects;
+    const xt_material* materials;
+    uint32_t num_materials;
+    xt_env env;
+} xt_scene_desc;
+
+/* PinholeCamera (Camera.h:17-55): cam_to_world is Transform::mat (row-major),
+ * tan_x = tanf(fov / 2), tan_y = tanf(fov / 2 * aspect) with aspect = H / W. */
+typedef struct xt_camera {
+    float cam_to_world[16];
+    float tan_x, tan_y;
+} xt_camera;
+
+enum {
+    XT_INTEGRATOR_STANDARD = 0,  /* StandardPathTracer: NEE + BSDF MIS (PathTracer.h:112-258) */
+    XT_INTEGRATOR_NAIVE = 1      /* NaivePathTracer: BSDF sampling only (PathTracer.h:47-110) */
+};
+enum {
+    XT_FLAG_UNIFORM_BARY = 1u,   /* area-uniform light points (the reference's are not, TriangleIndexed.h:184-192) */
+    XT_FLAG_ENV_NEE = 2u         /* sample sun + sky as a light, MIS with BSDF escapes */
+};
+
+typedef struct xt_render_params {
+    uint32_t width, height;
+    uint32_t spp;            /* samples per pixel in this call */
+    uint32_t spp_offset;     /* index of the first sample (progressive rendering) */
+    int32_t max_depth;       /* bounce cap; <= 0: unbounded, roulette only (as the reference) */
+    uint32_t seed;           /* Philox key word 0 */
+    uint32_t stream;         /* Philox key word 1 */
+    uint32_t integrator;     /* XT_INTEGRATOR_* */
+    uint32_t flags;          /* XT_FLAG_* */
+    int32_t rr_start;        /* roulette when pathLength > rr_start (5, PathTracer.h:255) */
+    float rr_prob;           /* 0.9 (PathTracer.h:256) */
+    float ray_eps;           /* 1e-5 origin offset along d (PathTracer.h:254) */
+    float shadow_margin;     /* 1e-4 (PathTracer.h:257) */
+} xt_render_params;
+
+/* Counters of the last xt_render on a scene. */
+typedef struct xt_render_stats {
+    uint64_t camera_rays, extension_rays, shadow_rays;  /* extension_rays includes camera rays */
+    double ms_total;         /* wall time of the call's GPU work (HIP events) */
+    double ms_extend, ms_shade, ms_shadow, ms_other;    /* per-stage kernel time (HIP events) */
+    uint32_t extend_launches;
+    /* BVH node (128 B) and triangle (48 B) fetches of the traversal kernels;
+     * filled only after xt_scene_set_counting(scene, 1). */
+    uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
+} xt_render_stats;
+
+typedef struct xt_ray {        /* 32 bytes */
+    float ox, oy, oz, tmax;
+    float dx, dy, dz, pad;
+} xt_ray;
+
+typedef struct xt_hit {        /* 16 bytes; miss: t = +inf, prim = 0xFFFFFFFF */
+    float t;
+    uint32_t prim;             /* global primitive id: object order, then triangle order */
+    float b0, b1;              /* barycentrics of vertices 0 and 1 (TriangleIndexed.h:127-131) */
+} xt_hit;
+
+/* Reads the .so's version string. */
+const char* xt_version(void);
+/* Thread-local description of the last error on this thread. */
+const char* xt_last_error(void);
+
+/* Opens HIP device `ordinal`; fails with XT_ERR_NO_DEVICE unless it is gfx950. */
+xt_status xt_device_open(int ordinal, xt_device** out);
+xt_status xt_device_close(xt_device* dev);
+
+/* Scene::addObject x N + Scene::buildAccelerationStructure (Scene.h:17-28):
+ * flattens instances to world space, builds the BVH on the host and uploads
+ * everything to HBM. */
+xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** out);
+xt_status xt_scene_destroy(xt_scene* scene);
+/* Host-side facts about the built scene: num_prims, bvh_nodes, bvh_depth,
+ * bytes_nodes, bytes_tris, build_ms (build_ms as double bits). */
+xt_status xt_scene_info(const xt_scene* scene, uint64_t out[6]);
+
+/* Scene::intersect (Scene.h:30-32): closest hit for each ray.  rays/hits are
+ * host pointers, or device pointers when on_device != 0. */
+xt_status xt_trace_closest(xt_scene* scene, const xt_ray* rays, uint32_t n, xt_hit* hits, int on_device);
+/* Scene::intersectAny (Scene.h:34-36): occluded[i] = 1 when any triangle lies
+ * in (0, tmax]. */
+xt_status xt_trace_any(xt_scene* scene, const xt_ray* rays, uint32_t n, uint8_t* occluded, int on_device);
+
+/* RenderTarget::render + PinholeCamera::generateRay + PathTracer::eval, for
+ * params->spp samples of every pixel: film (width*height*3, row-major, y
+ * down) += the sum of the path radiances, per pixel in sample order.  film is
+ * float32 or, with film_is_f64 != 0, float64 (the converged-oracle pass);
+ * host or device memory per film_on_device. */
+xt_status xt_render(xt_scene* scene, const xt_camera* camera, const xt_render_params* params, void* film,
+                    int film_is_f64, int film_on_device);
+xt_status xt_render_get_stats(const xt_scene* scene, xt_render_stats* out);
+/* Measurement hook (no reference counterpart): renders of this scene use the
+ * traversal variant that counts node and triangle fetches into the stats
+ * (algorithmic-traffic accounting for the roofline).  Results are identical. */
+xt_status xt_scene_set_counting(xt_scene* scene, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XITILS_AMD_H */

@@ -1,0 +1,248 @@
+// ORACLE — test infrastructure only: extern "C" entry points over xo_core.hpp
+// for the pytest suite (ctypes) and the CPU-baseline leg of bench.py.
+//
+// RenderTarget<T>::render (RenderTarget.h:110-127) is restated as an OpenMP
+// `schedule(dynamic, 1)` loop over 16x16 tiles, then samples, then pixels.
+#include "xo_core.hpp"
+
+#include <omp.h>
+
+using namespace xo;
+
+extern "C" {
+
+void* orc_scene_new(void) { return new Scene(); }
+void orc_scene_free(void* s) { delete (Scene*)s; }
+
+int orc_scene_add_material(void* s, uint32_t type, const float* p7) {
+    Scene* sc = (Scene*)s;
+    Material m;
+    m.type = type;
+    for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    sc->materials.push_back(m);
+    return (int)sc->materials.size() - 1;
+}
+
+// positions/normals are num_vertices*3 floats in object space; xf16/xfinv16
+// are the row-major 4x4 Transform::mat / matInv (NULL = identity).
+int orc_scene_add_object(void* s, const float* positions, const float* normals, int num_vertices,
+                         const int32_t* indices, int num_triangles, const float* xf16, const float* xfinv16,
+                         uint32_t material, const uint32_t* tri_materials) {
+    Scene* sc = (Scene*)s;
+    Object o;
+    o.mesh.pos.resize(num_vertices);
+    for (int i = 0; i < num_vertices; ++i) o.mesh.pos[i] = V3(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]);
+    if (normals) {
+        o.mesh.nrm.resize(num_vertices);
+        for (int i = 0; i < num_vertices; ++i) o.mesh.nrm[i] = V3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    }
+    o.mesh.idx.assign(indices, indices + 3 * num_triangles);
+    if (xf16) {
+        for (int i = 0; i < 16; ++i) o.xf.m[i / 4][i % 4] = xf16[i];
+        for (int i = 0; i < 16; ++i) o.xf.mi[i / 4][i % 4] = xfinv16[i];
+    }
+    o.material = material;
+    if (tri_materials) o.triMaterial.assign(tri_materials, tri_materials + num_triangles);
+    sc->objects.push_back(std::move(o));
+    return (int)sc->objects.size() - 1;
+}
+
+// sky: has_sky, rgb[3]; sun: dir[3] (unit), 1 - cos(half), sin^2(half), rgb[3]
+void orc_scene_set_env(void* s, int has_sky, const float* sky_rgb, const float* sun_dir, float sun_omc,
+                       float sun_sin2, const float* sun_rgb) {
+    Scene* sc = (Scene*)s;
+    sc->env.hasSky = has_sky != 0;
+    sc->env.sky = V3(sky_rgb[0], sky_rgb[1], sky_rgb[2]);
+    sc->env.sunDir = V3(sun_dir[0], sun_dir[1], sun_dir[2]);
+    sc->env.sunOmc = sun_omc;
+    sc->env.sunSin2 = sun_sin2;
+    sc->env.sunL = V3(sun_rgb[0], sun_rgb[1], sun_rgb[2]);
+}
+
+int orc_scene_build(void* s) {
+    ((Scene*)s)->build();
+    return 0;
+}
+
+uint32_t orc_scene_num_prims(void* s) { return ((Scene*)s)->nprim; }
+
+static Ray rayFrom(const float* r8) {
+    Ray r;
+    r.o = V3(r8[0], r8[1], r8[2]);
+    r.tMax = r8[3];
+    r.d = V3(r8[4], r8[5], r8[6]);
+    return r;
+}
+
+// rays: n x {ox, oy, oz, tmax, dx, dy, dz, pad}; hits: n x {t, prim(u32 bits), b0, b1}
+// mode 0 = two-level reference BVH, 1 = brute force.  Miss: t = +inf, prim = 0xFFFFFFFF.
+void orc_trace_closest(void* s, const float* rays, int n, float* hits, int mode, int nthreads) {
+    const Scene* sc = (const Scene*)s;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int i = 0; i < n; ++i) {
+        Ray r = rayFrom(rays + 8 * i);
+        float t = Infinity, b0 = 0, b1 = 0;
+        uint32_t prim = 0xFFFFFFFFu;
+        if (mode == 0) {
+            Isect is;
+            if (sc->intersect(r, &is)) { t = is.t; prim = is.prim; b0 = is.b0; b1 = is.b1; }
+        } else {
+            sc->intersectBrute(r, &t, &prim, &b0, &b1);
+        }
+        hits[4 * i + 0] = t;
+        std::memcpy(&hits[4 * i + 1], &prim, 4);
+        hits[4 * i + 2] = b0;
+        hits[4 * i + 3] = b1;
+    }
+}
+
+void orc_trace_any(void* s, const float* rays, int n, uint8_t* occluded, int mode, int nthreads) {
+    const Scene* sc = (const Scene*)s;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int i = 0; i < n; ++i) {
+        Ray r = rayFrom(rays + 8 * i);
+        occluded[i] = (mode == 0 ? sc->intersectAny(r) : sc->intersectAnyBrute(r)) ? 1 : 0;
+    }
+}
+
+// cam: 16 floats camToWorld (row-major mat) then tanX, tanY.
+// ip: width, height, spp, spp_offset, max_depth, seed, stream, integrator, flags, rr_start
+// fp: rr_prob, eps, shadow_margin
+// film32 (W*H*3, +=) and/or film64 (W*H*3, +=) may be NULL.  stats: {ext, shadow}.
+void orc_render(void* s, const float* cam, const int32_t* ip, const float* fp, float* film32, double* film64,
+                uint64_t* stats, int nthreads) {
+    const Scene* sc = (const Scene*)s;
+    Camera c;
+    for (int i = 0; i < 16; ++i) c.camToWorld.m[i / 4][i % 4] = cam[i];
+    c.tanX = cam[16];
+    c.tanY = cam[17];
+    RenderParams rp;
+    rp.width = ip[0]; rp.height = ip[1]; rp.spp = ip[2]; rp.sppOffset = ip[3]; rp.maxDepth = ip[4];
+    rp.seed = (uint32_t)ip[5]; rp.stream = (uint32_t)ip[6]; rp.integrator = ip[7]; rp.flags = (uint32_t)ip[8];
+    rp.rrStart = ip[9];
+    rp.rrProb = fp[0]; rp.eps = fp[1]; rp.shadowMargin = fp[2];
+    const int W = rp.width, H = rp.height, TS = 16;
+    const int tx = (W + TS - 1) / TS, ty = (H + TS - 1) / TS;
+    uint64_t ext = 0, shd = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : ext, shd) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int ti = 0; ti < tx * ty; ++ti) {
+        const int ox = (ti % tx) * TS, oy = (ti / tx) * TS;
+        PathStats st;
+        for (int sIdx = 0; sIdx < rp.spp; ++sIdx) {
+            for (int ly = 0; ly < TS; ++ly) {
+                if (oy + ly >= H) continue;
+                for (int lx = 0; lx < TS; ++lx) {
+                    if (ox + lx >= W) continue;
+                    const int x = ox + lx, y = oy + ly;
+                    PathRng rng;
+                    rng.key[0] = rp.seed; rng.key[1] = rp.stream;
+                    rng.pixel = (uint32_t)(y * W + x);
+                    rng.sample = (uint32_t)(rp.sppOffset + sIdx);
+                    float fx, fy;
+                    filmPosition(x, y, W, H, rng.uni(0, DIM_FILM_X), rng.uni(0, DIM_FILM_Y), &fx, &fy);
+                    Ray ray = c.generate(fx, fy);
+                    V3 L = rp.integrator == 1 ? evalNaive(*sc, rng, ray, rp, &st) : evalStandard(*sc, rng, ray, rp, &st);
+                    size_t o = 3 * (size_t)(y * W + x);
+                    if (film32) { film32[o] += L.x; film32[o + 1] += L.y; film32[o + 2] += L.z; }
+                    if (film64) { film64[o] += L.x; film64[o + 1] += L.y; film64[o + 2] += L.z; }
+                }
+            }
+        }
+        ext += st.extRays;
+        shd += st.shadowRays;
+    }
+    if (stats) { stats[0] = ext; stats[1] = shd; }
+}
+
+// ---- unit-level entry points for the pin tests --------------------------
+
+void orc_philox(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) { philox4x32_10(ctr4, key2, out4); }
+
+float orc_uniform(uint32_t seed, uint32_t stream, uint32_t pixel, uint32_t sample, uint32_t slot, int dim) {
+    PathRng r;
+    r.key[0] = seed; r.key[1] = stream; r.pixel = pixel; r.sample = sample;
+    return r.uni(slot, dim);
+}
+
+// isect9: p[3], n[3], wo[3] (shading normal = faceForward(n, wo)).
+static Isect isectFrom(const float* i9, const Material* m) {
+    Isect is;
+    is.p = V3(i9[0], i9[1], i9[2]);
+    is.n = V3(i9[3], i9[4], i9[5]);
+    is.wo = V3(i9[6], i9[7], i9[8]);
+    is.sn = faceForward(is.n, is.wo);
+    is.mat = m;
+    return is;
+}
+
+// out: wi[3], pdf, value[3]
+void orc_bsdf_sample(uint32_t type, const float* p7, const float* isect9, const float* u3, float* out7) {
+    Material m; m.type = type; for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    Isect is = isectFrom(isect9, &m);
+    V3 wi; float pdf = 0;
+    V3 v = evalAndSample(m, is, u3[0], u3[1], u3[2], &wi, &pdf);
+    out7[0] = wi.x; out7[1] = wi.y; out7[2] = wi.z; out7[3] = pdf; out7[4] = v.x; out7[5] = v.y; out7[6] = v.z;
+}
+
+// out: bsdfCos[3], pdf
+void orc_bsdf_eval(uint32_t type, const float* p7, const float* isect9, const float* wi3, float* out4) {
+    Material m; m.type = type; for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    Isect is = isectFrom(isect9, &m);
+    V3 wi(wi3[0], wi3[1], wi3[2]);
+    V3 f = bsdfCos(m, is, wi);
+    out4[0] = f.x; out4[1] = f.y; out4[2] = f.z;
+    out4[3] = bsdfPdf(m, is, wi);
+}
+
+float orc_ggx_D(float alpha, const float* h3, const float* n3) {
+    GGX g{alpha};
+    return g.D(V3(h3[0], h3[1], h3[2]), V3(n3[0], n3[1], n3[2]));
+}
+float orc_ggx_G1(float alpha, const float* x3, const float* n3) {
+    GGX g{alpha};
+    return g.G1(V3(x3[0], x3[1], x3[2]), V3(n3[0], n3[1], n3[2]));
+}
+
+int orc_tri_test(const float* p9, const float* ray8, float* out3) {
+    Ray r = rayFrom(ray8);
+    float t = 0, b0 = 0, b1 = 0;
+    int ok = triTest(V3(p9[0], p9[1], p9[2]), V3(p9[3], p9[4], p9[5]), V3(p9[6], p9[7], p9[8]), r, r.tMax, &t, &b0, &b1);
+    out3[0] = t; out3[1] = b0; out3[2] = b1;
+    return ok;
+}
+
+void orc_camera_ray(const float* cam, float fx, float fy, float* ray8) {
+    Camera c;
+    for (int i = 0; i < 16; ++i) c.camToWorld.m[i / 4][i % 4] = cam[i];
+    c.tanX = cam[16]; c.tanY = cam[17];
+    Ray r = c.generate(fx, fy);
+    ray8[0] = r.o.x; ray8[1] = r.o.y; ray8[2] = r.o.z; ray8[3] = r.tMax;
+    ray8[4] = r.d.x; ray8[5] = r.d.y; ray8[6] = r.d.z; ray8[7] = 0;
+}
+
+// Light sample for (seed, pixel, sample, slot): out p[3], n[3], sn[3], pdf, obj, tri
+void orc_sample_light(void* s, uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t slot, int uniform_bary,
+                      float* out12) {
+    const Scene* sc = (const Scene*)s;
+    PathRng r; r.key[0] = seed; r.key[1] = 0; r.pixel = pixel; r.sample = sample;
+    SampledLight l = sc->sampleLight(r, slot, uniform_bary != 0);
+    float v[12] = {l.p.x, l.p.y, l.p.z, l.n.x, l.n.y, l.n.z, l.sn.x, l.sn.y, l.sn.z, l.pdf, (float)l.obj, (float)l.tri};
+    std::memcpy(out12, v, sizeof v);
+}
+
+// Reference-BVH statistics: node count and max depth (structure checks).
+void orc_ref_bvh_info(void* s, int obj, int32_t* out2) {
+    const Scene* sc = (const Scene*)s;
+    const RefBVH& b = obj < 0 ? sc->tlas : sc->objects[obj].blas;
+    out2[0] = (int32_t)b.nodes.size();
+    int maxd = 0;
+    std::vector<std::pair<int, int>> st{{0, 0}};
+    while (!st.empty()) {
+        auto [n, d] = st.back(); st.pop_back();
+        maxd = std::max(maxd, d);
+        if (b.nodes[n].prim < 0) { st.push_back({b.nodes[n].child[0], d + 1}); st.push_back({b.nodes[n].child[1], d + 1}); }
+    }
+    out2[1] = maxd;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+"""ctypes binding of include/xitils_amd.h (the C ABI of libxitils_amd.so).
+
+This is the Python face of the drop-in: it drives the same entry points a
+C++ caller (include/xitils/xitils.hpp) or any other FFI would bind.  There is
+no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scenes as _sc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libxitils_amd.so")
+
+XT_OK = 0
+STATUS_NAMES = {0: "XT_OK", 1: "XT_ERR_INVALID_ARGUMENT", 2: "XT_ERR_UNSUPPORTED_MATERIAL", 3: "XT_ERR_OUT_OF_MEMORY",
+                4: "XT_ERR_DEVICE", 5: "XT_ERR_NO_DEVICE", 6: "XT_ERR_TOO_LARGE"}
+
+EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy",
+            "xt_scene_info", "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats",
+            "xt_scene_set_counting"]
+
+
+class XtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class xt_material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("p", C.c_float * 7)]
+
+
+class xt_object_desc(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("object_to_world", C.c_void_p), ("world_to_object", C.c_void_p),
+                ("material", C.c_uint32), ("tri_materials", C.c_void_p)]
+
+
+class xt_env(C.Structure):
+    _fields_ = [("has_sky", C.c_int32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
+                ("sun_one_minus_cos", C.c_float), ("sun_sin2", C.c_float), ("sun_radiance", C.c_float * 3)]
+
+
+class xt_scene_desc(C.Structure):
+    _fields_ = [("objects", C.c_void_p), ("num_objects", C.c_uint32), ("materials", C.c_void_p),
+                ("num_materials", C.c_uint32), ("env", xt_env)]
+
+
+class xt_camera(C.Structure):
+    _fields_ = [("cam_to_world", C.c_float * 16), ("tan_x", C.c_float), ("tan_y", C.c_float)]
+
+
+class xt_render_params(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_offset", C.c_uint32),
+                ("max_depth", C.c_int32), ("seed", C.c_uint32), ("stream", C.c_uint32), ("integrator", C.c_uint32),
+                ("flags", C.c_uint32), ("rr_start", C.c_int32), ("rr_prob", C.c_float), ("ray_eps", C.c_float),
+                ("shadow_margin", C.c_float)]
+
+
+class xt_render_stats(C.Structure):
+    _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
+                ("ms_shadow", C.c_double), ("ms_other", C.c_double), ("extend_launches", C.c_uint32),
+                ("ext_node_fetches", C.c_uint64), ("ext_tri_fetches", C.c_uint64),
+                ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libxitils_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libxitils_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.xt_version.restype = C.c_char_p
+        L.xt_last_error.restype = C.c_char_p
+        L.xt_device_open.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.xt_device_close.argtypes = [C.c_void_p]
+        L.xt_scene_create.argtypes = [C.c_void_p, C.POINTER(xt_scene_desc), C.POINTER(C.c_void_p)]
+        L.xt_scene_destroy.argtypes = [C.c_void_p]
+        L.xt_scene_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.xt_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
+        L.xt_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int]
+        L.xt_render.argtypes = [C.c_void_p, C.POINTER(xt_camera), C.POINTER(xt_render_params), C.c_void_p, C.c_int,
+                                C.c_int]
+        L.xt_render_get_stats.argtypes = [C.c_void_p, C.POINTER(xt_render_stats)]
+        L.xt_scene_set_counting.argtypes = [C.c_void_p, C.c_int]
+        for f in ("xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy", "xt_scene_info",
+                  "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != XT_OK:
+        raise XtError(status, lib().xt_last_error().decode())
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        h = C.c_void_p()
+        check(lib().xt_device_open(ordinal, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().xt_device_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def env_struct(e: "_sc.Env") -> xt_env:
+    omc, sin2 = e.sun_terms()
+    s = xt_env()
+    s.has_sky = int(e.has_sky)
+    s.sky[:] = [float(x) for x in e.sky]
+    s.sun_dir[:] = [float(x) for x in e.sun_dir]
+    s.sun_one_minus_cos = float(omc)
+    s.sun_sin2 = float(sin2)
+    s.sun_radiance[:] = [float(x) for x in e.sun_radiance]
+    return s
+
+
+def camera_struct(cam: "_sc.Camera") -> xt_camera:
+    c = xt_camera()
+    packed = cam.packed()
+    c.cam_to_world[:] = [float(x) for x in packed[:16]]
+    c.tan_x = float(packed[16])
+    c.tan_y = float(packed[17])
+    return c
+
+
+class Scene:
+    """A scene uploaded to HBM (Scene.h:11-61 after buildAccelerationStructure)."""
+
+    def __init__(self, device: Device, desc: "_sc.SceneDesc"):
+        self.device = device
+        self.desc = desc
+        keep = []
+        objs = (xt_object_desc * max(1, len(desc.objects)))()
+        for k, o in enumerate(desc.objects):
+            pos = np.ascontiguousarray(o.positions, np.float32)
+            idx = np.ascontiguousarray(o.indices, np.int32)
+            keep += [pos, idx]
+            objs[k].positions = pos.ctypes.data
+            objs[k].indices = idx.ctypes.data
+            objs[k].num_vertices = pos.shape[0]
+            objs[k].num_triangles = idx.shape[0]
+            if o.normals is not None:
+                n = np.ascontiguousarray(o.normals, np.float32)
+                keep.append(n)
+                objs[k].normals = n.ctypes.data
+            if o.xf is not None:
+                m = np.ascontiguousarray(o.xf, np.float32)
+                mi = np.ascontiguousarray(o.xf_inv, np.float32)
+                keep += [m, mi]
+                objs[k].object_to_world = m.ctypes.data
+                objs[k].world_to_object = mi.ctypes.data
+            objs[k].material = o.material
+            if o.tri_materials is not None:
+                tm = np.ascontiguousarray(o.tri_materials, np.uint32)
+                keep.append(tm)
+                objs[k].tri_materials = tm.ctypes.data
+        mats = _sc.pack_materials(desc.materials)
+        keep.append(mats)
+        sd = xt_scene_desc()
+        sd.objects = C.cast(objs, C.c_void_p)
+        sd.num_objects = len(desc.objects)
+        sd.materials = mats.ctypes.data
+        sd.num_materials = mats.shape[0]
+        sd.env = env_struct(desc.env)
+        h = C.c_void_p()
+        check(lib().xt_scene_create(device.h, C.byref(sd), C.byref(h)))
+        self.h = h
+        del keep
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            lib().xt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def info(self):
+        out = (C.c_uint64 * 6)()
+        check(lib().xt_scene_info(self.h, out))
+        build_ms = np.array([out[5]], np.uint64).view(np.float64)[0]
+        return {"num_prims": out[0], "bvh_nodes": out[1], "bvh_depth": out[2], "bytes_nodes": out[3],
+                "bytes_tris": out[4], "build_ms": float(build_ms)}
+
+    # ---- ray queries -----------------------------------------------------
+    def trace_closest(self, rays, on_device=False):
+        """rays: (n, 8) float32 {o, tmax, d, pad} (host array or device pointer
+        with on_device and an explicit n).  Returns (n, 4) float32 {t,
+        prim bits, b0, b1}."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        check(lib().xt_trace_closest(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data, 0))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        occ = np.zeros(rays.shape[0], np.uint8)
+        check(lib().xt_trace_any(self.h, rays.ctypes.data, rays.shape[0], occ.ctypes.data, 0))
+        return occ
+
+    def trace_closest_dev(self, rays_ptr, n, hits_ptr):
+        check(lib().xt_trace_closest(self.h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr), 1))
+
+    def trace_any_dev(self, rays_ptr, n, occ_ptr):
+        check(lib().xt_trace_any(self.h, C.c_void_p(rays_ptr), n, C.c_void_p(occ_ptr), 1))
+
+    # ---- rendering -------------------------------------------------------
+    @staticmethod
+    def params(width, height, spp, seed=1, max_depth=0, integrator=0, flags=0, spp_offset=0, stream=0,
+               rr_start=5, rr_prob=0.9, eps=1e-5, shadow_margin=1e-4):
+        p = xt_render_params()
+        p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
+        p.max_depth, p.seed, p.stream, p.integrator, p.flags = max_depth, seed, stream, integrator, flags
+        p.rr_start, p.rr_prob, p.ray_eps, p.shadow_margin = rr_start, rr_prob, eps, shadow_margin
+        return p
+
+    def render(self, width, height, spp, film=None, f64=False, camera=None, **kw):
+        """Host-film render: returns (height, width, 3) float32/float64 film +=
+        radiance sums."""
+        cam = camera_struct(camera or self.desc.camera)
+        p = self.params(width, height, spp, **kw)
+        if film is None:
+            film = np.zeros((height, width, 3), np.float64 if f64 else np.float32)
+        check(lib().xt_render(self.h, C.byref(cam), C.byref(p), film.ctypes.data, int(film.dtype == np.float64), 0))
+        return film
+
+    def render_device(self, film_ptr, width, height, spp, f64=False, camera=None, **kw):
+        """Renders into a device film (e.g. a torch tensor's data_ptr())."""
+        cam = camera_struct(camera or self.desc.camera)
+        p = self.params(width, height, spp, **kw)
+        check(lib().xt_render(self.h, C.byref(cam), C.byref(p), C.c_void_p(film_ptr), int(f64), 1))
+
+    def set_counting(self, on):
+        check(lib().xt_scene_set_counting(self.h, int(on)))
+
+    def stats(self):
+        s = xt_render_stats()
+        check(lib().xt_render_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in xt_render_stats._fields_}

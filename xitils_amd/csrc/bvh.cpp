@@ -1,0 +1,266 @@
+// Host binned-SAH BVH build, collapsed to BVH4 (see bvh.h).
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <thread>
+
+namespace xt {
+
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow(const float* p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct BNode {
+    Box box;
+    int left = -1, right = -1;  // left < 0: leaf
+    uint32_t first = 0, count = 0;
+};
+
+struct Builder {
+    const BVHBuildOptions& opt;
+    std::vector<Box> pbox;
+    std::vector<float> cen;  // 3 per prim
+    std::vector<uint32_t> idx;
+    std::vector<BNode> nodes;
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> spare_threads{0};
+
+    explicit Builder(const BVHBuildOptions& o) : opt(o) {}
+
+    uint32_t alloc() { return next.fetch_add(1); }
+
+    // Builds the subtree over idx[b, e) into node ni.
+    void build(uint32_t ni, uint32_t b, uint32_t e) {
+        BNode& nd = nodes[ni];
+        Box box, cb;
+        for (uint32_t i = b; i < e; ++i) {
+            box.grow(pbox[idx[i]]);
+            cb.grow(&cen[3 * idx[i]]);
+        }
+        nd.box = box;
+        const uint32_t n = e - b;
+        auto make_leaf = [&]() {
+            nd.left = nd.right = -1;
+            nd.first = b;
+            nd.count = n;
+        };
+        if (n <= 1) {
+            make_leaf();
+            return;
+        }
+        const int B = opt.bins;
+        float best_cost = INFINITY;
+        int best_axis = -1, best_split = -1;
+        const float parent_area = box.area();
+        for (int axis = 0; axis < 3; ++axis) {
+            float ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0.0f)) continue;
+            float scale = (float)B / ext;
+            std::vector<Box> bb(B);
+            std::vector<uint32_t> bc(B, 0);
+            for (uint32_t i = b; i < e; ++i) {
+                uint32_t p = idx[i];
+                int k = std::min(B - 1, (int)((cen[3 * p + axis] - cb.lo[axis]) * scale));
+                bb[k].grow(pbox[p]);
+                bc[k]++;
+            }
+            std::vector<float> ra(B);
+            std::vector<uint32_t> rc(B);
+            Box acc;
+            uint32_t cnt = 0;
+            for (int k = B - 1; k > 0; --k) {
+                acc.grow(bb[k]);
+                cnt += bc[k];
+                ra[k] = acc.area();
+                rc[k] = cnt;
+            }
+            Box lacc;
+            uint32_t lcnt = 0;
+            for (int k = 0; k < B - 1; ++k) {
+                lacc.grow(bb[k]);
+                lcnt += bc[k];
+                if (lcnt == 0 || rc[k + 1] == 0) continue;
+                float c = opt.cost_node + (lacc.area() * lcnt + ra[k + 1] * rc[k + 1]) / std::max(parent_area, 1e-30f);
+                if (c < best_cost) {
+                    best_cost = c;
+                    best_axis = axis;
+                    best_split = k;
+                }
+            }
+        }
+        if ((int)n <= opt.max_leaf && (best_axis < 0 || (float)n <= best_cost)) {
+            make_leaf();
+            return;
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = b + n / 2;  // identical centroids: split by index
+        } else {
+            const int axis = best_axis;
+            const float lo = cb.lo[axis], scale = (float)B / (cb.hi[axis] - cb.lo[axis]);
+            auto it = std::partition(idx.begin() + b, idx.begin() + e, [&](uint32_t p) {
+                int k = std::min(B - 1, (int)((cen[3 * p + axis] - lo) * scale));
+                return k <= best_split;
+            });
+            mid = (uint32_t)(it - idx.begin());
+            if (mid == b || mid == e) mid = b + n / 2;
+        }
+        uint32_t l = alloc(), r = alloc();
+        nd.left = (int)l;
+        nd.right = (int)r;
+        // large subtrees go to another thread while spare threads remain
+        if (n > 65536 && spare_threads.fetch_sub(1) > 0) {
+            auto f = std::async(std::launch::async, [this, l, b, mid]() { build(l, b, mid); });
+            build(r, mid, e);
+            f.get();
+            spare_threads.fetch_add(1);
+        } else {
+            if (n > 65536) spare_threads.fetch_add(1);
+            build(l, b, mid);
+            build(r, mid, e);
+        }
+    }
+};
+
+struct Collapser {
+    const std::vector<BNode>& bn;
+    const std::vector<uint32_t>& idx;
+    BVH4Result& out;
+    uint32_t depth = 0;
+
+    static void set_child(float* node, int k, const Box& b, int code) {
+        node[0 * 4 + k] = b.lo[0];
+        node[1 * 4 + k] = b.hi[0];
+        node[2 * 4 + k] = b.lo[1];
+        node[3 * 4 + k] = b.hi[1];
+        node[4 * 4 + k] = b.lo[2];
+        node[5 * 4 + k] = b.hi[2];
+        std::memcpy(&node[6 * 4 + k], &code, 4);
+    }
+
+    int leaf(const BNode& n) {
+        uint32_t first = (uint32_t)out.slot_prim.size();
+        for (uint32_t i = 0; i < n.count; ++i) out.slot_prim.push_back(idx[n.first + i]);
+        out.max_leaf = std::max(out.max_leaf, n.count);
+        return ~(int)((first << 3) | (n.count - 1));
+    }
+
+    // Emits a BVH4 node whose children are the binary subtrees `kids`.
+    uint32_t emit(std::vector<int> kids, uint32_t level) {
+        depth = std::max(depth, level);
+        uint32_t me = out.num_nodes++;
+        out.nodes.resize((size_t)out.num_nodes * 32);
+        {
+            float* node = &out.nodes[(size_t)me * 32];
+            for (int k = 0; k < 4; ++k) {
+                Box empty;
+                set_child(node, k, empty, 0);
+            }
+            std::memset(&node[7 * 4], 0, 16);
+        }
+        for (size_t k = 0; k < kids.size(); ++k) {
+            const BNode& c = bn[kids[k]];
+            int code;
+            if (c.left < 0) {
+                code = leaf(c);
+            } else {
+                std::vector<int> g{c.left, c.right};
+                expand(g);
+                code = (int)emit(g, level + 1);
+            }
+            set_child(&out.nodes[(size_t)me * 32], (int)k, c.box, code);
+        }
+        return me;
+    }
+
+    // Replaces the largest-area internal child by its two children until four.
+    void expand(std::vector<int>& g) {
+        while (g.size() < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (size_t k = 0; k < g.size(); ++k) {
+                const BNode& c = bn[g[k]];
+                if (c.left >= 0 && c.box.area() > ba) {
+                    ba = c.box.area();
+                    best = (int)k;
+                }
+            }
+            if (best < 0) break;
+            int l = bn[g[best]].left, r = bn[g[best]].right;
+            g[best] = l;
+            g.push_back(r);
+        }
+    }
+};
+
+}  // namespace
+
+BVH4Result build_bvh4(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt) {
+    BVH4Result out;
+    const uint32_t n = (uint32_t)tris.size();
+    Builder bld(opt);
+    int hw = opt.threads > 0 ? opt.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    bld.spare_threads = hw - 1;
+    bld.pbox.resize(n);
+    bld.cen.resize(3 * (size_t)n);
+    bld.idx.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Box b;
+        for (int v = 0; v < 3; ++v) b.grow(tris[i].p[v]);
+        bld.pbox[i] = b;
+        for (int k = 0; k < 3; ++k) bld.cen[3 * i + k] = 0.5f * (b.lo[k] + b.hi[k]);
+        bld.idx[i] = i;
+    }
+    if (n == 0) {
+        out.nodes.assign(32, 0.0f);
+        float* node = out.nodes.data();
+        for (int k = 0; k < 4; ++k) Collapser::set_child(node, k, Box(), 0);
+        out.num_nodes = 1;
+        out.depth = 1;
+        return out;
+    }
+    bld.nodes.resize(2 * (size_t)n);
+    uint32_t root = bld.alloc();
+    bld.build(root, 0, n);
+    bld.nodes.resize(bld.next.load());
+    Collapser col{bld.nodes, bld.idx, out};
+    out.slot_prim.reserve(n);
+    const BNode& r = bld.nodes[root];
+    std::vector<int> g;
+    if (r.left < 0) {
+        g.push_back((int)root);
+    } else {
+        g = {r.left, r.right};
+        col.expand(g);
+    }
+    col.emit(g, 1);
+    out.depth = col.depth;
+    return out;
+}
+
+}  // namespace xt

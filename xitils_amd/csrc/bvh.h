@@ -1,0 +1,36 @@
+// Host BVH builder: binned SAH binary tree collapsed to a 4-wide BVH with
+// 128-byte nodes and leaf-ordered triangles (layout in xt_device.h).
+//
+// Replaces the reference's build (AccelerationStructure.h:163-291: sort +
+// 8 equal-count buckets, "SAH" on 2xyz, one primitive per leaf).  Hit results
+// do not depend on the tree: traversal returns the lexicographic minimum of
+// (t, prim id) over triangles passing the reference test.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace xt {
+
+struct BuildTri {
+    float p[3][3];   // world-space vertices
+};
+
+struct BVH4Result {
+    std::vector<float> nodes;          // NODE_F4 * 4 floats per node
+    std::vector<uint32_t> slot_prim;   // leaf slot -> prim index (input order)
+    uint32_t num_nodes = 0;
+    uint32_t depth = 0;                // levels of BVH4 nodes (root = 1)
+    uint32_t max_leaf = 0;
+};
+
+struct BVHBuildOptions {
+    int bins = 32;
+    int max_leaf = 4;          // <= 8 (3-bit count in the leaf code)
+    float cost_node = 1.0f;    // SAH traversal cost relative to one triangle test
+    int threads = 0;           // 0 = hardware concurrency
+};
+
+BVH4Result build_bvh4(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt);
+
+}  // namespace xt

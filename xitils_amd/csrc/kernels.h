@@ -1,0 +1,62 @@
+// Kernel-side types and launch declarations shared by kernels.hip and the
+// host runtime (runtime.hip).
+#pragma once
+
+#include "xt_device.h"
+
+namespace xt {
+
+constexpr int TRACE_BLOCK = 128;      // lanes per block of the traversal kernels (2 waves)
+constexpr int TRACE_LDS_DEPTH = 32;   // LDS stack entries per lane (16 KB per block)
+
+// Per-batch constants of the wavefront pipeline.
+struct WaveParams {
+    uint32_t width, height;
+    uint32_t first_pixel, num_pixels;   // pixel range of the batch
+    uint32_t num_paths;                 // num_pixels * samples in the batch
+    uint32_t sample0;                   // global sample index of the batch's first sample
+    uint32_t seed, stream;
+    uint32_t integrator, flags;
+    int32_t max_depth, rr_start;
+    float rr_prob, eps, shadow_margin;
+    float tan_x, tan_y;
+    float cam[16];
+};
+
+// Extension-ray queue, SoA of float4 (64 B per entry):
+//   o = (origin, pdf of the BSDF sample that made the ray; -1 = delta)
+//   d = (direction, path id)
+//   w = (path weight, vertex index v | flags << 24)   v = vertex the ray will hit (1 = camera hit)
+//   e = (material_eval of the sample, env-light pdf of the direction)
+struct PathQueue {
+    float4* o;
+    float4* d;
+    float4* w;
+    float4* e;
+};
+constexpr uint32_t QF_PENDING = 1u;   // the previous vertex's NEE result waits in pending[id]
+
+// Shadow-ray queue (48 B per entry):
+//   o = (origin, tmax), d = (direction, path id), c = (contribution, mode: 1 = store to pending)
+struct ShadowQueue {
+    float4* o;
+    float4* d;
+    float4* c;
+};
+
+template <bool ANY, int LDS_DEPTH, bool COUNT>
+__global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, int ray_stride,
+                        const uint32_t* count_ptr, uint32_t count_fixed, float4* hits, uint8_t* occluded,
+                        int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals);
+
+__global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
+__global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_t* count_in, const float4* hits,
+                        PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
+                        float4* radiance, float4* pending);
+template <bool COUNT>
+__global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending,
+                         uint32_t* ovf, unsigned long long* visit_totals);
+__global__ void k_accumulate(WaveParams wp, const float4* radiance, void* film, int film_f64);
+__global__ void k_zero_u32(uint32_t* p, uint32_t n);
+
+}  // namespace xt

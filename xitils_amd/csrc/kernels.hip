@@ -1,0 +1,462 @@
+// gfx950 kernels: ray queries and the wavefront path tracer.
+//
+// Wavefront pipeline per batch of paths (one path = one pixel sample):
+//   generate -> [extend -> shade -> shadow]* -> accumulate
+// extend  : closest hit for every queued ray         (Scene::intersect, Scene.h:30)
+// shade   : one StandardPathTracer / NaivePathTracer loop step
+//           (PathTracer.h:147-246 / 65-101): emission at the new vertex,
+//           roulette, BSDF sample -> next queue, light sample -> shadow queue
+// shadow  : any hit for every shadow ray             (Scene::intersectAny, Scene.h:34)
+// Queues are compacted with one atomic per wave (ballot + popcount prefix).
+// Radiance is kept per path id and summed in the reference's order (BSDF-hit
+// emission of vertex v+1 before the light sample of vertex v), so the image
+// does not depend on queue order.
+#include "kernels.h"
+#include "xt_traverse.h"
+
+namespace xt {
+
+// Wave-aggregated append: returns this lane's slot (valid when pred).  Every
+// active lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    uint32_t lane = __lane_id();
+    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + prefix;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------- ray queries
+
+// Adds a wave's visit counts to the 64-bit totals (counting build only).
+__device__ __forceinline__ void flush_counts(unsigned long long* totals, const VisitCount& vc) {
+    unsigned long long a = vc.nodes, b = vc.tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&totals[0], a);
+        atomicAdd(&totals[1], b);
+    }
+}
+
+template <bool ANY, int LDS_DEPTH, bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
+                                                        const float4* __restrict__ ray_d, int ray_stride,
+                                                        const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
+                                                        float4* __restrict__ hits, uint8_t* __restrict__ occluded,
+                                                        int hit_prim_ids, int tmax_from_ray, uint32_t* __restrict__ ovf,
+                                                        unsigned long long* __restrict__ visit_totals) {
+    __shared__ uint32_t lds[LDS_DEPTH * TRACE_BLOCK];
+    const uint32_t n = count_ptr ? *count_ptr : count_fixed;
+    const size_t nthreads = (size_t)gridDim.x * blockDim.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* my_lds = lds + threadIdx.x;
+    uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
+    VisitCount vc;
+    for (size_t i = gtid; i < n; i += nthreads) {
+        float4 o4 = ray_o[i * ray_stride];
+        float4 d4 = ray_d[i * ray_stride];
+        RayPre r = ray_setup(xyz(o4), xyz(d4));
+        const float tmax = tmax_from_ray ? o4.w : INFINITY;   // extension rays: tMax = Infinity (PathTracer.h:167)
+        TraceResult res;
+        if (ANY) {
+            occluded[i] = traverse<true, LDS_DEPTH, COUNT>(sc, r, tmax, my_lds, TRACE_BLOCK, my_ovf, nthreads, &res, &vc)
+                              ? 1 : 0;
+        } else {
+            bool hit = traverse<false, LDS_DEPTH, COUNT>(sc, r, tmax, my_lds, TRACE_BLOCK, my_ovf, nthreads, &res, &vc);
+            hits[i] = hit ? make_float4(res.t, __uint_as_float(hit_prim_ids ? res.prim : res.slot), res.b0, res.b1)
+                          : make_float4(INFINITY, __uint_as_float(INVALID), 0.0f, 0.0f);
+        }
+    }
+    if (COUNT) flush_counts(visit_totals, vc);
+}
+
+#define XT_INST_TRACE(ANY, COUNT)                                                                                  \
+    template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT>(DevScene, const float4*, const float4*, int,     \
+                                                                  const uint32_t*, uint32_t, float4*, uint8_t*, int, \
+                                                                  int, uint32_t*, unsigned long long*);
+XT_INST_TRACE(false, false)
+XT_INST_TRACE(true, false)
+XT_INST_TRACE(false, true)
+XT_INST_TRACE(true, true)
+#undef XT_INST_TRACE
+
+// ---------------------------------------------------------------- wavefront PT
+
+// Camera ray of a path (PinholeCamera::generateRay, Camera.h:45-55) at the
+// jittered film position of RenderTargetTile::GenerateFilmPosition
+// (RenderTarget.h:68-80).
+__global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, float4* __restrict__ radiance) {
+    const uint32_t P = wp.num_paths;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < P; id += gridDim.x * blockDim.x) {
+        uint32_t s_local = id / wp.num_pixels;
+        uint32_t pixel = wp.first_pixel + (id - s_local * wp.num_pixels);
+        uint32_t x = pixel % wp.width, y = pixel / wp.width;
+        Rng rng{make_uint2(wp.seed, wp.stream), pixel, wp.sample0 + s_local};
+        uint4 b0 = rng.block(0, 0);
+        float ux = Rng::to_unit(b0.x), uy = Rng::to_unit(b0.y);
+        float fx = (float)x, fy = (float)y;
+        fx /= (float)wp.width;
+        fy /= (float)wp.height;
+        fy = 1.0f - fy;
+        fx += (ux - 0.5f) / (float)wp.width;
+        fy += (uy - 0.5f) / (float)wp.height;
+        f3 dc = normalize(mk(wp.tan_x * (fx - 0.5f) * 2, wp.tan_y * (fy - 0.5f) * 2, 1));
+        f3 o = xf_point(wp.cam, mk(0, 0, 0));
+        f3 d = xf_vector(wp.cam, dc);
+        q.o[id] = make_float4(o.x, o.y, o.z, 0.0f);
+        q.d[id] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
+        q.w[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));
+        q.e[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        radiance[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// SurfaceIntersection at a hit (TriangleIndexed.h:133-172, then the identity
+// pass of Object.h:106-144 where Transform::asNormal normalises n, wo and
+// shading.n once more).
+__device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 d, float b0, float b1) {
+    const float4* tr = sc.tris + (size_t)slot * TRI_F4;
+    float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
+    f3 p0 = xyz(v0), p1 = xyz(v1), p2 = xyz(v2);
+    Surf s;
+    s.slot = slot;
+    s.mat = __float_as_uint(v1.w);
+    uint32_t flags = __float_as_uint(v2.w);
+    s.p = lerp3(p0, p1, p2, b0, b1);
+    f3 wo = normalize(-d);
+    f3 n;
+    if (flags & TRI_FLAG_NORMALS) {
+        const float4* nr = sc.normals + (size_t)slot * TRI_F4;
+        n = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), b0, b1));
+    } else {
+        n = normalize(cross(p1 - p0, p2 - p0));
+    }
+    f3 sn = face_forward(n, wo);
+    s.n = normalize(n);
+    s.wo = normalize(wo);
+    s.sn = normalize(sn);
+    return s;
+}
+
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
+                                               const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
+                                               PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
+                                               uint32_t* __restrict__ count_shadow, float4* __restrict__ radiance,
+                                               float4* __restrict__ pending) {
+    const uint32_t n = *count_in;
+    const bool standard = wp.integrator == XT_INTEGRATOR_STANDARD;
+    const bool can_sample = sc.num_lights > 0;
+    const EnvNee envn = env_nee_setup(sc.env, standard && (wp.flags & XT_FLAG_ENV_NEE) != 0, can_sample);
+    const float p_area = can_sample ? 1.0f - envn.p_select : 0.0f;
+    const bool uniform_bary = (wp.flags & XT_FLAG_UNIFORM_BARY) != 0;
+    const uint2 key = make_uint2(wp.seed, wp.stream);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // whole-wave iterations so the appends below always see a converged wave
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool push_ext = false, push_sh = false;
+        f3 ext_o = splat(0.0f), ext_d = splat(0.0f), ext_w = splat(0.0f), ext_e = splat(0.0f);
+        float ext_pdf = 0.0f, ext_pe = 0.0f;
+        uint32_t ext_v = 0;
+        f3 sh_o = splat(0.0f), sh_d = splat(0.0f), sh_c = splat(0.0f);
+        float sh_tmax = 0.0f;
+        uint32_t id = 0;
+        if (i < n) {
+            const float4 o4 = qin.o[i], d4 = qin.d[i], w4 = qin.w[i], e4 = qin.e[i], h = hits[i];
+            id = __float_as_uint(d4.w);
+            const uint32_t vf = __float_as_uint(w4.w);
+            const uint32_t v = vf & 0xFFFFFFu, qflags = vf >> 24;
+            const float pdf_prev = o4.w, pe_prev = e4.w;
+            const f3 d = xyz(d4), meval = xyz(e4);
+            f3 weight = xyz(w4);
+            f3 rad = xyz(radiance[id]);
+            const uint32_t slot = __float_as_uint(h.y);
+            const bool hit = slot != INVALID;
+            Surf s;
+            xt_material mat;
+            bool emissive = false;
+            if (hit) {
+                s = make_surf(sc, slot, d, h.z, h.w);
+                mat = sc.mats[s.mat];
+                emissive = mat.type == XT_MAT_EMISSION;
+            }
+            // ---- radiance arriving along the ray that reached this vertex
+            if (v == 1 && standard) {
+                if (!hit) {
+                    if (sc.env.has_sky) rad += weight * env_radiance(sc.env, d);          // PathTracer.h:133-136
+                } else if (emissive) {
+                    rad += weight * emitted(mat, -d, s.n);                                // PathTracer.h:143-145
+                }
+            } else if (standard) {
+                if (!hit) {
+                    if (sc.env.has_sky) {                                                  // PathTracer.h:192-195
+                        f3 Le = env_radiance(sc.env, d);
+                        if (envn.on && pdf_prev >= 0.0f) {
+                            float mis = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe_prev * pe_prev);
+                            rad += weight * mis * meval * Le;
+                        } else {
+                            rad += weight * meval * Le;
+                        }
+                    }
+                } else if (emissive) {                                                     // PathTracer.h:171-189
+                    float mis;
+                    if (pdf_prev >= 0.0f && can_sample) {
+                        float pl = sc.light_pdf[slot] * p_area;
+                        float cosL = fabsf(dot(d, s.sn));
+                        float distSq = h.x * h.x;
+                        pl *= distSq / cosL;
+                        mis = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pl * pl);
+                    } else {
+                        mis = 1.0f;
+                    }
+                    if (mis > 0.0f) rad += weight * mis * meval * emitted(mat, -d, s.n);
+                }
+            } else {                                                                       // PathTracer.h:75-98
+                if (!hit) {
+                    if (sc.env.has_sky) rad += clamp_pos(weight * env_radiance(sc.env, d));
+                } else if (emissive) {
+                    rad += weight * emitted(mat, -d, s.n);
+                }
+            }
+            if (qflags & QF_PENDING) rad += xyz(pending[id]);  // light sample of the previous vertex
+            radiance[id] = f4(rad, 0.0f);
+
+            bool live = hit && !emissive;
+            Rng rng{key, 0u, 0u};
+            if (live) {
+                // path id -> (pixel, sample) as in k_generate
+                uint32_t s_local = id / wp.num_pixels;
+                rng.pixel = wp.first_pixel + (id - s_local * wp.num_pixels);
+                rng.sample = wp.sample0 + s_local;
+            }
+            if (live && standard) {
+                if (v > 1) {                                       // PathTracer.h:239-245
+                    weight = weight * meval;
+                    if (is_zero(weight)) live = false;
+                }
+                if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
+                uint4 blk0 = make_uint4(0, 0, 0, 0);
+                if (live) blk0 = rng.block(v, 0);
+                if (live && (int)(v + 1) > wp.rr_start) {          // PathTracer.h:149-155
+                    if (Rng::to_unit(blk0.x) >= wp.rr_prob) live = false;
+                    else weight = div(weight, wp.rr_prob);
+                }
+                if (live) {
+                    f3 wi = splat(0.0f);
+                    float pdfB = 0.0f;
+                    f3 me = bsdf_sample(mat, s, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi, &pdfB);
+                    if (!is_zero(me)) {                            // PathTracer.h:165-167
+                        push_ext = true;
+                        ext_d = wi;
+                        ext_o = s.p + wp.eps * wi;
+                        ext_w = weight;
+                        ext_e = me;
+                        ext_pdf = pdfB;
+                        ext_v = v + 1;
+                        ext_pe = (envn.on && pdfB >= 0.0f) ? envn.p_select * env_nee_pdf(sc.env, envn, normalize(wi), s.sn) : 0.0f;
+                    }
+                    // ---- light sample (PathTracer.h:202-233)
+                    uint4 blk1 = rng.block(v, 1);
+                    bool sample_env = false;
+                    if (envn.on) sample_env = !can_sample || Rng::to_unit(blk1.x) < envn.p_select;
+                    if (sample_env) {
+                        if (pdfB >= 0.0f) {
+                            uint4 blk2 = rng.block(v, 2), blk3 = rng.block(v, 3);
+                            f3 ld = env_nee_sample(sc.env, envn, s.sn, Rng::to_unit(blk1.w), Rng::to_unit(blk2.w),
+                                                   Rng::to_unit(blk3.x));
+                            float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
+                            f3 fc = bsdf_cos(mat, s, ld);
+                            if (pe > 0.0f && !is_zero(fc)) {
+                                float pb = bsdf_pdf(mat, s, ld);
+                                float mis = (pe * pe) / (pb * pb + pe * pe);
+                                f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
+                                push_sh = true;
+                                sh_d = ld;
+                                sh_o = s.p + wp.eps * ld;
+                                sh_tmax = INFINITY;
+                                sh_c = c;
+                            }
+                        }
+                    } else if (can_sample) {
+                        uint2 lo = sc.light_objs[blk1.y % sc.num_lights];
+                        uint32_t lslot = sc.light_slots[lo.x + blk1.z % lo.y];
+                        uint4 blk2 = rng.block(v, 2);
+                        float t0, t1;
+                        if (uniform_bary) {
+                            float su = sqrtf(Rng::to_unit(blk2.x));
+                            t0 = 1.0f - su;
+                            t1 = Rng::to_unit(blk2.y) * su;
+                        } else {                                   // TriangleIndexed.h:184-192
+                            float a = Rng::to_unit(blk2.x), b = Rng::to_unit(blk2.y), c = Rng::to_unit(blk2.z);
+                            float sum = a + b + c;
+                            if (sum == 0.0f) {
+                                a = b = c = 1.0f;
+                                sum = 3.0f;
+                            }
+                            float inv = 1.0f / sum;
+                            t0 = a * inv;
+                            t1 = b * inv;
+                        }
+                        const float4* lt = sc.tris + (size_t)lslot * TRI_F4;
+                        float4 l0 = lt[0], l1 = lt[1], l2 = lt[2];
+                        f3 q0 = xyz(l0), q1 = xyz(l1), q2 = xyz(l2);
+                        f3 lp = lerp3(q0, q1, q2, t0, t1);
+                        f3 ln = normalize(cross(q1 - q0, q2 - q0));
+                        f3 lsn = ln;
+                        if (__float_as_uint(l2.w) & TRI_FLAG_NORMALS) {
+                            const float4* nr = sc.normals + (size_t)lslot * TRI_F4;
+                            lsn = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), t0, t1));
+                        }
+                        ln = normalize(ln);
+                        lsn = normalize(lsn);
+                        float pdfL = sc.light_pdf[lslot] * p_area;
+                        float dist = length(lp - s.p);
+                        f3 sd = div(lp - s.p, dist);
+                        if (dot(sd, ln) < 0) {
+                            float distSq = dist * dist;
+                            float mis;
+                            if (pdfB >= 0.0f) {
+                                float pb = bsdf_pdf(mat, s, sd);
+                                float cosL = fabsf(dot(-sd, lsn));
+                                pb *= cosL / distSq;
+                                mis = (pdfL * pdfL) / (pb * pb + pdfL * pdfL);
+                            } else {
+                                mis = 0.0f;
+                            }
+                            if (mis > 0.0f) {
+                                float G = fabsf(dot(-sd, lsn)) / distSq;
+                                const xt_material& lm = sc.mats[__float_as_uint(l1.w)];
+                                f3 c = div(weight * mis * bsdf_cos(mat, s, sd) * emitted(lm, -sd, ln) * G, pdfL);
+                                push_sh = true;
+                                sh_d = sd;
+                                sh_o = s.p + wp.eps * sd;
+                                sh_tmax = dist - wp.shadow_margin;
+                                sh_c = c;
+                            }
+                        }
+                    }
+                }
+            } else if (live) {  // NaivePathTracer (PathTracer.h:86-100), then the roulette of the next loop head
+                uint4 blk0 = rng.block(v, 0);
+                f3 wi = splat(0.0f);
+                float pdfB = 0.0f;
+                weight = weight * bsdf_sample(mat, s, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi, &pdfB);
+                if (is_zero(weight)) live = false;
+                if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
+                if (live && (int)(v + 1) > wp.rr_start) {
+                    uint4 nb = rng.block(v + 1, 0);
+                    if (Rng::to_unit(nb.x) >= wp.rr_prob) live = false;
+                    else weight = div(weight, wp.rr_prob);
+                }
+                if (live) {
+                    push_ext = true;
+                    ext_d = wi;
+                    ext_o = s.p + wp.eps * wi;
+                    ext_w = weight;
+                    ext_e = splat(1.0f);
+                    ext_pdf = pdfB;
+                    ext_v = v + 1;
+                }
+            }
+        }
+        // ---- compacted appends (whole wave participates)
+        const uint32_t ei = wave_append(count_out, push_ext);
+        const uint32_t si = wave_append(count_shadow, push_sh);
+        if (push_ext) {
+            uint32_t fl = push_sh ? QF_PENDING : 0u;
+            qout.o[ei] = f4(ext_o, ext_pdf);
+            qout.d[ei] = f4(ext_d, __uint_as_float(id));
+            qout.w[ei] = f4(ext_w, __uint_as_float(ext_v | (fl << 24)));
+            qout.e[ei] = f4(ext_e, ext_pe);
+        }
+        if (push_sh) {
+            sq.o[si] = f4(sh_o, sh_tmax);
+            sq.d[si] = f4(sh_d, __uint_as_float(id));
+            sq.c[si] = f4(sh_c, push_ext ? 1.0f : 0.0f);
+        }
+    }
+}
+
+// Any-hit for queued shadow rays; unoccluded contributions go straight to the
+// path radiance, or to pending[] when the path continues (see k_shade).
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* __restrict__ count,
+                                                        float4* __restrict__ radiance, float4* __restrict__ pending,
+                                                        uint32_t* __restrict__ ovf,
+                                                        unsigned long long* __restrict__ visit_totals) {
+    __shared__ uint32_t lds[TRACE_LDS_DEPTH * TRACE_BLOCK];
+    const uint32_t n = *count;
+    const size_t nthreads = (size_t)gridDim.x * blockDim.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
+    VisitCount vc;
+    for (size_t i = gtid; i < n; i += nthreads) {
+        float4 o4 = sq.o[i], d4 = sq.d[i], c4 = sq.c[i];
+        RayPre r = ray_setup(xyz(o4), xyz(d4));
+        TraceResult res;
+        bool occ = traverse<true, TRACE_LDS_DEPTH, COUNT>(sc, r, o4.w, lds + threadIdx.x, TRACE_BLOCK, my_ovf, nthreads,
+                                                          &res, &vc);
+        uint32_t id = __float_as_uint(d4.w);
+        f3 c = occ ? splat(0.0f) : xyz(c4);
+        if (c4.w != 0.0f) {
+            pending[id] = f4(c, 0.0f);
+        } else if (!occ) {
+            radiance[id] = f4(xyz(radiance[id]) + c, 0.0f);
+        }
+    }
+    if (COUNT) flush_counts(visit_totals, vc);
+}
+template __global__ void k_shadow<false>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
+                                         unsigned long long*);
+template __global__ void k_shadow<true>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
+                                        unsigned long long*);
+
+// film[pixel] += radiance of each of the batch's samples, in sample order.
+__global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, const float4* __restrict__ radiance, void* film,
+                                                    int film_f64) {
+    const uint32_t np = wp.num_pixels, S = wp.num_paths / wp.num_pixels;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        size_t o = 3 * (size_t)(wp.first_pixel + p);
+        if (film_f64) {
+            double* f = (double*)film;
+            double r = f[o], g = f[o + 1], b = f[o + 2];
+            for (uint32_t s = 0; s < S; ++s) {
+                float4 L = radiance[(size_t)s * np + p];
+                r += (double)L.x;
+                g += (double)L.y;
+                b += (double)L.z;
+            }
+            f[o] = r;
+            f[o + 1] = g;
+            f[o + 2] = b;
+        } else {
+            float* f = (float*)film;
+            float r = f[o], g = f[o + 1], b = f[o + 2];
+            for (uint32_t s = 0; s < S; ++s) {
+                float4 L = radiance[(size_t)s * np + p];
+                r += L.x;
+                g += L.y;
+                b += L.z;
+            }
+            f[o] = r;
+            f[o + 1] = g;
+            f[o + 2] = b;
+        }
+    }
+}
+
+}  // namespace xt

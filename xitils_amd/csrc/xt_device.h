@@ -1,0 +1,328 @@
+// Device-resident scene layout, counter-based RNG, BSDFs and light sampling
+// shared by the gfx950 kernels (and host code that prepares the tables).
+#pragma once
+
+#include "xt_math.h"
+#include "../../include/xitils_amd.h"
+
+namespace xt {
+
+// ------------------------------------------------------------------ HBM layout
+//
+// BVH4 node: 128 bytes (one L2 line on gfx950), 8 float4:
+//   [0] lo.x[4]  [1] hi.x[4]  [2] lo.y[4]  [3] hi.y[4]  [4] lo.z[4]  [5] hi.z[4]
+//   [6] child[4] as int: >= 0 internal node index; < 0 leaf ~(first << 3 | (count - 1))
+//   [7] reserved
+// Empty child slots have lo = +inf, hi = -inf and never pass the slab test.
+//
+// Triangles, in leaf order ("slots"), 3 float4 = 48 bytes each:
+//   [0] p0.xyz, global prim id   [1] p1.xyz, material index   [2] p2.xyz, flags
+// Optional vertex normals per slot, 3 float4 (n0, n1, n2).
+constexpr int NODE_F4 = 8;
+constexpr int TRI_F4 = 3;
+constexpr uint32_t TRI_FLAG_NORMALS = 1u;
+constexpr uint32_t TRI_FLAG_EMISSIVE = 2u;
+constexpr uint32_t INVALID = 0xFFFFFFFFu;
+
+XT_HD int leaf_code(uint32_t first, uint32_t count) { return ~(int)((first << 3) | (count - 1)); }
+
+struct DevEnv {
+    int has_sky;
+    f3 sky;
+    f3 sun_dir;
+    float sun_omc;   // 1 - cos(half angle)
+    float sun_sin2;  // sin^2(half angle)
+    f3 sun_L;
+};
+
+struct DevScene {
+    const float4* nodes;
+    const float4* tris;
+    const float4* normals;      // nullptr when no object has vertex normals
+    const xt_material* mats;
+    const float* light_pdf;     // per slot; area pdf of emissive slots (Scene.h:49-55)
+    const uint2* light_objs;    // {first, count} into light_slots, one per emissive object
+    const uint32_t* light_slots;
+    uint32_t num_lights;
+    uint32_t num_prims;
+    DevEnv env;
+};
+
+// ------------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Counter = (pixel, sample, vertex slot, block), key =
+// (seed, stream); block b yields dims 4b..4b+3.
+XT_HD uint4 philox(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Dimension map (per vertex slot; slot 0 = camera, slot v = v-th surface vertex)
+enum : int {
+    D_RR = 0, D_BSDF_SEL = 1, D_BSDF_U1 = 2, D_BSDF_U2 = 3,
+    D_LIGHT_KIND = 4, D_LIGHT_OBJ = 5, D_LIGHT_TRI = 6, D_ENV_SEL = 7,
+    D_BARY0 = 8, D_BARY1 = 9, D_BARY2 = 10, D_ENV_U1 = 11, D_ENV_U2 = 12,
+    D_FILM_X = 0, D_FILM_Y = 1
+};
+
+struct Rng {
+    uint2 key;
+    uint32_t pixel, sample;
+    XT_HD uint4 block(uint32_t slot, int b) const { return philox(make_uint4(pixel, sample, slot, (uint32_t)b), key); }
+    XT_HD static uint32_t pick(uint4 v, int w) { return w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w)); }
+    XT_HD static float to_unit(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+};
+
+// ------------------------------------------------------------------ surface
+struct Surf {
+    f3 p, n, wo, sn;   // position, normal, outgoing dir, shading normal (world)
+    uint32_t mat;
+    uint32_t slot;
+};
+
+// ------------------------------------------------------------------ BSDFs
+// Metal / GGX pieces (Material.h:288-378).  pow(float, int) promotes to
+// double in the reference (Material.h:297, 301, 312): those steps are double.
+XT_HD float ggx_D(float alpha, f3 x, f3 n) {
+    float alpha2 = alpha * alpha;
+    float dn = dot(n, x);
+    double dd = (double)dn;
+    float c2 = (float)(dd * dd);
+    float c4 = c2 * c2;
+    float tan2 = 1 / c2 - 1;
+    double s = (double)(alpha2 + tan2);
+    float h = dn > 0 ? 1.0f : 0.0f;
+    return (float)((double)(alpha2 * h) / (3.14159265358979323846 * (double)c4 * (s * s)));
+}
+XT_HD float ggx_G1(float alpha, f3 x, f3 n) {
+    float alpha2 = alpha * alpha;
+    double dd = (double)dot(n, x);
+    float c2 = (float)(dd * dd);
+    float tan2 = 1 / c2 - 1;
+    float lambda = (-1 + safe_sqrt(1 + alpha2 * tan2)) / 2;
+    return 1 / (1 + lambda);
+}
+XT_HD float ggx_DV(float alpha, f3 h, f3 n, f3 eye) {
+    return ggx_G1(alpha, eye, n) * clamp_pos(dot(eye, h)) * ggx_D(alpha, h, n) / dot(eye, n);
+}
+XT_HD float ggx_pdf(float alpha, f3 wo, f3 n, f3 wi) {
+    f3 h = normalize(wi + wo);
+    return ggx_DV(alpha, h, n, wo) / (4 * dot(wo, h));
+}
+// Visible-normal sampling, Heitz 2018 (Material.h:323-345).
+XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, f3 n, float U1, float U2) {
+    const float PI = 3.14159265358979323846f;
+    Frame fr = make_frame(n);
+    f3 Ve = to_local(fr, eye);
+    f3 Vh = normalize(mk(Ve.x, alpha * Ve.y, alpha * Ve.z));
+    float lensq = Vh.y * Vh.y + Vh.z * Vh.z;
+    f3 T1 = lensq > 0 ? div(mk(0, -Vh.z, Vh.y), safe_sqrt(lensq)) : mk(0, 0, 1);
+    f3 T2 = cross(Vh, T1);
+    float r = sqrtf(U1);
+    float phi = 2.0f * PI * U2;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float s = 0.5f * (1.0f + Vh.x);
+    t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+    f3 Nh = t1 * T1 + t2 * T2 + safe_sqrt(1.0f - t1 * t1 - t2 * t2) * Vh;
+    f3 Ne = normalize(mk(clamp_pos(Nh.x), alpha * Nh.y, alpha * Nh.z));
+    return to_world(fr, Ne.x, Ne.y, Ne.z);
+}
+XT_HD f3 schlick(f3 f0, float c) { return f0 + (splat(1.0f) - f0) * powf(1 - c, 5.0f); }
+
+// cosine-weighted hemisphere about v (Vector.h:817-824)
+XT_HD f3 sample_cos_hemisphere(f3 v, float u, float uphi) {
+    const float PI = 3.14159265358979323846f;
+    Frame fr = make_frame(v);
+    float r = sqrtf(u);
+    float phi = uphi * (2.0f * PI);
+    return to_world(fr, safe_sqrt(1.0f - u), r * cosf(phi), r * sinf(phi));
+}
+
+XT_HD float cos_pdf(float c) { return (float)((double)c / 3.14159265358979323846); }
+
+// Material::bsdfCos = f * cos(theta_i) (Material.h:17).
+XT_HD f3 bsdf_cos(const xt_material& m, const Surf& s, f3 wi) {
+    const float PI = 3.14159265358979323846f;
+    switch (m.type) {
+    case XT_MAT_DIFFUSE:  // Material.h:59-65
+        return div(mk(m.p[0], m.p[1], m.p[2]), PI) * clamp_pos(dot(s.sn, wi));
+    case XT_MAT_METAL: {  // Material.h:347-356
+        float a = m.p[0];
+        f3 h = normalize(wi + s.wo);
+        f3 Fr = schlick(mk(m.p[1], m.p[2], m.p[3]), dot(s.wo, h));
+        float D = ggx_D(a, h, s.sn);
+        float G = ggx_G1(a, wi, s.sn) * ggx_G1(a, s.wo, s.sn);
+        return div(Fr * D * G, clamp_pos(4 * dot(s.wo, s.sn)));
+    }
+    case XT_MAT_GGX_LAMBERT: {
+        float cosI = dot(s.sn, wi);
+        if (!(cosI > 0.0f)) return splat(0.0f);
+        float a = m.p[3];
+        f3 h = normalize(wi + s.wo);
+        f3 Fr = schlick(splat(m.p[4]), dot(s.wo, h));
+        float D = ggx_D(a, h, s.sn);
+        float G = ggx_G1(a, wi, s.sn) * ggx_G1(a, s.wo, s.sn);
+        f3 spec = div(Fr * D * G, clamp_pos(4 * dot(s.wo, s.sn)));
+        f3 diff = mk(m.p[0], m.p[1], m.p[2]) * ((1.0f - m.p[4]) / PI) * cosI;
+        return spec + diff;
+    }
+    default:
+        return splat(0.0f);
+    }
+}
+
+// Material::getPDF (Material.h:31), solid angle.
+XT_HD float bsdf_pdf(const xt_material& m, const Surf& s, f3 wi) {
+    switch (m.type) {
+    case XT_MAT_DIFFUSE:  // Material.h:78-81
+        return cos_pdf(clamp_pos(dot(wi, s.sn)));
+    case XT_MAT_METAL:
+        return ggx_pdf(m.p[0], s.wo, s.sn, wi);
+    case XT_MAT_GGX_LAMBERT: {
+        float cosI = dot(s.sn, wi);
+        if (!(cosI > 0.0f)) return 0.0f;
+        float ps = m.p[5];
+        return ps * ggx_pdf(m.p[3], s.wo, s.sn, wi) + (1.0f - ps) * cos_pdf(cosI);
+    }
+    default:
+        return 0.0f;
+    }
+}
+
+XT_HD float fresnel_dielectric(f3 wo, f3 n, float eta_o, float eta_i) {  // Material.h:265-271
+    const float c = fabsf(dot(wo, n));
+    float r = eta_o / eta_i;
+    float g_sq = r * r - 1.0f + c * c;
+    if (g_sq <= 0.0f) return 1.0f;
+    float g = sqrtf(g_sq);
+    float a = (g - c) / (g + c);
+    float b = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+    return 0.5f * (a * a) * (1.0f + b * b);
+}
+
+// Material::evalAndSample (Material.h:25): f*cos/pdf, wi, pdf (-1 = delta).
+XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, float u2, f3* wi, float* pdf) {
+    switch (m.type) {
+    case XT_MAT_DIFFUSE: {  // Material.h:67-76
+        *wi = sample_cos_hemisphere(s.sn, u1, u2);
+        *pdf = cos_pdf(dot(*wi, s.sn));
+        return mk(m.p[0], m.p[1], m.p[2]);
+    }
+    case XT_MAT_SPEC_REFL: {  // Material.h:180-187
+        *wi = -normalize(s.wo - 2 * dot(s.sn, s.wo) * s.sn);
+        *pdf = -1.0f;
+        return splat(1.0f);
+    }
+    case XT_MAT_SPEC_REFR:
+    case XT_MAT_SPEC_FRESNEL: {  // Material.h:196-250
+        float index = m.p[0];
+        float eta_i, eta_o;
+        if (dot(s.n, s.wo) > 0.0f) {
+            eta_o = 1.0f;
+            eta_i = index;
+        } else {
+            eta_o = index;
+            eta_i = 1.0f;
+        }
+        if (m.type == XT_MAT_SPEC_FRESNEL && fresnel_dielectric(s.wo, s.sn, eta_o, eta_i) >= usel) {
+            *wi = -normalize(s.wo - 2 * dot(s.sn, s.wo) * s.sn);
+        } else {
+            float ri = eta_o / eta_i;
+            float c = dot(-s.wo, s.sn);
+            *wi = ri * (-s.wo - c * s.sn) - safe_sqrt(1 - ri * ri * (1 - c * c)) * s.sn;
+        }
+        *pdf = -1.0f;
+        return splat(1.0f);
+    }
+    case XT_MAT_METAL: {  // Material.h:358-370
+        float a = m.p[0];
+        f3 h = ggx_sample_vndf(a, s.wo, s.sn, u1, u2);
+        *wi = 2 * dot(s.wo, h) * h - s.wo;
+        *pdf = ggx_pdf(a, s.wo, s.sn, *wi);
+        f3 r = div(bsdf_cos(m, s, *wi), *pdf);
+        if (!(*pdf > 0.0f) || !finite3(r)) return splat(0.0f);
+        return r;
+    }
+    case XT_MAT_GGX_LAMBERT: {
+        float a = m.p[3];
+        if (usel < m.p[5]) {
+            f3 h = ggx_sample_vndf(a, s.wo, s.sn, u1, u2);
+            *wi = 2 * dot(s.wo, h) * h - s.wo;
+        } else {
+            *wi = sample_cos_hemisphere(s.sn, u1, u2);
+        }
+        *pdf = bsdf_pdf(m, s, *wi);
+        if (!(*pdf > 0.0f)) return splat(0.0f);
+        f3 r = div(bsdf_cos(m, s, *wi), *pdf);
+        if (!finite3(r)) return splat(0.0f);
+        return r;
+    }
+    default:  // Emission (Material.h:400-402)
+        *pdf = 0.0f;
+        return splat(0.0f);
+    }
+}
+
+XT_HD bool mat_is_delta(uint32_t t) { return t == XT_MAT_SPEC_REFL || t == XT_MAT_SPEC_REFR || t == XT_MAT_SPEC_FRESNEL; }
+
+// Emission::getEmission, one-sided on n (Material.h:408-410).
+XT_HD f3 emitted(const xt_material& m, f3 wo, f3 n) { return dot(wo, n) > 0.0f ? mk(m.p[0], m.p[1], m.p[2]) : splat(0.0f); }
+
+// ------------------------------------------------------------------ environment
+XT_HD bool env_has_sun(const DevEnv& e) { return !is_zero(e.sun_L) && e.sun_omc > 0.0f; }
+XT_HD bool env_in_sun(const DevEnv& e, f3 du) {
+    f3 c = cross(du, e.sun_dir);
+    return dot(du, e.sun_dir) > 0.0f && dot(c, c) <= e.sun_sin2;
+}
+// SkySphereUniform::getRadiance (SkySphere.h:40-42) plus the sun disk.
+XT_HD f3 env_radiance(const DevEnv& e, f3 d) {
+    f3 L = e.sky;
+    if (env_has_sun(e) && env_in_sun(e, normalize(d))) L += e.sun_L;
+    return L;
+}
+
+// Environment light strategy: pick env vs area lights, then sun cone vs
+// cosine sky.
+struct EnvNee {
+    bool on;
+    float p_select, p_sun;
+};
+XT_HD EnvNee env_nee_setup(const DevEnv& e, bool enabled, bool have_area) {
+    EnvNee n;
+    bool sky = !is_zero(e.sky);
+    n.on = enabled && e.has_sky && (env_has_sun(e) || sky);
+    n.p_select = n.on ? (have_area ? 0.5f : 1.0f) : 0.0f;
+    n.p_sun = env_has_sun(e) ? (sky ? 0.5f : 1.0f) : 0.0f;
+    return n;
+}
+XT_HD float env_nee_pdf(const DevEnv& e, const EnvNee& s, f3 du, f3 n) {
+    const float PI = 3.14159265358979323846f;
+    float p = 0.0f;
+    if (s.p_sun > 0.0f && env_in_sun(e, du)) p += s.p_sun * (1.0f / (2.0f * PI * e.sun_omc));
+    if (s.p_sun < 1.0f) p += (1.0f - s.p_sun) * cos_pdf(clamp_pos(dot(du, n)));
+    return p;
+}
+XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, f3 n, float usel, float u1, float u2) {
+    const float PI = 3.14159265358979323846f;
+    if (usel < s.p_sun) {
+        float uo = u1 * e.sun_omc;
+        float cosT = 1.0f - uo;
+        float sinT = safe_sqrt(uo * (2.0f - uo));
+        float phi = 2.0f * PI * u2;
+        Frame fr = make_frame(e.sun_dir);
+        return to_world(fr, cosT, sinT * cosf(phi), sinT * sinf(phi));
+    }
+    return sample_cos_hemisphere(n, u1, u2);
+}
+
+}  // namespace xt

@@ -1,0 +1,393 @@
+"""Synthetic scenes for the path-tracing workloads (pure numpy, seeded).
+
+Every scene is returned as a :class:`SceneDesc`: a list of objects in the
+reference's sense (one mesh, one transform, one material per object, as
+``Object(shape, material, objectToWorld)`` in Object.h:16-40, with an optional
+per-triangle material override), a material table, an environment and a
+pinhole camera.  The same description feeds the HIP library and the CPU
+checker, so both see bit-identical inputs.
+
+Material parameter layout (7 floats, shared with include/xitils_amd.h):
+  DIFFUSE       albedo r, g, b                          (Material.h:50-86)
+  SPEC_REFL     -                                       (Material.h:177-189)
+  SPEC_REFR     index                                   (Material.h:191-218)
+  SPEC_FRESNEL  index                                   (Material.h:220-273)
+  METAL         alpha (= roughness^2), f0 r, g, b       (Material.h:275-384)
+  EMISSION      power r, g, b                           (Material.h:386-411)
+  GGX_LAMBERT   albedo r, g, b, alpha, f0, p_spec       (new; lobe mixture as
+                                                          GlossyWithHighLight,
+                                                          Material.h:134-175)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+MAT_DIFFUSE, MAT_SPEC_REFL, MAT_SPEC_REFR, MAT_SPEC_FRESNEL, MAT_METAL, MAT_EMISSION, MAT_GGX_LAMBERT = range(7)
+
+FLAG_UNIFORM_BARY = 1
+FLAG_ENV_NEE = 2
+
+INTEGRATOR_STANDARD = 0
+INTEGRATOR_NAIVE = 1
+
+
+@dataclass
+class Material:
+    type: int
+    params: tuple = (0.0,) * 7
+
+    def p7(self) -> np.ndarray:
+        p = np.zeros(7, np.float32)
+        p[: len(self.params)] = self.params
+        return p
+
+
+def diffuse(r, g=None, b=None):
+    g = r if g is None else g
+    b = r if b is None else b
+    return Material(MAT_DIFFUSE, (r, g, b))
+
+
+def emission(r, g, b):
+    return Material(MAT_EMISSION, (r, g, b))
+
+
+def metal(roughness, f0):
+    # Metal(roughness, f0): alpha = roughness * roughness (Material.h:282-286)
+    a = np.float32(roughness) * np.float32(roughness)
+    return Material(MAT_METAL, (float(a), f0[0], f0[1], f0[2]))
+
+
+def ggx_lambert(albedo, alpha, f0, p_spec=0.5):
+    return Material(MAT_GGX_LAMBERT, (albedo[0], albedo[1], albedo[2], alpha, f0, p_spec))
+
+
+@dataclass
+class ObjectDesc:
+    positions: np.ndarray                 # (nv, 3) float32, object space
+    indices: np.ndarray                   # (nt, 3) int32
+    material: int = 0
+    normals: Optional[np.ndarray] = None  # (nv, 3) float32
+    xf: Optional[np.ndarray] = None       # (4, 4) float32 Transform::mat
+    xf_inv: Optional[np.ndarray] = None   # (4, 4) float32 Transform::matInv
+    tri_materials: Optional[np.ndarray] = None  # (nt,) uint32
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.indices.shape[0])
+
+
+@dataclass
+class Env:
+    has_sky: bool = False
+    sky: tuple = (0.0, 0.0, 0.0)
+    sun_dir: tuple = (0.0, 1.0, 0.0)
+    sun_half_angle_deg: float = 0.0
+    sun_radiance: tuple = (0.0, 0.0, 0.0)
+
+    def sun_terms(self):
+        """(omc, sin2) of the disk half angle, in float64 then rounded."""
+        th = math.radians(self.sun_half_angle_deg)
+        omc = 2.0 * math.sin(th / 2.0) ** 2  # 1 - cos, without cancellation
+        sin2 = math.sin(th) ** 2
+        return np.float32(omc), np.float32(sin2)
+
+
+@dataclass
+class Camera:
+    cam_to_world: np.ndarray   # (4, 4) float32
+    fov: float                 # radians
+    aspect: float              # height / width, as PinholeCamera (Camera.h:32-36)
+
+    def tans(self):
+        # Camera.h:49-50: tanf(fov/2) and tanf(fov/2 * aspect) (quirk Q9)
+        f = np.float32(self.fov)
+        a = np.float32(self.aspect)
+        tx = np.tan(np.float32(f / np.float32(2)), dtype=np.float32)
+        ty = np.tan(np.float32(np.float32(f / np.float32(2)) * a), dtype=np.float32)
+        return np.float32(tx), np.float32(ty)
+
+    def packed(self) -> np.ndarray:
+        tx, ty = self.tans()
+        return np.concatenate([self.cam_to_world.astype(np.float32).reshape(16), np.array([tx, ty], np.float32)])
+
+
+@dataclass
+class SceneDesc:
+    objects: List[ObjectDesc] = field(default_factory=list)
+    materials: List[Material] = field(default_factory=list)
+    env: Env = field(default_factory=Env)
+    camera: Optional[Camera] = None
+    name: str = ""
+
+    @property
+    def num_triangles(self) -> int:
+        return sum(o.num_triangles for o in self.objects)
+
+    def add_material(self, m: Material) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+
+# --------------------------------------------------------------------------- helpers
+
+def translate(x, y, z):
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = (x, y, z)
+    mi = np.eye(4, dtype=np.float32)
+    mi[:3, 3] = (-np.float32(x), -np.float32(y), -np.float32(z))
+    return m, mi
+
+
+def look_at(pos, target, up=(0.0, 1.0, 0.0)):
+    """Camera-to-world with columns (right, up, forward), translation pos,
+    as Transform.h lookAt (camera space looks down +z)."""
+    pos = np.asarray(pos, np.float64)
+    d = np.asarray(target, np.float64) - pos
+    d /= np.linalg.norm(d)
+    right = np.cross(np.asarray(up, np.float64), d)
+    right /= np.linalg.norm(right)
+    nup = np.cross(d, right)
+    m = np.eye(4, dtype=np.float64)
+    m[:3, 0] = right
+    m[:3, 1] = nup
+    m[:3, 2] = d
+    m[:3, 3] = pos
+    return m.astype(np.float32)
+
+
+def box_mesh(size, center, rot_y_deg):
+    """Axis box of extent size=(sx, sy, sz) resting on its center, rotated
+    about +y, emitted directly in world space (12 triangles, outward faces)."""
+    sx, sy, sz = (np.float64(v) / 2 for v in size)
+    c = np.array([[-sx, -sy, -sz], [sx, -sy, -sz], [sx, sy, -sz], [-sx, sy, -sz],
+                  [-sx, -sy, sz], [sx, -sy, sz], [sx, sy, sz], [-sx, sy, sz]], np.float64)
+    t = math.radians(rot_y_deg)
+    rot = np.array([[math.cos(t), 0, math.sin(t)], [0, 1, 0], [-math.sin(t), 0, math.cos(t)]])
+    v = c @ rot.T + np.asarray(center, np.float64)
+    # faces (outward normals via cross(p1-p0, p2-p0))
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2)]
+    idx = []
+    for a, b, cc, d in quads:
+        idx += [(a, b, cc), (a, cc, d)]
+    return v.astype(np.float32), np.array(idx, np.int32)
+
+
+def quad(p0, p1, p2, p3):
+    v = np.array([p0, p1, p2, p3], np.float32)
+    return v, np.array([(0, 1, 2), (0, 2, 3)], np.int32)
+
+
+# --------------------------------------------------------------------------- Cornell box
+
+def cornell(width=256, height=256):
+    """Unit-cube room, 5 walls as 10 triangles (left red, right green, others
+    0.73 grey), a 0.3 short box rotated -18 deg and a 0.3x0.6x0.3 tall box
+    rotated +18 deg, and a 0.25 x 0.2 ceiling quad light of radiance
+    (17, 12, 4) facing down.  Pinhole camera at (0.5, 0.5, -1.35) looking +z,
+    fov 40 deg."""
+    s = SceneDesc(name="cornell")
+    white = s.add_material(diffuse(0.73))
+    red = s.add_material(diffuse(0.63, 0.065, 0.05))
+    green = s.add_material(diffuse(0.14, 0.45, 0.09))
+    light = s.add_material(emission(17.0, 12.0, 4.0))
+    # normals point into the room
+    walls = [
+        (quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)), white),   # floor y=0, normal +y
+        (quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), white),   # ceiling y=1, normal -y
+        (quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)), white),   # back z=1, normal -z
+        (quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), red),     # left x=0, normal +x
+        (quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), green),   # right x=1, normal -x
+    ]
+    wv = np.concatenate([w[0][0] for w in walls])
+    wi = np.concatenate([w[0][1] + 4 * k for k, w in enumerate(walls)])
+    wm = np.concatenate([[w[1], w[1]] for w in walls]).astype(np.uint32)
+    s.objects.append(ObjectDesc(wv, wi, material=white, tri_materials=wm))
+    bv, bi = box_mesh((0.3, 0.3, 0.3), (0.33, 0.15, 0.35), -18.0)
+    s.objects.append(ObjectDesc(bv, bi, material=white))
+    tv, ti = box_mesh((0.3, 0.6, 0.3), (0.66, 0.3, 0.66), 18.0)
+    s.objects.append(ObjectDesc(tv, ti, material=white))
+    lx0, lx1, lz0, lz1, ly = 0.375, 0.625, 0.4, 0.6, 0.999
+    lv, li = quad((lx0, ly, lz0), (lx1, ly, lz0), (lx1, ly, lz1), (lx0, ly, lz1))  # normal -y
+    s.objects.append(ObjectDesc(lv, li, material=light))
+    m, _ = translate(0.5, 0.5, -1.35)
+    s.camera = Camera(m, math.radians(40.0), height / width)
+    return s
+
+
+# --------------------------------------------------------------------------- heightfield
+
+def _hash2(ix, iz, seed):
+    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iz.astype(np.uint64) * np.uint64(0xD8163841)
+         ^ np.uint64(seed) * np.uint64(0xCB1AB31F)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(13)
+    h = (h * np.uint64(0x5BD1E995)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    return (h & np.uint64(0xFFFFFF)).astype(np.float64) / float(0xFFFFFF) * 2.0 - 1.0
+
+
+def value_noise(x, z, seed):
+    ix, iz = np.floor(x), np.floor(z)
+    fx, fz = x - ix, z - iz
+    ux = fx * fx * (3 - 2 * fx)
+    uz = fz * fz * (3 - 2 * fz)
+    ix = ix.astype(np.int64)
+    iz = iz.astype(np.int64)
+    a = _hash2(ix, iz, seed)
+    b = _hash2(ix + 1, iz, seed)
+    c = _hash2(ix, iz + 1, seed)
+    d = _hash2(ix + 1, iz + 1, seed)
+    return (a * (1 - ux) + b * ux) * (1 - uz) + (c * (1 - ux) + d * ux) * uz
+
+
+def fbm(x, z, seed, octaves=6, base=4.0, lacunarity=2.0, gain=0.5):
+    out = np.zeros_like(x)
+    amp, freq = 0.5, base
+    for o in range(octaves):
+        out += amp * value_noise(x * freq, z * freq, seed + 7919 * o)
+        amp *= gain
+        freq *= lacunarity
+    return out
+
+
+def heightfield(cells=1024, width=1920, height=1080, seed=2, amplitude=0.1, materials=True):
+    """cells x cells grid over [0,1]^2 (2 * cells^2 triangles), y =
+    amplitude * fBm_6(x, z); per-triangle GGX-over-Lambert with alpha ~
+    U[0.05, 0.5], F0 ~ U[0.02, 0.08], albedo ~ U[0.2, 0.8]; constant sky 1.0 plus
+    a 0.5 deg sun of radiance 5e4 at elevation 30 deg, azimuth 45 deg."""
+    s = SceneDesc(name=f"heightfield{cells}")
+    n = cells
+    g = np.linspace(0.0, 1.0, n + 1, dtype=np.float64)
+    X, Z = np.meshgrid(g, g, indexing="ij")          # X[i, j] = i/n, Z[i, j] = j/n
+    Y = amplitude * fbm(X, Z, seed)
+    pos = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    v00 = (i * (n + 1) + j).reshape(-1)
+    v01 = (i * (n + 1) + j + 1).reshape(-1)
+    v10 = ((i + 1) * (n + 1) + j).reshape(-1)
+    v11 = ((i + 1) * (n + 1) + j + 1).reshape(-1)
+    t0 = np.stack([v00, v01, v11], axis=1)
+    t1 = np.stack([v00, v11, v10], axis=1)
+    idx = np.stack([t0, t1], axis=1).reshape(-1, 3).astype(np.int32)
+    nt = idx.shape[0]
+    if materials:
+        rng = np.random.default_rng(seed)
+        alb = rng.uniform(0.2, 0.8, size=(nt, 3)).astype(np.float32)
+        alpha = rng.uniform(0.05, 0.5, size=nt).astype(np.float32)
+        f0 = rng.uniform(0.02, 0.08, size=nt).astype(np.float32)
+        s.materials = _ggx_lambert_table(alb, alpha, f0)
+        tri_m = np.arange(nt, dtype=np.uint32)
+    else:
+        s.add_material(diffuse(0.5))
+        tri_m = None
+    s.objects.append(ObjectDesc(pos, idx, material=0, tri_materials=tri_m))
+    el, az = math.radians(30.0), math.radians(45.0)
+    sun = (math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az))
+    s.env = Env(True, (1.0, 1.0, 1.0), sun, 0.25, (5e4, 5e4, 5e4))
+    cam = look_at((0.5, 0.42, -0.28), (0.5, 0.0, 0.55))
+    s.camera = Camera(cam, math.radians(50.0), height / width)
+    return s
+
+
+class MaterialTable(list):
+    """Material list backed by a packed (n, 8) array, so a per-triangle table
+    of millions of materials is not built as Python objects."""
+
+    def __init__(self, packed: np.ndarray):
+        super().__init__()
+        self.packed = packed
+
+    def __len__(self):
+        return self.packed.shape[0]
+
+    def __getitem__(self, i):
+        row = self.packed[i]
+        return Material(int(row[0].view(np.uint32)) if row.dtype == np.float32 else int(row[0]), tuple(row[1:8]))
+
+
+def _ggx_lambert_table(alb, alpha, f0, p_spec=0.5):
+    nt = alb.shape[0]
+    packed = np.zeros((nt, 8), np.float32)
+    packed[:, 0] = np.array([MAT_GGX_LAMBERT], np.uint32).view(np.float32)[0]
+    packed[:, 1:4] = alb
+    packed[:, 4] = alpha
+    packed[:, 5] = f0
+    packed[:, 6] = p_spec
+    return MaterialTable(packed)
+
+
+def pack_materials(mats) -> np.ndarray:
+    """(n, 8) float32 records {type bits, p[7]} as xt_material."""
+    if isinstance(mats, MaterialTable):
+        return np.ascontiguousarray(mats.packed)
+    out = np.zeros((len(mats), 8), np.float32)
+    for k, m in enumerate(mats):
+        out[k, 0] = np.array([m.type], np.uint32).view(np.float32)[0]
+        out[k, 1:] = m.p7()
+    return out
+
+
+# --------------------------------------------------------------------------- small test scenes
+
+def uv_sphere(center, radius, nu=24, nv=12, smooth=True):
+    th = np.linspace(0, math.pi, nv + 1)
+    ph = np.linspace(0, 2 * math.pi, nu + 1)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    nrm = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+    pos = np.asarray(center) + radius * nrm
+    idx = []
+    for a in range(nv):
+        for b in range(nu):
+            v0 = a * (nu + 1) + b
+            v1 = v0 + 1
+            v2 = v0 + (nu + 1)
+            v3 = v2 + 1
+            if a != 0:
+                idx.append((v0, v1, v2))
+            if a != nv - 1:
+                idx.append((v1, v3, v2))
+    return pos.astype(np.float32), np.array(idx, np.int32), (nrm.astype(np.float32) if smooth else None)
+
+
+def material_zoo(width=64, height=64):
+    """Cornell room whose contents cover every material kind: a smooth-normal
+    sphere of GGX-over-Lambert, a metal box, a mirror quad, a glass sphere and
+    a Fresnel sphere; sky off (closed room)."""
+    s = cornell(width, height)
+    s.name = "material_zoo"
+    s.objects = s.objects[:1] + s.objects[3:]   # walls + light
+    ggx = s.add_material(ggx_lambert((0.6, 0.5, 0.3), 0.2, 0.04))
+    met = s.add_material(metal(0.4, (0.9, 0.6, 0.3)))
+    mir = s.add_material(Material(MAT_SPEC_REFL))
+    glass = s.add_material(Material(MAT_SPEC_REFR, (1.5,)))
+    fres = s.add_material(Material(MAT_SPEC_FRESNEL, (1.5,)))
+    p, i, nrm = uv_sphere((0.3, 0.18, 0.4), 0.17)
+    s.objects.append(ObjectDesc(p, i, material=ggx, normals=nrm))
+    bv, bi = box_mesh((0.25, 0.45, 0.25), (0.7, 0.225, 0.7), 25.0)
+    s.objects.append(ObjectDesc(bv, bi, material=met))
+    qv, qi = quad((0.05, 0.3, 0.95), (0.05, 0.8, 0.95), (0.45, 0.8, 0.95), (0.45, 0.3, 0.95))
+    s.objects.append(ObjectDesc(qv, qi, material=mir))
+    p, i, nrm = uv_sphere((0.72, 0.62, 0.35), 0.1, 16, 8)
+    s.objects.append(ObjectDesc(p, i, material=glass, normals=nrm))
+    p, i, nrm = uv_sphere((0.25, 0.55, 0.3), 0.08, 16, 8)
+    s.objects.append(ObjectDesc(p, i, material=fres, normals=nrm))
+    return s
+
+
+def furnace_object(albedo=0.6, width=32, height=32):
+    """A convex mesh (box) of Lambert albedo a under a uniform sky of 1 with
+    nothing else: every camera ray that hits it returns exactly a (one bounce
+    escapes with weight a), every miss returns 1."""
+    s = SceneDesc(name="furnace")
+    m = s.add_material(diffuse(albedo))
+    bv, bi = box_mesh((0.6, 0.6, 0.6), (0.0, 0.0, 0.0), 30.0)
+    s.objects.append(ObjectDesc(bv, bi, material=m))
+    s.env = Env(True, (1.0, 1.0, 1.0))
+    cam = look_at((0.9, 0.7, -1.6), (0.0, 0.0, 0.0))
+    s.camera = Camera(cam, math.radians(45.0), height / width)
+    return s
