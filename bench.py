@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain 1 GiB streaming kernel (out = in * 1) to calibrate rocprof byte counters")
     return ap.parse_args()
 
 
@@ -119,6 +121,13 @@ def main():
     info = scene.info()
     log(f"[bench] scene {info}")
     film = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB read + 1 GiB written
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy done (1 GiB in, 1 GiB out)")
     kw = dict(seed=1, max_depth=max_depth, flags=flags)
 
     # fetch counts of the identical work (same seed, same rays): one untimed counting render
