@@ -14,7 +14,7 @@ import numpy as np
 from . import scenes as _sc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libxitils_amd.so")
+LIB_PATH = os.environ.get("XITILS_AMD_LIB") or os.path.join(HERE, "_lib", "libxitils_amd.so")
 
 XT_OK = 0
 STATUS_NAMES = {0: "XT_OK", 1: "XT_ERR_INVALID_ARGUMENT", 2: "XT_ERR_UNSUPPORTED_MATERIAL", 3: "XT_ERR_OUT_OF_MEMORY",

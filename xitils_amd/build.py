@@ -34,12 +34,13 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose: bool = False, out_lib: str = LIB, defines=()) -> str:
+    out_dir = os.path.dirname(out_lib)
+    os.makedirs(out_dir, exist_ok=True)
     objs = []
     for src in SOURCES:
-        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, "--offload-arch=gfx950", *COMMON, "-I", os.path.join(ROOT, "include"), "-c",
+        obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, "--offload-arch=gfx950", *COMMON, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC, *COMMON, "-x", "c++", "-I", os.path.join(ROOT, "include"), "-c",
@@ -48,12 +49,12 @@ def build(verbose: bool = False) -> str:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out_lib + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out_lib)
     for o in objs:
         os.remove(o)
-    return LIB
+    return out_lib
 
 
 if __name__ == "__main__":

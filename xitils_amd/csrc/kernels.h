@@ -6,8 +6,14 @@
 
 namespace xt {
 
-constexpr int TRACE_BLOCK = 128;      // lanes per block of the traversal kernels (2 waves)
-constexpr int TRACE_LDS_DEPTH = 32;   // LDS stack entries per lane (16 KB per block)
+#ifndef XT_TRACE_BLOCK
+#define XT_TRACE_BLOCK 256
+#endif
+#ifndef XT_TRACE_LDS_DEPTH
+#define XT_TRACE_LDS_DEPTH 16
+#endif
+constexpr int TRACE_BLOCK = XT_TRACE_BLOCK;          // lanes per block of the traversal kernels
+constexpr int TRACE_LDS_DEPTH = XT_TRACE_LDS_DEPTH;  // LDS stack entries per lane
 
 // Per-batch constants of the wavefront pipeline.
 struct WaveParams {

@@ -50,6 +50,9 @@ __device__ __forceinline__ void flush_counts(unsigned long long* totals, const V
     }
 }
 
+// Ray queries over a ray array (stride in float4 between rays): closest hit
+// (ANY = false) -> hits[i] = (t, slot or prim id, b0, b1); any hit ->
+// occluded[i].  Persistent waves with in-place lane refill (xt_traverse.h).
 template <bool ANY, int LDS_DEPTH, bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
                                                         const float4* __restrict__ ray_d, int ray_stride,
@@ -63,20 +66,45 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t* my_lds = lds + threadIdx.x;
     uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
+    const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
+    const uint32_t per = (n + nwaves - 1) / nwaves;
+    uint32_t next = min(wave * per, n);
+    const uint32_t end = min(next + per, n);
+    const uint64_t below = (1ull << __lane_id()) - 1ull;
     VisitCount vc;
-    for (size_t i = gtid; i < n; i += nthreads) {
-        float4 o4 = ray_o[i * ray_stride];
-        float4 d4 = ray_d[i * ray_stride];
-        RayPre r = ray_setup(xyz(o4), xyz(d4));
-        const float tmax = tmax_from_ray ? o4.w : INFINITY;   // extension rays: tMax = Infinity (PathTracer.h:167)
-        TraceResult res;
-        if (ANY) {
-            occluded[i] = traverse<true, LDS_DEPTH, COUNT>(sc, r, tmax, my_lds, TRACE_BLOCK, my_ovf, nthreads, &res, &vc)
-                              ? 1 : 0;
-        } else {
-            bool hit = traverse<false, LDS_DEPTH, COUNT>(sc, r, tmax, my_lds, TRACE_BLOCK, my_ovf, nthreads, &res, &vc);
-            hits[i] = hit ? make_float4(res.t, __uint_as_float(hit_prim_ids ? res.prim : res.slot), res.b0, res.b1)
-                          : make_float4(INFINITY, __uint_as_float(INVALID), 0.0f, 0.0f);
+    bool active = false;
+    uint32_t rid = 0;
+    RayPre r;
+    Trav t;
+    while (true) {
+        const uint64_t busy = __ballot(active);
+        const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
+        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
+            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < end) {
+                rid = my;
+                float4 o4 = ray_o[(size_t)rid * ray_stride];
+                float4 d4 = ray_d[(size_t)rid * ray_stride];
+                r = ray_setup(xyz(o4), xyz(d4));
+                trav_init(t, tmax_from_ray ? o4.w : INFINITY);   // extension rays: tMax = Infinity (PathTracer.h:167)
+                active = true;
+            }
+            next = min(next + nidle, end);
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            bool occ = false;
+            if (trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+                if (ANY) {
+                    occluded[rid] = occ ? 1 : 0;
+                } else {
+                    hits[rid] = t.best_slot != INVALID
+                                    ? make_float4(t.best_t, __uint_as_float(hit_prim_ids ? t.best_prim : t.best_slot),
+                                                  t.b0, t.b1)
+                                    : make_float4(INFINITY, __uint_as_float(INVALID), 0.0f, 0.0f);
+                }
+                active = false;
+            }
         }
     }
     if (COUNT) flush_counts(visit_totals, vc);
@@ -267,7 +295,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                         ext_e = me;
                         ext_pdf = pdfB;
                         ext_v = v + 1;
-                        ext_pe = (envn.on && pdfB >= 0.0f) ? envn.p_select * env_nee_pdf(sc.env, envn, normalize(wi), s.sn) : 0.0f;
+                        ext_pe = (envn.on && pdfB >= 0.0f) ? envn.p_select * env_nee_pdf(sc.env, envn, normalize(wi), s.sn)
+                                                           : 0.0f;
                     }
                     // ---- light sample (PathTracer.h:202-233)
                     uint4 blk1 = rng.block(v, 1);
@@ -275,13 +304,13 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                     if (envn.on) sample_env = !can_sample || Rng::to_unit(blk1.x) < envn.p_select;
                     if (sample_env) {
                         if (pdfB >= 0.0f) {
-                            uint4 blk2 = rng.block(v, 2), blk3 = rng.block(v, 3);
-                            f3 ld = env_nee_sample(sc.env, envn, s.sn, Rng::to_unit(blk1.w), Rng::to_unit(blk2.w),
-                                                   Rng::to_unit(blk3.x));
+                            uint4 blk2 = rng.block(v, 2);
+                            f3 ld = env_nee_sample(sc.env, envn, s.sn, Rng::to_unit(blk1.w), Rng::to_unit(blk2.x),
+                                                   Rng::to_unit(blk2.y));
                             float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
-                            f3 fc = bsdf_cos(mat, s, ld);
+                            float pb;
+                            f3 fc = bsdf_eval(mat, s, ld, &pb);
                             if (pe > 0.0f && !is_zero(fc)) {
-                                float pb = bsdf_pdf(mat, s, ld);
                                 float mis = (pe * pe) / (pb * pb + pe * pe);
                                 f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
                                 push_sh = true;
@@ -329,8 +358,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                         if (dot(sd, ln) < 0) {
                             float distSq = dist * dist;
                             float mis;
+                            float pb;
+                            f3 fc = bsdf_eval(mat, s, sd, &pb);
                             if (pdfB >= 0.0f) {
-                                float pb = bsdf_pdf(mat, s, sd);
                                 float cosL = fabsf(dot(-sd, lsn));
                                 pb *= cosL / distSq;
                                 mis = (pdfL * pdfL) / (pb * pb + pdfL * pdfL);
@@ -340,7 +370,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                             if (mis > 0.0f) {
                                 float G = fabsf(dot(-sd, lsn)) / distSq;
                                 const xt_material& lm = sc.mats[__float_as_uint(l1.w)];
-                                f3 c = div(weight * mis * bsdf_cos(mat, s, sd) * emitted(lm, -sd, ln) * G, pdfL);
+                                f3 c = div(weight * mis * fc * emitted(lm, -sd, ln) * G, pdfL);
                                 push_sh = true;
                                 sh_d = sd;
                                 sh_o = s.p + wp.eps * sd;
@@ -402,20 +432,48 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
     const uint32_t n = *count;
     const size_t nthreads = (size_t)gridDim.x * blockDim.x;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* my_lds = lds + threadIdx.x;
     uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
+    const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
+    const uint32_t per = (n + nwaves - 1) / nwaves;
+    uint32_t next = min(wave * per, n);
+    const uint32_t end = min(next + per, n);
+    const uint64_t below = (1ull << __lane_id()) - 1ull;
     VisitCount vc;
-    for (size_t i = gtid; i < n; i += nthreads) {
-        float4 o4 = sq.o[i], d4 = sq.d[i], c4 = sq.c[i];
-        RayPre r = ray_setup(xyz(o4), xyz(d4));
-        TraceResult res;
-        bool occ = traverse<true, TRACE_LDS_DEPTH, COUNT>(sc, r, o4.w, lds + threadIdx.x, TRACE_BLOCK, my_ovf, nthreads,
-                                                          &res, &vc);
-        uint32_t id = __float_as_uint(d4.w);
-        f3 c = occ ? splat(0.0f) : xyz(c4);
-        if (c4.w != 0.0f) {
-            pending[id] = f4(c, 0.0f);
-        } else if (!occ) {
-            radiance[id] = f4(xyz(radiance[id]) + c, 0.0f);
+    bool active = false;
+    uint32_t rid = 0;
+    RayPre r;
+    Trav t;
+    while (true) {
+        const uint64_t busy = __ballot(active);
+        const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
+        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
+            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < end) {
+                rid = my;
+                float4 o4 = sq.o[rid], d4 = sq.d[rid];
+                r = ray_setup(xyz(o4), xyz(d4));
+                trav_init(t, o4.w);
+                active = true;
+            }
+            next = min(next + nidle, end);
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            bool occ = false;
+            if (trav_step<true, TRACE_LDS_DEPTH, COUNT>(sc, r, t, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+                // unoccluded contributions go straight to the path radiance, or to
+                // pending[] when the path continues (see k_shade)
+                float4 c4 = sq.c[rid];
+                uint32_t id = __float_as_uint(sq.d[rid].w);
+                f3 c = occ ? splat(0.0f) : xyz(c4);
+                if (c4.w != 0.0f) {
+                    pending[id] = f4(c, 0.0f);
+                } else if (!occ) {
+                    radiance[id] = f4(xyz(radiance[id]) + c, 0.0f);
+                }
+                active = false;
+            }
         }
     }
     if (COUNT) flush_counts(visit_totals, vc);

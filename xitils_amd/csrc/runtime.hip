@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -314,7 +315,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     // traversal stack: at most 3 pushes per level plus one
     const uint32_t stack_bound = 3 * bvh.depth + 1;
     s->ovf_depth = stack_bound > (uint32_t)TRACE_LDS_DEPTH ? stack_bound - TRACE_LDS_DEPTH : 0;
-    s->trace_grid_max = (uint32_t)dev->num_cu * 16;
+    s->trace_grid_max = (uint32_t)dev->num_cu * 8;
     if (s->ovf_depth) XT_HIP_S(dalloc(&s->d_ovf, (size_t)s->ovf_depth * s->trace_grid_max * TRACE_BLOCK));
 #undef XT_HIP_S
     DevScene& ds = s->ds;
@@ -458,7 +459,16 @@ struct StageTimer {
     }
 };
 
-static uint32_t g_path_capacity = 1u << 22;  // 4 Mi paths in flight per batch
+// Paths in flight per batch (224 B each, 3.6 GB at the default 16 Mi);
+// XITILS_AMD_PATHS overrides.
+static uint32_t path_capacity() {
+    static uint32_t cap = [] {
+        const char* e = getenv("XITILS_AMD_PATHS");
+        long v = e ? atol(e) : 0;
+        return v >= 65536 && v <= (1l << 26) ? (uint32_t)v : (1u << 24);
+    }();
+    return cap;
+}
 
 // Records the first launch failure of a render (bad configuration, missing
 // code object) so it is reported instead of leaving the film untouched.
@@ -482,7 +492,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     const size_t film_bytes = npix_all * 3 * (film_f64 ? 8 : 4);
     s->stats = xt_render_stats{};
     if (p->spp == 0) return XT_OK;
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(g_path_capacity, npix_all * p->spp);
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix_all * p->spp);
     xt_status st = ensure_buffers(s, cap);
     if (st != XT_OK) return st;
     void* d_film = film;

@@ -70,7 +70,7 @@ XT_HD uint4 philox(uint4 c, uint2 k) {
 enum : int {
     D_RR = 0, D_BSDF_SEL = 1, D_BSDF_U1 = 2, D_BSDF_U2 = 3,
     D_LIGHT_KIND = 4, D_LIGHT_OBJ = 5, D_LIGHT_TRI = 6, D_ENV_SEL = 7,
-    D_BARY0 = 8, D_BARY1 = 9, D_BARY2 = 10, D_ENV_U1 = 11, D_ENV_U2 = 12,
+    D_BARY0 = 8, D_BARY1 = 9, D_BARY2 = 10, D_ENV_U1 = 8, D_ENV_U2 = 9,   // area and env samples are exclusive
     D_FILM_X = 0, D_FILM_Y = 1
 };
 
@@ -111,13 +111,6 @@ XT_HD float ggx_G1(float alpha, f3 x, f3 n) {
     float lambda = (-1 + safe_sqrt(1 + alpha2 * tan2)) / 2;
     return 1 / (1 + lambda);
 }
-XT_HD float ggx_DV(float alpha, f3 h, f3 n, f3 eye) {
-    return ggx_G1(alpha, eye, n) * clamp_pos(dot(eye, h)) * ggx_D(alpha, h, n) / dot(eye, n);
-}
-XT_HD float ggx_pdf(float alpha, f3 wo, f3 n, f3 wi) {
-    f3 h = normalize(wi + wo);
-    return ggx_DV(alpha, h, n, wo) / (4 * dot(wo, h));
-}
 // Visible-normal sampling, Heitz 2018 (Material.h:323-345).
 XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, f3 n, float U1, float U2) {
     const float PI = 3.14159265358979323846f;
@@ -150,52 +143,49 @@ XT_HD f3 sample_cos_hemisphere(f3 v, float u, float uphi) {
 
 XT_HD float cos_pdf(float c) { return (float)((double)c / 3.14159265358979323846); }
 
-// Material::bsdfCos = f * cos(theta_i) (Material.h:17).
-XT_HD f3 bsdf_cos(const xt_material& m, const Surf& s, f3 wi) {
+// Material::bsdfCos (Material.h:17) and Material::getPDF (Material.h:31) of
+// one direction, sharing the half vector, D and G1(wo) between the two.
+XT_HD f3 bsdf_eval(const xt_material& m, const Surf& s, f3 wi, float* pdf) {
     const float PI = 3.14159265358979323846f;
     switch (m.type) {
-    case XT_MAT_DIFFUSE:  // Material.h:59-65
-        return div(mk(m.p[0], m.p[1], m.p[2]), PI) * clamp_pos(dot(s.sn, wi));
-    case XT_MAT_METAL: {  // Material.h:347-356
+    case XT_MAT_DIFFUSE: {  // Material.h:59-65, 78-81
+        float c = clamp_pos(dot(s.sn, wi));
+        *pdf = cos_pdf(c);
+        return div(mk(m.p[0], m.p[1], m.p[2]), PI) * c;
+    }
+    case XT_MAT_METAL: {  // Material.h:347-356, 372-378
         float a = m.p[0];
         f3 h = normalize(wi + s.wo);
-        f3 Fr = schlick(mk(m.p[1], m.p[2], m.p[3]), dot(s.wo, h));
         float D = ggx_D(a, h, s.sn);
-        float G = ggx_G1(a, wi, s.sn) * ggx_G1(a, s.wo, s.sn);
+        float g1o = ggx_G1(a, s.wo, s.sn);
+        float dv = g1o * clamp_pos(dot(s.wo, h)) * D / dot(s.wo, s.sn);
+        *pdf = dv / (4 * dot(s.wo, h));
+        f3 Fr = schlick(mk(m.p[1], m.p[2], m.p[3]), dot(s.wo, h));
+        float G = ggx_G1(a, wi, s.sn) * g1o;
         return div(Fr * D * G, clamp_pos(4 * dot(s.wo, s.sn)));
     }
     case XT_MAT_GGX_LAMBERT: {
         float cosI = dot(s.sn, wi);
-        if (!(cosI > 0.0f)) return splat(0.0f);
+        if (!(cosI > 0.0f)) {
+            *pdf = 0.0f;
+            return splat(0.0f);
+        }
         float a = m.p[3];
         f3 h = normalize(wi + s.wo);
-        f3 Fr = schlick(splat(m.p[4]), dot(s.wo, h));
         float D = ggx_D(a, h, s.sn);
-        float G = ggx_G1(a, wi, s.sn) * ggx_G1(a, s.wo, s.sn);
+        float g1o = ggx_G1(a, s.wo, s.sn);
+        float dv = g1o * clamp_pos(dot(s.wo, h)) * D / dot(s.wo, s.sn);
+        float ps = m.p[5];
+        *pdf = ps * (dv / (4 * dot(s.wo, h))) + (1.0f - ps) * cos_pdf(cosI);
+        f3 Fr = schlick(splat(m.p[4]), dot(s.wo, h));
+        float G = ggx_G1(a, wi, s.sn) * g1o;
         f3 spec = div(Fr * D * G, clamp_pos(4 * dot(s.wo, s.sn)));
         f3 diff = mk(m.p[0], m.p[1], m.p[2]) * ((1.0f - m.p[4]) / PI) * cosI;
         return spec + diff;
     }
     default:
+        *pdf = 0.0f;
         return splat(0.0f);
-    }
-}
-
-// Material::getPDF (Material.h:31), solid angle.
-XT_HD float bsdf_pdf(const xt_material& m, const Surf& s, f3 wi) {
-    switch (m.type) {
-    case XT_MAT_DIFFUSE:  // Material.h:78-81
-        return cos_pdf(clamp_pos(dot(wi, s.sn)));
-    case XT_MAT_METAL:
-        return ggx_pdf(m.p[0], s.wo, s.sn, wi);
-    case XT_MAT_GGX_LAMBERT: {
-        float cosI = dot(s.sn, wi);
-        if (!(cosI > 0.0f)) return 0.0f;
-        float ps = m.p[5];
-        return ps * ggx_pdf(m.p[3], s.wo, s.sn, wi) + (1.0f - ps) * cos_pdf(cosI);
-    }
-    default:
-        return 0.0f;
     }
 }
 
@@ -248,8 +238,8 @@ XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, 
         float a = m.p[0];
         f3 h = ggx_sample_vndf(a, s.wo, s.sn, u1, u2);
         *wi = 2 * dot(s.wo, h) * h - s.wo;
-        *pdf = ggx_pdf(a, s.wo, s.sn, *wi);
-        f3 r = div(bsdf_cos(m, s, *wi), *pdf);
+        f3 fc = bsdf_eval(m, s, *wi, pdf);
+        f3 r = div(fc, *pdf);
         if (!(*pdf > 0.0f) || !finite3(r)) return splat(0.0f);
         return r;
     }
@@ -261,9 +251,9 @@ XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, 
         } else {
             *wi = sample_cos_hemisphere(s.sn, u1, u2);
         }
-        *pdf = bsdf_pdf(m, s, *wi);
+        f3 fc = bsdf_eval(m, s, *wi, pdf);
         if (!(*pdf > 0.0f)) return splat(0.0f);
-        f3 r = div(bsdf_cos(m, s, *wi), *pdf);
+        f3 r = div(fc, *pdf);
         if (!finite3(r)) return splat(0.0f);
         return r;
     }

@@ -20,6 +20,7 @@ struct RayPre {
     f3 o, d;
     float ix, iy, iz;      // 1 / d (Bounds.h:179)
     bool sx_neg, sy_neg, sz_neg;
+    int px, py, pz;        // float4 index of the near plane per axis: x 0|1, y 2|3, z 4|5
     int kx, ky, kz;        // permutation (TriangleIndexed.h:88-91)
     float shx, shy, shz;   // shear (TriangleIndexed.h:96-98)
 };
@@ -34,6 +35,9 @@ __device__ __forceinline__ RayPre ray_setup(f3 o, f3 d) {
     r.sx_neg = signbit(d.x);
     r.sy_neg = signbit(d.y);
     r.sz_neg = signbit(d.z);
+    r.px = r.sx_neg ? 1 : 0;
+    r.py = r.sy_neg ? 3 : 2;
+    r.pz = r.sz_neg ? 5 : 4;
     int kz = max_dim(mk(fabsf(d.x), fabsf(d.y), fabsf(d.z)));
     int kx = kz + 1;
     if (kx == 3) kx = 0;
@@ -91,12 +95,13 @@ struct TraceResult {
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
-// Four child slab tests; returns entry distances (inf = miss).
+// Four child slab tests; returns entry distances (inf = miss).  The near and
+// far planes of each axis are picked by address (the ray's direction signs),
+// so no per-child selects are needed.
 __device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const RayPre& r, float cull, float tn[4]) {
-    float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-    float4 nx = r.sx_neg ? hx : lx, fx = r.sx_neg ? lx : hx;
-    float4 ny = r.sy_neg ? hy : ly, fy = r.sy_neg ? ly : hy;
-    float4 nz = r.sz_neg ? hz : lz, fz = r.sz_neg ? lz : hz;
+    float4 nx = nd[r.px], fx = nd[r.px ^ 1];
+    float4 ny = nd[r.py], fy = nd[r.py ^ 1];
+    float4 nz = nd[r.pz], fz = nd[r.pz ^ 1];
     const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -189,6 +194,95 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, const RayPre& r, fl
     res->b0 = best_b0;
     res->b1 = best_b1;
     return best_slot != INVALID;
+}
+
+// ---------------------------------------------------------------- persistent traversal
+//
+// A wave owns the contiguous ray range [begin, end) and keeps its 64 lanes
+// busy: whenever at least REFILL_IDLE lanes have finished their ray, the idle
+// lanes take the next rays of the range (a wave-uniform cursor, no atomics).
+// Each loop iteration advances every busy lane by one node visit or one leaf.
+constexpr int REFILL_IDLE = 16;
+
+struct Trav {
+    int node, sp;
+    float best_t, b0, b1, cull, tmax;
+    uint32_t best_prim, best_slot;
+};
+
+__device__ __forceinline__ void trav_init(Trav& t, float tmax) {
+    t.node = 0;
+    t.sp = 0;
+    t.best_t = INFINITY;
+    t.b0 = t.b1 = 0.0f;
+    t.cull = tmax;
+    t.tmax = tmax;
+    t.best_prim = INVALID;
+    t.best_slot = INVALID;
+}
+
+// One traversal step; returns true when the ray is finished (for ANY, *occ
+// tells whether a blocker was found).
+template <bool ANY, int LDS_DEPTH, bool COUNT>
+__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, uint32_t* lds, int lds_stride,
+                                          uint32_t* ovf, size_t ovf_stride, VisitCount& vc, bool* occ) {
+    if (t.node >= 0) {
+        const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
+        if (COUNT) vc.nodes++;
+        float tn[4];
+        node_slabs(nd, r, t.cull, tn);
+        float4 chf = nd[6];
+        int ch[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
+#define XT_CSWAP(i, j)                                   \
+    if (tn[j] < tn[i]) {                                 \
+        float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;     \
+        int cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;       \
+    }
+        XT_CSWAP(0, 1) XT_CSWAP(2, 3) XT_CSWAP(0, 2) XT_CSWAP(1, 3) XT_CSWAP(1, 2)
+#undef XT_CSWAP
+        int nhit = (tn[0] < INFINITY) + (tn[1] < INFINITY) + (tn[2] < INFINITY) + (tn[3] < INFINITY);
+        if (nhit > 0) {
+            for (int k = nhit - 1; k >= 1; --k) {
+                if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
+                else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
+                ++t.sp;
+            }
+            t.node = ch[0];
+            return false;
+        }
+    } else {
+        // one triangle per step: a divergent wave then pays for one node
+        // visit plus one triangle test per iteration, not for a whole leaf
+        uint32_t code = (uint32_t)(~t.node);
+        uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
+        if (COUNT) vc.tris++;
+        const float4* tr = sc.tris + (size_t)first * TRI_F4;
+        float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
+        float th, b0, b1;
+        if (tri_test(xyz(v0), xyz(v1), xyz(v2), r, t.tmax, &th, &b0, &b1)) {
+            if (ANY) {
+                *occ = true;
+                return true;
+            }
+            uint32_t prim = __float_as_uint(v0.w);
+            if (th < t.best_t || (th == t.best_t && prim < t.best_prim)) {
+                t.best_t = th;
+                t.best_prim = prim;
+                t.best_slot = first;
+                t.b0 = b0;
+                t.b1 = b1;
+                t.cull = fminf(t.tmax, th);
+            }
+        }
+        if (cnt > 1) {  // rest of the leaf: first + 1, count - 1
+            t.node = ~(int)(((first + 1u) << 3) | (cnt - 2u));
+            return false;
+        }
+    }
+    if (t.sp == 0) return true;
+    --t.sp;
+    t.node = (int)(t.sp < LDS_DEPTH ? lds[t.sp * lds_stride] : ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride]);
+    return false;
 }
 
 }  // namespace xt
