@@ -168,7 +168,7 @@ enum RngDim : int {
     DIM_RR = 0, DIM_BSDF_SEL = 1, DIM_BSDF_U1 = 2, DIM_BSDF_U2 = 3,
     DIM_LIGHT_KIND = 4, DIM_LIGHT_OBJ = 5, DIM_LIGHT_TRI = 6, DIM_ENV_SEL = 7,
     DIM_BARY0 = 8, DIM_BARY1 = 9, DIM_BARY2 = 10,
-    DIM_ENV_U1 = 8, DIM_ENV_U2 = 9,   // an NEE sample is either an area point or an env direction
+    DIM_ENV_U1 = 5, DIM_ENV_U2 = 6,   // an NEE sample is either an area point or an env direction
     DIM_FILM_X = 0, DIM_FILM_Y = 1   // vertex slot 0 (camera)
 };
 

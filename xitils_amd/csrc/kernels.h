@@ -12,6 +12,10 @@ namespace xt {
 #ifndef XT_TRACE_LDS_DEPTH
 #define XT_TRACE_LDS_DEPTH 16
 #endif
+#ifndef XT_SHADE_BLOCK
+#define XT_SHADE_BLOCK 512
+#endif
+constexpr int SHADE_BLOCK = XT_SHADE_BLOCK;          // threads per block of the shade kernel
 constexpr int TRACE_BLOCK = XT_TRACE_BLOCK;          // lanes per block of the traversal kernels
 constexpr int TRACE_LDS_DEPTH = XT_TRACE_LDS_DEPTH;  // LDS stack entries per lane
 

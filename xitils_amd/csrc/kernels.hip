@@ -7,7 +7,9 @@
 //           (PathTracer.h:147-246 / 65-101): emission at the new vertex,
 //           roulette, BSDF sample -> next queue, light sample -> shadow queue
 // shadow  : any hit for every shadow ray             (Scene::intersectAny, Scene.h:34)
-// Queues are compacted with one atomic per wave (ballot + popcount prefix).
+// Queues are compacted with one atomic per queue per 512-thread block (wave
+// ballots + an LDS prefix); per-wave atomics on the two queue counters were
+// the shade kernel's bottleneck.
 // Radiance is kept per path id and summed in the reference's order (BSDF-hit
 // emission of vertex v+1 before the light sample of vertex v), so the image
 // does not depend on queue order.
@@ -16,18 +18,39 @@
 
 namespace xt {
 
-// Wave-aggregated append: returns this lane's slot (valid when pred).  Every
-// active lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    uint32_t lane = __lane_id();
-    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + prefix;
+// Block-aggregated appends to two queues at once (extension and shadow): one
+// atomic per queue per block instead of per wave.  Every thread of the block
+// must call it (the callers' loops are block-uniform).  Returns this thread's
+// slots (valid when its predicate is set).
+template <int BLOCK>
+__device__ __forceinline__ uint2 block_append2(uint32_t* counter_a, bool pa, uint32_t* counter_b, bool pb,
+                                               uint32_t* sh /* 2 * (BLOCK / 64) + 2 words of LDS */) {
+    constexpr int NW = BLOCK / 64;
+    const uint64_t ma = __ballot(pa), mb = __ballot(pb);
+    const uint32_t wave = threadIdx.x / 64, lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (lane == 0) {
+        sh[wave] = (uint32_t)__popcll(ma);
+        sh[NW + wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (int w = 0; w < NW; ++w) {
+            uint32_t ca = sh[w], cb = sh[NW + w];
+            sh[w] = ta;
+            sh[NW + w] = tb;
+            ta += ca;
+            tb += cb;
+        }
+        sh[2 * NW] = ta ? atomicAdd(counter_a, ta) : 0u;
+        sh[2 * NW + 1] = tb ? atomicAdd(counter_b, tb) : 0u;
+    }
+    __syncthreads();
+    uint2 r = make_uint2(sh[2 * NW] + sh[wave] + (uint32_t)__popcll(ma & below),
+                         sh[2 * NW + 1] + sh[NW + wave] + (uint32_t)__popcll(mb & below));
+    __syncthreads();   // sh is reused by the next iteration
+    return r;
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -180,7 +203,7 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 
 
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-__global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
                                                const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
                                                PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
                                                uint32_t* __restrict__ count_shadow, float4* __restrict__ radiance,
@@ -193,7 +216,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
     const bool uniform_bary = (wp.flags & XT_FLAG_UNIFORM_BARY) != 0;
     const uint2 key = make_uint2(wp.seed, wp.stream);
     const uint32_t stride = gridDim.x * blockDim.x;
-    // whole-wave iterations so the appends below always see a converged wave
+    __shared__ uint32_t append_lds[2 * (SHADE_BLOCK / 64) + 2];
+    // whole-block iterations so the appends below always see every thread
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool push_ext = false, push_sh = false;
@@ -284,9 +308,11 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                     else weight = div(weight, wp.rr_prob);
                 }
                 if (live) {
+                    const Frame fs = make_frame(s.sn);   // BasisVectors(shading.n), shared by both samplers
                     f3 wi = splat(0.0f);
                     float pdfB = 0.0f;
-                    f3 me = bsdf_sample(mat, s, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi, &pdfB);
+                    f3 me = bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
+                                        &pdfB);
                     if (!is_zero(me)) {                            // PathTracer.h:165-167
                         push_ext = true;
                         ext_d = wi;
@@ -304,9 +330,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                     if (envn.on) sample_env = !can_sample || Rng::to_unit(blk1.x) < envn.p_select;
                     if (sample_env) {
                         if (pdfB >= 0.0f) {
-                            uint4 blk2 = rng.block(v, 2);
-                            f3 ld = env_nee_sample(sc.env, envn, s.sn, Rng::to_unit(blk1.w), Rng::to_unit(blk2.x),
-                                                   Rng::to_unit(blk2.y));
+                            f3 ld = env_nee_sample(sc.env, envn, fs, Rng::to_unit(blk1.w), Rng::to_unit(blk1.y),
+                                                   Rng::to_unit(blk1.z));
                             float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
                             float pb;
                             f3 fc = bsdf_eval(mat, s, ld, &pb);
@@ -384,7 +409,8 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                 uint4 blk0 = rng.block(v, 0);
                 f3 wi = splat(0.0f);
                 float pdfB = 0.0f;
-                weight = weight * bsdf_sample(mat, s, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi, &pdfB);
+                weight = weight * bsdf_sample(mat, s, make_frame(s.sn), Rng::to_unit(blk0.y), Rng::to_unit(blk0.z),
+                                              Rng::to_unit(blk0.w), &wi, &pdfB);
                 if (is_zero(weight)) live = false;
                 if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
                 if (live && (int)(v + 1) > wp.rr_start) {
@@ -403,9 +429,9 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams wp, DevScene sc, PathQ
                 }
             }
         }
-        // ---- compacted appends (whole wave participates)
-        const uint32_t ei = wave_append(count_out, push_ext);
-        const uint32_t si = wave_append(count_shadow, push_sh);
+        // ---- compacted appends (whole block participates)
+        const uint2 slots = block_append2<SHADE_BLOCK>(count_out, push_ext, count_shadow, push_sh, append_lds);
+        const uint32_t ei = slots.x, si = slots.y;
         if (push_ext) {
             uint32_t fl = push_sh ? QF_PENDING : 0u;
             qout.o[ei] = f4(ext_o, ext_pdf);

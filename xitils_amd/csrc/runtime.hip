@@ -335,6 +335,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     ds.env.sun_omc = e.sun_one_minus_cos;
     ds.env.sun_sin2 = e.sun_sin2;
     ds.env.sun_L = mk(e.sun_radiance[0], e.sun_radiance[1], e.sun_radiance[2]);
+    ds.env.sun_frame = make_frame(ds.env.sun_dir);
     s->bvh_nodes = bvh.num_nodes;
     s->bvh_depth = bvh.depth;
     s->bytes_nodes = (uint64_t)bvh.num_nodes * 128;
@@ -521,6 +522,10 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     hipEvent_t t_begin = tm.mark();
     const uint32_t gblocks_max = (uint32_t)s->dev->num_cu * 8;
     auto blocks256 = [&](uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, gblocks_max)); };
+    const uint32_t shade_max = (uint32_t)s->dev->num_cu * (2048 / SHADE_BLOCK);
+    auto shade_blocks = [&](uint64_t n) {
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + SHADE_BLOCK - 1) / SHADE_BLOCK, shade_max));
+    };
     // batches: S samples of a pixel range, S * pixels <= capacity
     const uint32_t S = npix_all <= cap ? (uint32_t)std::min<uint64_t>(cap / npix_all, p->spp) : 1u;
     const uint32_t pix_chunk = (uint32_t)std::min<uint64_t>(npix_all, cap);
@@ -562,7 +567,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
                 XT_LAUNCHED(result);
                 a = tm.mark();
-                hipLaunchKernelGGL(k_shade, dim3(blocks256(live)), dim3(256), 0, s->stream, wp, s->ds, rb.q[cur],
+                hipLaunchKernelGGL(k_shade, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
                                    rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
                                    rb.radiance, rb.pending);
                 XT_LAUNCHED(result);

@@ -33,6 +33,7 @@ struct DevEnv {
     float sun_omc;   // 1 - cos(half angle)
     float sun_sin2;  // sin^2(half angle)
     f3 sun_L;
+    Frame sun_frame; // make_frame(sun_dir), built once on the host
 };
 
 struct DevScene {
@@ -70,7 +71,8 @@ XT_HD uint4 philox(uint4 c, uint2 k) {
 enum : int {
     D_RR = 0, D_BSDF_SEL = 1, D_BSDF_U1 = 2, D_BSDF_U2 = 3,
     D_LIGHT_KIND = 4, D_LIGHT_OBJ = 5, D_LIGHT_TRI = 6, D_ENV_SEL = 7,
-    D_BARY0 = 8, D_BARY1 = 9, D_BARY2 = 10, D_ENV_U1 = 8, D_ENV_U2 = 9,   // area and env samples are exclusive
+    D_BARY0 = 8, D_BARY1 = 9, D_BARY2 = 10,
+    D_ENV_U1 = 5, D_ENV_U2 = 6,   // an env sample uses the area-light object/triangle dims: one block
     D_FILM_X = 0, D_FILM_Y = 1
 };
 
@@ -111,19 +113,30 @@ XT_HD float ggx_G1(float alpha, f3 x, f3 n) {
     float lambda = (-1 + safe_sqrt(1 + alpha2 * tan2)) / 2;
     return 1 / (1 + lambda);
 }
-// Visible-normal sampling, Heitz 2018 (Material.h:323-345).
-XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, f3 n, float U1, float U2) {
+// Disk point shared by the sampling routines: r = sqrt(u1), phi = 2 pi u2
+// (Material.h:335-338, Vector.h:821-823 compute the same values).
+struct DiskPt {
+    float r, c, s;
+};
+XT_HD DiskPt disk_pt(float u1, float u2) {
     const float PI = 3.14159265358979323846f;
-    Frame fr = make_frame(n);
+    DiskPt d;
+    d.r = sqrtf(u1);
+    float phi = u2 * (2.0f * PI);
+    d.c = cosf(phi);
+    d.s = sinf(phi);
+    return d;
+}
+
+// Visible-normal sampling, Heitz 2018 (Material.h:323-345); fr = make_frame(n).
+XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, const Frame& fr, DiskPt dp) {
     f3 Ve = to_local(fr, eye);
     f3 Vh = normalize(mk(Ve.x, alpha * Ve.y, alpha * Ve.z));
     float lensq = Vh.y * Vh.y + Vh.z * Vh.z;
     f3 T1 = lensq > 0 ? div(mk(0, -Vh.z, Vh.y), safe_sqrt(lensq)) : mk(0, 0, 1);
     f3 T2 = cross(Vh, T1);
-    float r = sqrtf(U1);
-    float phi = 2.0f * PI * U2;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = dp.r * dp.c;
+    float t2 = dp.r * dp.s;
     float s = 0.5f * (1.0f + Vh.x);
     t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
     f3 Nh = t1 * T1 + t2 * T2 + safe_sqrt(1.0f - t1 * t1 - t2 * t2) * Vh;
@@ -132,13 +145,9 @@ XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, f3 n, float U1, float U2) {
 }
 XT_HD f3 schlick(f3 f0, float c) { return f0 + (splat(1.0f) - f0) * powf(1 - c, 5.0f); }
 
-// cosine-weighted hemisphere about v (Vector.h:817-824)
-XT_HD f3 sample_cos_hemisphere(f3 v, float u, float uphi) {
-    const float PI = 3.14159265358979323846f;
-    Frame fr = make_frame(v);
-    float r = sqrtf(u);
-    float phi = uphi * (2.0f * PI);
-    return to_world(fr, safe_sqrt(1.0f - u), r * cosf(phi), r * sinf(phi));
+// cosine-weighted hemisphere in frame fr (Vector.h:817-824)
+XT_HD f3 sample_cos_hemisphere(const Frame& fr, float u, DiskPt dp) {
+    return to_world(fr, safe_sqrt(1.0f - u), dp.r * dp.c, dp.r * dp.s);
 }
 
 XT_HD float cos_pdf(float c) { return (float)((double)c / 3.14159265358979323846); }
@@ -201,10 +210,13 @@ XT_HD float fresnel_dielectric(f3 wo, f3 n, float eta_o, float eta_i) {  // Mate
 }
 
 // Material::evalAndSample (Material.h:25): f*cos/pdf, wi, pdf (-1 = delta).
-XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, float u2, f3* wi, float* pdf) {
+// fr = make_frame(s.sn), shared with the light sampler.
+XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, const Frame& fr, float usel, float u1, float u2, f3* wi,
+                     float* pdf) {
+    const DiskPt dp = disk_pt(u1, u2);
     switch (m.type) {
     case XT_MAT_DIFFUSE: {  // Material.h:67-76
-        *wi = sample_cos_hemisphere(s.sn, u1, u2);
+        *wi = sample_cos_hemisphere(fr, u1, dp);
         *pdf = cos_pdf(dot(*wi, s.sn));
         return mk(m.p[0], m.p[1], m.p[2]);
     }
@@ -236,7 +248,7 @@ XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, 
     }
     case XT_MAT_METAL: {  // Material.h:358-370
         float a = m.p[0];
-        f3 h = ggx_sample_vndf(a, s.wo, s.sn, u1, u2);
+        f3 h = ggx_sample_vndf(a, s.wo, fr, dp);
         *wi = 2 * dot(s.wo, h) * h - s.wo;
         f3 fc = bsdf_eval(m, s, *wi, pdf);
         f3 r = div(fc, *pdf);
@@ -246,10 +258,10 @@ XT_HD f3 bsdf_sample(const xt_material& m, const Surf& s, float usel, float u1, 
     case XT_MAT_GGX_LAMBERT: {
         float a = m.p[3];
         if (usel < m.p[5]) {
-            f3 h = ggx_sample_vndf(a, s.wo, s.sn, u1, u2);
+            f3 h = ggx_sample_vndf(a, s.wo, fr, dp);
             *wi = 2 * dot(s.wo, h) * h - s.wo;
         } else {
-            *wi = sample_cos_hemisphere(s.sn, u1, u2);
+            *wi = sample_cos_hemisphere(fr, u1, dp);
         }
         f3 fc = bsdf_eval(m, s, *wi, pdf);
         if (!(*pdf > 0.0f)) return splat(0.0f);
@@ -302,17 +314,20 @@ XT_HD float env_nee_pdf(const DevEnv& e, const EnvNee& s, f3 du, f3 n) {
     if (s.p_sun < 1.0f) p += (1.0f - s.p_sun) * cos_pdf(clamp_pos(dot(du, n)));
     return p;
 }
-XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, f3 n, float usel, float u1, float u2) {
-    const float PI = 3.14159265358979323846f;
-    if (usel < s.p_sun) {
-        float uo = u1 * e.sun_omc;
-        float cosT = 1.0f - uo;
-        float sinT = safe_sqrt(uo * (2.0f - uo));
-        float phi = 2.0f * PI * u2;
-        Frame fr = make_frame(e.sun_dir);
-        return to_world(fr, cosT, sinT * cosf(phi), sinT * sinf(phi));
-    }
-    return sample_cos_hemisphere(n, u1, u2);
+// Sun cone (uniform in solid angle) or cosine sky about the shading normal
+// (frame fr); both branches share phi and the frame arithmetic, so the
+// choice is a select, not a divergent branch.
+XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, const Frame& fr, float usel, float u1, float u2) {
+    const bool sun = usel < s.p_sun;
+    const DiskPt dp = disk_pt(u1, u2);
+    float uo = u1 * e.sun_omc;
+    float a = sun ? 1.0f - uo : safe_sqrt(1.0f - u1);
+    float b = sun ? safe_sqrt(uo * (2.0f - uo)) : dp.r;
+    Frame f;
+    f.e1 = sun ? e.sun_frame.e1 : fr.e1;
+    f.e2 = sun ? e.sun_frame.e2 : fr.e2;
+    f.e3 = sun ? e.sun_frame.e3 : fr.e3;
+    return to_world(f, a, b * dp.c, b * dp.s);
 }
 
 }  // namespace xt

@@ -121,81 +121,6 @@ struct VisitCount {
     uint32_t nodes = 0, tris = 0;
 };
 
-// Closest hit (ANY = false) or any hit (ANY = true) against the reference
-// triangle test with bound tmax; closest = lexicographic min (t, prim id).
-// COUNT adds node-fetch and triangle-fetch counts to *vc.
-template <bool ANY, int LDS_DEPTH, bool COUNT = false>
-__device__ __forceinline__ bool traverse(const DevScene& sc, const RayPre& r, float tmax, uint32_t* lds, int lds_stride,
-                                         uint32_t* ovf, size_t ovf_stride, TraceResult* res,
-                                         VisitCount* vc = nullptr) {
-    float best_t = INFINITY;
-    uint32_t best_prim = INVALID, best_slot = INVALID;
-    float best_b0 = 0.0f, best_b1 = 0.0f;
-    float cull = tmax;
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        if (node >= 0) {
-            const float4* nd = sc.nodes + (size_t)node * NODE_F4;
-            if (COUNT) vc->nodes++;
-            float tn[4];
-            node_slabs(nd, r, cull, tn);
-            float4 chf = nd[6];
-            int ch[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
-            // sort (tn, ch) ascending with a 4-input network; misses sort last
-#define XT_CSWAP(i, j)                                   \
-    if (tn[j] < tn[i]) {                                 \
-        float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;     \
-        int cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;       \
-    }
-            XT_CSWAP(0, 1) XT_CSWAP(2, 3) XT_CSWAP(0, 2) XT_CSWAP(1, 3) XT_CSWAP(1, 2)
-#undef XT_CSWAP
-            int nhit = (tn[0] < INFINITY) + (tn[1] < INFINITY) + (tn[2] < INFINITY) + (tn[3] < INFINITY);
-            if (nhit > 0) {
-                // push the far ones, farthest first; continue with the nearest
-                for (int k = nhit - 1; k >= 1; --k) {
-                    if (sp < LDS_DEPTH) lds[sp * lds_stride] = (uint32_t)ch[k];
-                    else ovf[(size_t)(sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
-                    ++sp;
-                }
-                node = ch[0];
-                continue;
-            }
-        } else {
-            uint32_t code = (uint32_t)(~node);
-            uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                if (COUNT) vc->tris++;
-                const float4* tr = sc.tris + (size_t)(first + k) * TRI_F4;
-                float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
-                float t, b0, b1;
-                if (tri_test(xyz(v0), xyz(v1), xyz(v2), r, tmax, &t, &b0, &b1)) {
-                    if (ANY) return true;
-                    uint32_t prim = __float_as_uint(v0.w);
-                    if (t < best_t || (t == best_t && prim < best_prim)) {
-                        best_t = t;
-                        best_prim = prim;
-                        best_slot = first + k;
-                        best_b0 = b0;
-                        best_b1 = b1;
-                        cull = fminf(tmax, t);
-                    }
-                }
-            }
-        }
-        if (sp == 0) break;
-        --sp;
-        node = (int)(sp < LDS_DEPTH ? lds[sp * lds_stride] : ovf[(size_t)(sp - LDS_DEPTH) * ovf_stride]);
-    }
-    if (ANY) return false;
-    res->t = best_t;
-    res->prim = best_prim;
-    res->slot = best_slot;
-    res->b0 = best_b0;
-    res->b1 = best_b1;
-    return best_slot != INVALID;
-}
-
 // ---------------------------------------------------------------- persistent traversal
 //
 // A wave owns the contiguous ray range [begin, end) and keeps its 64 lanes
