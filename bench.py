@@ -171,7 +171,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles("extend_heightfield1080p"),
+        "traffic": traffic_from_profiles("extend_heightfield1080p") if (args.cells, W, H, spp) == (1024, 1920, 1080, 64)
+        else None,
         "kernel": "k_trace<closest> (extend)",
         "bytes_per_launch": round(per_step_ext_bytes / launches),
         "avg_launch_ms": round(agg["ms_extend"] / agg["extend_launches"], 4),
