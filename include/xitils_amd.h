@@ -112,7 +112,9 @@ enum {
 };
 enum {
     XT_FLAG_UNIFORM_BARY = 1u,   /* area-uniform light points (the reference's are not, TriangleIndexed.h:184-192) */
-    XT_FLAG_ENV_NEE = 2u         /* sample sun + sky as a light, MIS with BSDF escapes */
+    XT_FLAG_ENV_NEE = 2u,        /* sample sun + sky as a light, MIS with BSDF escapes */
+    XT_FLAG_GUIDE = 4u,          /* standard integrator: sample with the scene's guiding network */
+    XT_FLAG_GUIDE_TRAIN = 8u     /* with XT_FLAG_GUIDE: record training samples, take Adam steps per batch */
 };
 
 typedef struct xt_render_params {
@@ -139,6 +141,10 @@ typedef struct xt_render_stats {
     /* BVH node (128 B) and triangle (48 B) fetches of the traversal kernels;
      * filled only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
+    /* guiding: network inference and training kernel time, Adam steps taken
+     * and training samples kept in the call */
+    double ms_guide_infer, ms_guide_train;
+    uint64_t guide_steps, guide_samples;
 } xt_render_stats;
 
 typedef struct xt_ray {        /* 32 bytes */
@@ -189,6 +195,51 @@ xt_status xt_render_get_stats(const xt_scene* scene, xt_render_stats* out);
  * traversal variant that counts node and triangle fetches into the stats
  * (algorithmic-traffic accounting for the roofline).  Results are identical. */
 xt_status xt_scene_set_counting(xt_scene* scene, int on);
+
+/* ---- learned path guiding --------------------------------------------------
+ * No reference counterpart: the reference ships only the vMF mixture
+ * (VonMisesFisherDistribution.h:9-217).  A 3x64 MLP maps (position, shading
+ * normal, view direction) to an 8-lobe vMF mixture over incident directions;
+ * raw outputs per lobe j are y[5j..5j+4] = (mu unnormalised, log kappa,
+ * alpha logit).  Weights live per scene on the device: fp32 master copy,
+ * bf16 copies for the MFMA kernels, Adam state. */
+#define XT_GUIDE_NUM_PARAMS 17648   /* W0 64x96, b0 64, W1 64x64, b1, W2 64x64, b2, W3 48x64, b3 48 */
+#define XT_GUIDE_NUM_INPUTS 9       /* query: p.xyz, shading n.xyz, wo.xyz */
+#define XT_GUIDE_NUM_OUTPUTS 40
+
+typedef struct xt_guide_config {
+    float guide_prob;            /* one-sample MIS probability of sampling the guide (0.5) */
+    float learning_rate;         /* Adam step size (1e-2) */
+    float beta1, beta2, adam_eps;/* Adam (0.9, 0.999, 1e-8) */
+    uint32_t minibatch;          /* samples per Adam step (65536) */
+    uint32_t max_steps_per_batch;/* Adam steps after one render batch (32) */
+    float weight_clip;           /* cap on sample weights normalised to mean 1 (20) */
+    uint32_t seed;               /* parameter initialisation */
+    uint32_t max_records;        /* training records kept per render batch (8 Mi) */
+} xt_guide_config;
+
+void xt_guide_config_default(xt_guide_config* cfg);
+/* Allocates the scene's network and initialises its parameters from cfg->seed
+ * (He-uniform hidden layers, small output layer).  Encoding bounds = the
+ * scene's bounding box. */
+xt_status xt_guide_init(xt_scene* scene, const xt_guide_config* cfg);
+/* Flat fp32 parameters (XT_GUIDE_NUM_PARAMS, layout above, row-major [out][in]). */
+xt_status xt_guide_get_params(xt_scene* scene, float* params);
+/* Replaces the parameters and clears the Adam state. */
+xt_status xt_guide_set_params(xt_scene* scene, const float* params);
+/* Bounds used to scale positions to [0, 1] for the frequency encoding. */
+xt_status xt_guide_get_bounds(xt_scene* scene, float lo[3], float hi[3]);
+/* Raw outputs (n x 40) of the bf16 network for n host queries (n x 9). */
+xt_status xt_guide_forward(xt_scene* scene, const float* queries, uint32_t n, float* outputs);
+/* Gradient of sum_i -min(w_i * weight_scale, weight_clip) log q(dir_i | query_i)
+ * with respect to the flat parameters (host arrays: queries n x 9, dirs n x 3,
+ * weights n; grad XT_GUIDE_NUM_PARAMS). */
+xt_status xt_guide_gradient(xt_scene* scene, const float* queries, const float* dirs, const float* weights, uint32_t n,
+                            float weight_scale, float weight_clip, float* grad);
+/* One Adam step on the mean loss of n host samples (weights normalised to mean
+ * 1 and clipped as in training). */
+xt_status xt_guide_train_step(xt_scene* scene, const float* queries, const float* dirs, const float* weights,
+                              uint32_t n);
 
 #ifdef __cplusplus
 }

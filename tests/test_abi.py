@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "xitils_amd.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:const char\*|xt_status)\s+(xt_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:const char\*|xt_status|void)\s+(xt_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -43,6 +43,7 @@ STRUCTS = {
     "xt_camera": api.xt_camera,
     "xt_render_params": api.xt_render_params,
     "xt_render_stats": api.xt_render_stats,
+    "xt_guide_config": api.xt_guide_config,
 }
 
 

@@ -22,7 +22,15 @@ STATUS_NAMES = {0: "XT_OK", 1: "XT_ERR_INVALID_ARGUMENT", 2: "XT_ERR_UNSUPPORTED
 
 EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy",
             "xt_scene_info", "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats",
-            "xt_scene_set_counting"]
+            "xt_scene_set_counting", "xt_guide_config_default", "xt_guide_init", "xt_guide_get_params",
+            "xt_guide_set_params", "xt_guide_get_bounds", "xt_guide_forward", "xt_guide_gradient",
+            "xt_guide_train_step"]
+
+XT_FLAG_UNIFORM_BARY = 1
+XT_FLAG_ENV_NEE = 2
+XT_FLAG_GUIDE = 4
+XT_FLAG_GUIDE_TRAIN = 8
+GUIDE_NUM_PARAMS = 17648
 
 
 class XtError(RuntimeError):
@@ -68,7 +76,15 @@ class xt_render_stats(C.Structure):
                 ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_other", C.c_double), ("extend_launches", C.c_uint32),
                 ("ext_node_fetches", C.c_uint64), ("ext_tri_fetches", C.c_uint64),
-                ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64)]
+                ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64),
+                ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double), ("guide_steps", C.c_uint64),
+                ("guide_samples", C.c_uint64)]
+
+
+class xt_guide_config(C.Structure):
+    _fields_ = [("guide_prob", C.c_float), ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("adam_eps", C.c_float), ("minibatch", C.c_uint32), ("max_steps_per_batch", C.c_uint32),
+                ("weight_clip", C.c_float), ("seed", C.c_uint32), ("max_records", C.c_uint32)]
 
 
 _lib = None
@@ -94,8 +110,20 @@ def lib():
                                 C.c_int]
         L.xt_render_get_stats.argtypes = [C.c_void_p, C.POINTER(xt_render_stats)]
         L.xt_scene_set_counting.argtypes = [C.c_void_p, C.c_int]
+        L.xt_guide_config_default.argtypes = [C.POINTER(xt_guide_config)]
+        L.xt_guide_config_default.restype = None
+        L.xt_guide_init.argtypes = [C.c_void_p, C.POINTER(xt_guide_config)]
+        L.xt_guide_get_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.xt_guide_set_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.xt_guide_get_bounds.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xt_guide_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.xt_guide_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                        C.c_float, C.c_void_p]
+        L.xt_guide_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         for f in ("xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy", "xt_scene_info",
-                  "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting"):
+                  "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting",
+                  "xt_guide_init", "xt_guide_get_params", "xt_guide_set_params", "xt_guide_get_bounds",
+                  "xt_guide_forward", "xt_guide_gradient", "xt_guide_train_step"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -262,3 +290,52 @@ class Scene:
         s = xt_render_stats()
         check(lib().xt_render_get_stats(self.h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in xt_render_stats._fields_}
+
+    # ---- learned guiding -------------------------------------------------
+    @staticmethod
+    def guide_config(**kw):
+        """xt_guide_config with the library defaults, fields overridden by kw."""
+        c = xt_guide_config()
+        lib().xt_guide_config_default(C.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def guide_init(self, **kw):
+        check(lib().xt_guide_init(self.h, C.byref(self.guide_config(**kw))))
+
+    def guide_params(self):
+        out = np.zeros(GUIDE_NUM_PARAMS, np.float32)
+        check(lib().xt_guide_get_params(self.h, out.ctypes.data))
+        return out
+
+    def guide_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(GUIDE_NUM_PARAMS)
+        check(lib().xt_guide_set_params(self.h, p.ctypes.data))
+
+    def guide_bounds(self):
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        check(lib().xt_guide_get_bounds(self.h, lo.ctypes.data, hi.ctypes.data))
+        return lo, hi
+
+    def guide_forward(self, queries):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        out = np.zeros((q.shape[0], 40), np.float32)
+        check(lib().xt_guide_forward(self.h, q.ctypes.data, q.shape[0], out.ctypes.data))
+        return out
+
+    def guide_gradient(self, queries, dirs, weights, weight_scale=1.0, weight_clip=float("inf")):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        g = np.zeros(GUIDE_NUM_PARAMS, np.float32)
+        check(lib().xt_guide_gradient(self.h, q.ctypes.data, d.ctypes.data, w.ctypes.data, q.shape[0], weight_scale,
+                                      weight_clip, g.ctypes.data))
+        return g
+
+    def guide_train_step(self, queries, dirs, weights):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        check(lib().xt_guide_train_step(self.h, q.ctypes.data, d.ctypes.data, w.ctypes.data, q.shape[0]))

@@ -1,0 +1,430 @@
+// Guiding MLP on gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16) forward and
+// backward passes with FP32 accumulation, Adam, and the render-side glue
+// (per-vertex inference, training-sample finalisation).
+//
+// Matrix products use the 16x16x32 MFMA.  Lane l of a wave64 holds
+//   A[m0 + l%16][k0 + 8(l/16) + j], B[k0 + 8(l/16) + j][n0 + l%16] (j = 0..7),
+//   C[m0 + 4(l/16) + i][n0 + l%16] (i = 0..3)
+// (checked on the device).  Every operand is stored with its K index
+// contiguous, so a fragment is one 16-byte load.  Activations travel
+// query-major through LDS: X/H[query][feature].
+#include <hip/hip_runtime.h>
+
+#include "guide.h"
+
+namespace xt {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+__device__ __forceinline__ bf16x8 frag(const __bf16* M, int ld, int row0, int col0) {
+    const int l = __lane_id();
+    return *(const bf16x8*)(M + (size_t)(row0 + (l & 15)) * ld + col0 + 8 * (l >> 4));
+}
+
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS written by some lanes of a wave is read by others: order the wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One dense layer for 64 queries: out[q][o] = act(sum_k W[o][k] in[q][k] + b[o])
+// W: [OUT][IN] bf16 (OUT multiple of 16, IN multiple of 32), in: [64][IN] bf16
+// (query-major); writes bf16 [64][OUT] (hidden) or fp32 [64][OUT] (last), and
+// optionally the bf16 transpose [OUT][64].
+template <int IN, int OUT, bool RELU, bool F32OUT>
+__device__ __forceinline__ void dense64(const __bf16* W, const float* bias, const __bf16* in, void* out,
+                                        __bf16* out_t = nullptr) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+#pragma unroll 1
+    for (int b = 0; b < OUT / 16; ++b) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            floatx4 acc;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = bias[16 * b + 4 * g + i];
+#pragma unroll
+            for (int c = 0; c < IN / 32; ++c) acc = mfma(frag(W, IN, 16 * b, 32 * c), frag(in, IN, 16 * t, 32 * c), acc);
+            const int q = 16 * t + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = 16 * b + 4 * g + i;
+                float v = RELU ? fmaxf(acc[i], 0.0f) : acc[i];
+                if (F32OUT) ((float*)out)[q * OUT + o] = v;
+                else ((__bf16*)out)[q * OUT + o] = (__bf16)v;
+                if (out_t) out_t[o * 64 + q] = (__bf16)v;
+            }
+        }
+    }
+}
+
+// Four layers for the 64 queries whose features sit in X ([64][96] bf16);
+// H is a [64][64] bf16 buffer; the raw outputs land in Y ([64][48] fp32,
+// which may alias X).
+__device__ __forceinline__ void mlp64(const __bf16* w0, const __bf16* w1, const __bf16* w2, const __bf16* w3,
+                                      const float* b0, const float* b1, const float* b2, const float* b3, __bf16* X,
+                                      __bf16* H, float* Y) {
+    dense64<G_IN, G_HID, true, false>(w0, b0, X, H);
+    wave_sync();
+    dense64<G_HID, G_HID, true, false>(w1, b1, H, X);
+    wave_sync();
+    dense64<G_HID, G_HID, true, false>(w2, b2, X, H);
+    wave_sync();
+    dense64<G_HID, G_OUT_PAD, false, true>(w3, b3, H, Y);
+    wave_sync();
+}
+
+// ---------------------------------------------------------------- parameters
+
+// fp32 master parameters -> bf16 copies: natural [out][in] for the forward
+// pass and transposed [in][out] (w3t padded to [64][64]) for the backward
+// pass; biases stay fp32.
+__global__ void k_guide_prepare(const float* __restrict__ P, GuideNet net) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < G_HID * G_IN) {
+        int o = i / G_IN, k = i % G_IN;
+        net.w0[i] = (__bf16)P[G_OFF_W0 + i];
+        net.w0t[k * G_HID + o] = (__bf16)P[G_OFF_W0 + i];
+    }
+    if (i < G_HID * G_HID) {
+        int o = i / G_HID, k = i % G_HID;
+        net.w1[i] = (__bf16)P[G_OFF_W1 + i];
+        net.w1t[k * G_HID + o] = (__bf16)P[G_OFF_W1 + i];
+        net.w2[i] = (__bf16)P[G_OFF_W2 + i];
+        net.w2t[k * G_HID + o] = (__bf16)P[G_OFF_W2 + i];
+    }
+    if (i < G_OUT_PAD * G_HID) {
+        int o = i / G_HID, k = i % G_HID;
+        net.w3[i] = (__bf16)P[G_OFF_W3 + i];
+        net.w3t[k * 64 + o] = (__bf16)P[G_OFF_W3 + i];
+    }
+    if (i < G_HID * 16) {
+        int k = i / 16, o = G_OUT_PAD + i % 16;
+        net.w3t[k * 64 + o] = (__bf16)0.0f;
+    }
+    if (i < G_HID) {
+        net.b0[i] = P[G_OFF_B0 + i];
+        net.b1[i] = P[G_OFF_B1 + i];
+        net.b2[i] = P[G_OFF_B2 + i];
+    }
+    if (i < G_OUT_PAD) net.b3[i] = P[G_OFF_B3 + i];
+}
+
+// ---------------------------------------------------------------- inference
+
+constexpr int GI_WAVE_LDS = 64 * G_IN * 2 + 64 * 64 * 2;   // X|Y region (12 KB) + H region (8 KB)
+
+struct InferLds {
+    __bf16 w0[G_HID * G_IN], w1[G_HID * G_HID], w2[G_HID * G_HID], w3[G_OUT_PAD * G_HID];
+    float b[G_HID * 3 + G_OUT_PAD];
+    unsigned char wave[GI_WAVES][GI_WAVE_LDS];
+};
+
+__device__ __forceinline__ void load_net_lds(const GuideNet& net, InferLds& s) {
+    for (int i = threadIdx.x; i < G_HID * G_IN; i += blockDim.x) s.w0[i] = net.w0[i];
+    for (int i = threadIdx.x; i < G_HID * G_HID; i += blockDim.x) {
+        s.w1[i] = net.w1[i];
+        s.w2[i] = net.w2[i];
+    }
+    for (int i = threadIdx.x; i < G_OUT_PAD * G_HID; i += blockDim.x) s.w3[i] = net.w3[i];
+    for (int i = threadIdx.x; i < G_HID; i += blockDim.x) {
+        s.b[i] = net.b0[i];
+        s.b[G_HID + i] = net.b1[i];
+        s.b[2 * G_HID + i] = net.b2[i];
+    }
+    for (int i = threadIdx.x; i < G_OUT_PAD; i += blockDim.x) s.b[3 * G_HID + i] = net.b3[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void zero_row(__bf16* x) {
+    for (int k = 0; k < G_IN; ++k) x[k] = (__bf16)0.0f;
+}
+
+// Raw network outputs for n queries (p, n, wo as 9 floats each): out[q * 40 + o].
+__global__ __launch_bounds__(64 * GI_WAVES) void k_guide_forward(GuideNet net, GuideBounds gb, const float* __restrict__ qin,
+                                                                 uint32_t n, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) InferLds s;
+    load_net_lds(net, s);
+    const int wave = threadIdx.x >> 6, l = __lane_id();
+    __bf16* X = (__bf16*)s.wave[wave];
+    __bf16* H = (__bf16*)(s.wave[wave] + 64 * G_IN * 2);
+    float* Y = (float*)X;
+    const float* sb = s.b;
+    for (uint32_t base = (blockIdx.x * GI_WAVES + wave) * 64u; base < n; base += gridDim.x * GI_WAVES * 64u) {
+        const uint32_t q = base + l;
+        if (q < n) {
+            const float* in = qin + (size_t)q * 9;
+            guide_encode(gb, mk(in[0], in[1], in[2]), mk(in[3], in[4], in[5]), mk(in[6], in[7], in[8]), X + l * G_IN);
+        } else {
+            zero_row(X + l * G_IN);
+        }
+        wave_sync();
+        mlp64(s.w0, s.w1, s.w2, s.w3, sb, sb + G_HID, sb + 2 * G_HID, sb + 3 * G_HID, X, H, Y);
+        if (q < n)
+            for (int o = 0; o < G_OUT; ++o) out[(size_t)q * G_OUT + o] = Y[l * G_OUT_PAD + o];
+        wave_sync();
+    }
+}
+
+// Decoded mixtures for the extension-queue entries of one bounce that the
+// shade kernel may guide (hit, not emissive, not a delta material).
+__global__ __launch_bounds__(64 * GI_WAVES) void k_guide_infer(GuideNet net, GuideBounds gb, WaveParams wp, DevScene sc,
+                                                               PathQueue pq, const uint32_t* __restrict__ count,
+                                                               const float4* __restrict__ hits, float* __restrict__ lobes,
+                                                               uint32_t stride) {
+    __shared__ __attribute__((aligned(16))) InferLds s;
+    load_net_lds(net, s);
+    const uint32_t n = *count;
+    const int wave = threadIdx.x >> 6, l = __lane_id();
+    __bf16* X = (__bf16*)s.wave[wave];
+    __bf16* H = (__bf16*)(s.wave[wave] + 64 * G_IN * 2);
+    float* Y = (float*)X;
+    const float* sb = s.b;
+    for (uint32_t base = (blockIdx.x * GI_WAVES + wave) * 64u; base < n; base += gridDim.x * GI_WAVES * 64u) {
+        const uint32_t i = base + l;
+        bool use = false;
+        if (i < n) {
+            const float4 h = hits[i];
+            const uint32_t slot = __float_as_uint(h.y);
+            if (slot != INVALID) {
+                const uint32_t mat_id = __float_as_uint(sc.tris[(size_t)slot * TRI_F4 + 1].w);
+                const uint32_t mt = sc.mats[mat_id].type;
+                if (mt != XT_MAT_EMISSION && !mat_is_delta(mt)) {
+                    const Surf sf = make_surf(sc, slot, xyz(pq.d[i]), h.z, h.w);
+                    guide_encode(gb, sf.p, sf.sn, sf.wo, X + l * G_IN);
+                    use = true;
+                }
+            }
+        }
+        if (!use) zero_row(X + l * G_IN);
+        // whole waves only: skip the products when no lane has a query
+        if (__ballot(use) != 0) {
+            wave_sync();
+            mlp64(s.w0, s.w1, s.w2, s.w3, sb, sb + G_HID, sb + 2 * G_HID, sb + 3 * G_HID, X, H, Y);
+            if (use) {
+                VmfLobe lb[G_LOBES];
+                guide_decode(Y + l * G_OUT_PAD, lb);
+#pragma unroll
+                for (int j = 0; j < G_LOBES; ++j) {
+                    float* f = lobes + (size_t)(6 * j) * stride + i;
+                    f[0] = lb[j].mu.x;
+                    f[(size_t)stride] = lb[j].mu.y;
+                    f[(size_t)2 * stride] = lb[j].mu.z;
+                    f[(size_t)3 * stride] = lb[j].kappa;
+                    f[(size_t)4 * stride] = lb[j].alpha;
+                    f[(size_t)5 * stride] = lb[j].alpha * vmf_norm(lb[j].kappa);
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- training
+
+// C[m0..m0+15][n0..n0+15] = sum_k A[m][k] B[k][n], A stored [M][K] (lda), B
+// stored transposed [N][K] (ldb); K a multiple of 32.
+template <int K>
+__device__ __forceinline__ floatx4 prod_tile(const __bf16* A, int lda, int m0, const __bf16* Bt, int ldb, int n0) {
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < K / 32; ++c) acc = mfma(frag(A, lda, m0, 32 * c), frag(Bt, ldb, n0, 32 * c), acc);
+    return acc;
+}
+
+// acc[m][n] += C tile for rows m < mlim.  Lanes of the wave touch distinct
+// elements, and the accumulator belongs to this wave alone.
+__device__ __forceinline__ void add_tile(float* acc, int ld, int m0, int n0, floatx4 c, int mlim) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + 4 * g + i;
+        if (m < mlim) acc[m * ld + n0 + r] += c[i];
+    }
+}
+
+// dW[out][in] += dZ^T H over the 64 samples: dZt [OUTP][64], Ht [IN][64].
+template <int OUT, int IN>
+__device__ __forceinline__ void weight_grad(float* acc, const __bf16* dZt, const __bf16* Ht) {
+#pragma unroll 1
+    for (int b = 0; b < (OUT + 15) / 16; ++b)
+#pragma unroll 1
+        for (int t = 0; t < IN / 16; ++t) add_tile(acc, IN, 16 * b, 16 * t, prod_tile<64>(dZt, 64, 16 * b, Ht, 64, 16 * t), OUT);
+}
+
+// db[o] += sum_q dZ[q][o] (lane o; dZt [OUT][64]).
+template <int OUT>
+__device__ __forceinline__ void bias_grad(float* acc, const __bf16* dZt) {
+    const int l = __lane_id();
+    if (l < OUT) {
+        float sum = 0.0f;
+        for (int q = 0; q < 64; ++q) sum += (float)dZt[l * 64 + q];
+        acc[l] += sum;
+    }
+}
+
+// dZ' = (dZ W) masked by H > 0, in both layouts: dZ [64 q][OUTP], W^T
+// stored [IN][OUTP], Ht [IN][64]; writes dZp [64][IN] and dZpt [IN][64].
+template <int IN, int OUTP>
+__device__ __forceinline__ void back_layer(const __bf16* Wt, const __bf16* dZ, const __bf16* Ht, __bf16* dZp,
+                                           __bf16* dZpt) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+#pragma unroll 1
+    for (int b = 0; b < IN / 16; ++b)
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) {
+            floatx4 acc = prod_tile<OUTP>(Wt, OUTP, 16 * b, dZ, OUTP, 16 * t);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int u = 16 * b + 4 * g + i, q = 16 * t + r;
+                const float v = (float)Ht[u * 64 + q] > 0.0f ? acc[i] : 0.0f;
+                dZp[q * IN + u] = (__bf16)v;
+                dZpt[u * 64 + q] = (__bf16)v;
+            }
+        }
+}
+
+struct GradLds {
+    float acc[G_NPARAMS];                 // this block's gradient, flat parameter layout
+    __bf16 xy[64 * G_IN];                 // X [64][96] bf16, later Y [64][48] fp32
+    __bf16 xt[G_IN * 64];                 // X^T
+    __bf16 ht[3][64 * 64];                // H1^T, H2^T, H3^T
+    __bf16 p[2][64 * 64];                 // forward ping-pong, then dZ / dZ^T
+    __bf16 r[2][64 * 64];                 // dZ / dZ^T
+};
+
+// Gradient of sum_i -min(w_i * weight_scale, weight_clip) log q(dir_i | query_i)
+// over the samples.  One wave per block; each block adds its share into LDS
+// and writes it to partial[blockIdx.x][...] (reduced by k_guide_adam).
+__global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* __restrict__ smp,
+                                                         uint32_t n, float weight_scale, float weight_clip,
+                                                         float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) GradLds s;
+    const int l = __lane_id();
+    for (int i = l; i < G_NPARAMS; i += 64) s.acc[i] = 0.0f;
+    __bf16* X = s.xy;
+    float* Y = (float*)s.xy;
+    __bf16 *Ha = s.p[0], *Hb = s.p[1];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t si = base + l;
+        const bool valid = si < n;
+        if (valid) {
+            const float* qq = smp[si].q;
+            guide_encode(gb, mk(qq[0], qq[1], qq[2]), mk(qq[3], qq[4], qq[5]), mk(qq[6], qq[7], qq[8]), X + l * G_IN);
+        } else {
+            zero_row(X + l * G_IN);
+        }
+        for (int k = 0; k < G_IN; ++k) s.xt[k * 64 + l] = X[l * G_IN + k];
+        wave_sync();
+        // forward, keeping H^T of every hidden layer
+        dense64<G_IN, G_HID, true, false>(net.w0, net.b0, X, Ha, s.ht[0]);
+        wave_sync();
+        dense64<G_HID, G_HID, true, false>(net.w1, net.b1, Ha, Hb, s.ht[1]);
+        wave_sync();
+        dense64<G_HID, G_HID, true, false>(net.w2, net.b2, Hb, Ha, s.ht[2]);
+        wave_sync();
+        dense64<G_HID, G_OUT_PAD, false, true>(net.w3, net.b3, Ha, Y);
+        wave_sync();
+        // dL/dy -> dY [64][64] and dY^T (outputs >= 40 and padding lanes zero)
+        __bf16 *dY = s.r[0], *dYt = s.r[1];
+        {
+            for (int o = 0; o < 64; ++o) {
+                dY[l * 64 + o] = (__bf16)0.0f;
+                dYt[o * 64 + l] = (__bf16)0.0f;
+            }
+            if (valid) {
+                const GuideSample& sm = smp[si];
+                const float w = fminf(sm.weight * weight_scale, weight_clip);
+                guide_loss_grad(Y + l * G_OUT_PAD, mk(sm.dir[0], sm.dir[1], sm.dir[2]), w, [&](int o, float v) {
+                    dY[l * 64 + o] = (__bf16)v;
+                    dYt[o * 64 + l] = (__bf16)v;
+                });
+            }
+        }
+        wave_sync();
+        weight_grad<G_OUT, G_HID>(s.acc + G_OFF_W3, dYt, s.ht[2]);
+        bias_grad<G_OUT>(s.acc + G_OFF_B3, dYt);
+        back_layer<G_HID, 64>(net.w3t, dY, s.ht[2], s.p[0], s.p[1]);     // dZ3
+        wave_sync();
+        weight_grad<G_HID, G_HID>(s.acc + G_OFF_W2, s.p[1], s.ht[1]);
+        bias_grad<G_HID>(s.acc + G_OFF_B2, s.p[1]);
+        back_layer<G_HID, G_HID>(net.w2t, s.p[0], s.ht[1], s.r[0], s.r[1]);   // dZ2
+        wave_sync();
+        weight_grad<G_HID, G_HID>(s.acc + G_OFF_W1, s.r[1], s.ht[0]);
+        bias_grad<G_HID>(s.acc + G_OFF_B1, s.r[1]);
+        back_layer<G_HID, G_HID>(net.w1t, s.r[0], s.ht[0], s.p[0], s.p[1]);   // dZ1
+        wave_sync();
+        weight_grad<G_HID, G_IN>(s.acc + G_OFF_W0, s.p[1], s.xt);
+        bias_grad<G_HID>(s.acc + G_OFF_B0, s.p[1]);
+        wave_sync();
+    }
+    wave_sync();
+    float* out = partial + (size_t)blockIdx.x * G_NPARAMS;
+    for (int i = l; i < G_NPARAMS; i += 64) out[i] = s.acc[i];
+}
+
+// Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the
+// in-order sum of nparts partial rows times grad_scale; b1t, b2t = beta^t.
+__global__ void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                             const float* __restrict__ partial, uint32_t nparts, float lr, float b1, float b2, float eps,
+                             float b1t, float b2t, float grad_scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G_NPARAMS) return;
+    float g = 0.0f;
+    for (uint32_t b = 0; b < nparts; ++b) g += partial[(size_t)b * G_NPARAMS + i];
+    g *= grad_scale;
+    if (!isfinite(g)) g = 0.0f;
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
+    P[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Training records -> samples: incident radiance along the sampled direction
+// (L_path - pre) / T, weight = luminance / pmix; keeps positive finite
+// weights, appended to out (order not fixed), and sums the weights.
+__global__ void k_guide_finalize(const TrainRec* __restrict__ recs, uint32_t n, const float4* __restrict__ radiance,
+                                 GuideSample* __restrict__ out, uint32_t* __restrict__ count, float* __restrict__ weight_sum) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float w = 0.0f;
+    bool keep = false;
+    if (i < n && recs[i].valid) {
+        const TrainRec& r = recs[i];
+        const float4 L = radiance[r.id];
+        float Li[3] = {L.x - r.pre[0], L.y - r.pre[1], L.z - r.pre[2]};
+        for (int c = 0; c < 3; ++c) Li[c] = r.T[c] > 0.0f ? fmaxf(Li[c], 0.0f) / r.T[c] : 0.0f;
+        const float lum = 0.2126f * Li[0] + 0.7152f * Li[1] + 0.0722f * Li[2];
+        w = lum / r.pmix;
+        keep = r.pmix > 0.0f && w > 0.0f && isfinite(w);
+    }
+    const uint64_t mask = __ballot(keep);
+    if (mask == 0) return;
+    const int l = __lane_id();
+    const uint32_t nk = (uint32_t)__popcll(mask);
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << l) - 1ull));
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    float wsum = keep ? w : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
+    uint32_t base = 0;
+    if (l == leader) {
+        base = atomicAdd(count, nk);
+        atomicAdd(weight_sum, wsum);
+    }
+    base = __shfl(base, leader);
+    if (keep) {
+        const TrainRec& r = recs[i];
+        GuideSample& o = out[base + rank];
+        for (int k = 0; k < 9; ++k) o.q[k] = r.q[k];
+        for (int k = 0; k < 3; ++k) o.dir[k] = r.dir[k];
+        o.weight = w;
+    }
+}
+
+}  // namespace xt

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "xt_device.h"
+#include "xt_guide.h"
 
 namespace xt {
 
@@ -38,11 +39,14 @@ struct WaveParams {
 //   d = (direction, path id)
 //   w = (path weight, vertex index v | flags << 24)   v = vertex the ray will hit (1 = camera hit)
 //   e = (material_eval of the sample, env-light pdf of the direction)
+//   rec = training-record index of the vertex that sampled the ray (INVALID: none);
+//         only touched while training records are kept
 struct PathQueue {
     float4* o;
     float4* d;
     float4* w;
     float4* e;
+    uint32_t* rec;
 };
 constexpr uint32_t QF_PENDING = 1u;   // the previous vertex's NEE result waits in pending[id]
 
@@ -54,15 +58,46 @@ struct ShadowQueue {
     float4* c;
 };
 
+// Per-vertex training record written while rendering: the query, the
+// sampled direction and its mixture pdf, the path throughput after the
+// vertex (T) and the path radiance collected up to and including the
+// vertex's own light sample (pre).  After the path ends, the incident
+// radiance along dir is (L_path - pre) / T per channel.
+struct TrainRec {
+    float q[9];
+    float dir[3];
+    float pmix;
+    float T[3];
+    float pre[3];
+    uint32_t id;      // path id
+    uint32_t valid;   // 1 = guided vertex
+    float pad;
+};
+
+// Decoded mixture per path-queue entry, SoA: field f of entry i at
+// lobes[f * stride + i]; per lobe j fields 6j.. = mu.x, mu.y, mu.z, kappa,
+// alpha, alpha * vmf_norm(kappa).
+constexpr int G_LOBE_FIELDS = 6 * G_LOBES;
+
+struct GuideRender {
+    const float* lobes;    // null: guiding off
+    uint32_t stride;
+    float prob;            // probability of sampling the guide (one-sample MIS with the BSDF)
+    TrainRec* recs;        // null: no training records
+    uint32_t rec_base;     // record index of this bounce's first queue entry
+    uint32_t rec_cap;
+};
+
 template <bool ANY, int LDS_DEPTH, bool COUNT>
 __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, int ray_stride,
                         const uint32_t* count_ptr, uint32_t count_fixed, float4* hits, uint8_t* occluded,
                         int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
+template <bool GUIDE>
 __global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_t* count_in, const float4* hits,
                         PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
-                        float4* radiance, float4* pending);
+                        float4* radiance, float4* pending, GuideRender gr);
 template <bool COUNT>
 __global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending,
                          uint32_t* ovf, unsigned long long* visit_totals);

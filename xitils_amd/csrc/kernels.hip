@@ -170,44 +170,17 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, fl
         q.d[id] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
         q.w[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));
         q.e[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        if (q.rec) q.rec[id] = INVALID;
         radiance[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
-// SurfaceIntersection at a hit (TriangleIndexed.h:133-172, then the identity
-// pass of Object.h:106-144 where Transform::asNormal normalises n, wo and
-// shading.n once more).
-__device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 d, float b0, float b1) {
-    const float4* tr = sc.tris + (size_t)slot * TRI_F4;
-    float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
-    f3 p0 = xyz(v0), p1 = xyz(v1), p2 = xyz(v2);
-    Surf s;
-    s.slot = slot;
-    s.mat = __float_as_uint(v1.w);
-    uint32_t flags = __float_as_uint(v2.w);
-    s.p = lerp3(p0, p1, p2, b0, b1);
-    f3 wo = normalize(-d);
-    f3 n;
-    if (flags & TRI_FLAG_NORMALS) {
-        const float4* nr = sc.normals + (size_t)slot * TRI_F4;
-        n = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), b0, b1));
-    } else {
-        n = normalize(cross(p1 - p0, p2 - p0));
-    }
-    f3 sn = face_forward(n, wo);
-    s.n = normalize(n);
-    s.wo = normalize(wo);
-    s.sn = normalize(sn);
-    return s;
-}
-
-__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-
+template <bool GUIDE>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
                                                const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
                                                PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
                                                uint32_t* __restrict__ count_shadow, float4* __restrict__ radiance,
-                                               float4* __restrict__ pending) {
+                                               float4* __restrict__ pending, GuideRender gr) {
     const uint32_t n = *count_in;
     const bool standard = wp.integrator == XT_INTEGRATOR_STANDARD;
     const bool can_sample = sc.num_lights > 0;
@@ -224,6 +197,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
         f3 ext_o = splat(0.0f), ext_d = splat(0.0f), ext_w = splat(0.0f), ext_e = splat(0.0f);
         float ext_pdf = 0.0f, ext_pe = 0.0f;
         uint32_t ext_v = 0;
+        bool guided = false;
+        Surf s;
         f3 sh_o = splat(0.0f), sh_d = splat(0.0f), sh_c = splat(0.0f);
         float sh_tmax = 0.0f;
         uint32_t id = 0;
@@ -236,9 +211,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
             const f3 d = xyz(d4), meval = xyz(e4);
             f3 weight = xyz(w4);
             f3 rad = xyz(radiance[id]);
+            const f3 pend = (qflags & QF_PENDING) ? xyz(pending[id]) : splat(0.0f);
+            if (GUIDE && gr.recs) {
+                // the vertex that sampled this ray: radiance up to and including its light sample
+                const uint32_t rin = qin.rec[i];
+                if (rin != INVALID) {
+                    TrainRec& rr = gr.recs[rin];
+                    const f3 pre = rad + pend;
+                    rr.pre[0] = pre.x;
+                    rr.pre[1] = pre.y;
+                    rr.pre[2] = pre.z;
+                }
+            }
             const uint32_t slot = __float_as_uint(h.y);
             const bool hit = slot != INVALID;
-            Surf s;
             xt_material mat;
             bool emissive = false;
             if (hit) {
@@ -284,7 +270,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     rad += weight * emitted(mat, -d, s.n);
                 }
             }
-            if (qflags & QF_PENDING) rad += xyz(pending[id]);  // light sample of the previous vertex
+            if (qflags & QF_PENDING) rad += pend;  // light sample of the previous vertex
             radiance[id] = f4(rad, 0.0f);
 
             bool live = hit && !emissive;
@@ -311,8 +297,32 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     const Frame fs = make_frame(s.sn);   // BasisVectors(shading.n), shared by both samplers
                     f3 wi = splat(0.0f);
                     float pdfB = 0.0f;
-                    f3 me = bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
+                    f3 me;
+                    // Guided vertex: one-sample MIS of the learned mixture (probability
+                    // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
+                    // pdf wherever the BSDF pdf is positive, and zero elsewhere.
+                    guided = GUIDE && gr.lobes != nullptr && !mat_is_delta(mat.type);
+                    auto lobe = [&](int f) { return gr.lobes[(size_t)f * gr.stride + i]; };
+                    auto strategy_pdf = [&](f3 dir, float pb) {
+                        return pb > 0.0f ? gr.prob * guide_pdf_fields(lobe, dir) + (1.0f - gr.prob) * pb : 0.0f;
+                    };
+                    if (guided) {
+                        const uint4 blk3 = rng.block(v, 3);
+                        if (Rng::to_unit(blk3.x) < gr.prob) {
+                            wi = guide_sample_fields(lobe, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z), Rng::to_unit(blk3.w));
+                        } else {
+                            bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
                                         &pdfB);
+                        }
+                        float pb;
+                        const f3 fc = bsdf_eval(mat, s, wi, &pb);
+                        pdfB = strategy_pdf(wi, pb);
+                        me = pdfB > 0.0f ? div(fc, pdfB) : splat(0.0f);
+                        if (!finite3(me)) me = splat(0.0f);
+                    } else {
+                        me = bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
+                                         &pdfB);
+                    }
                     if (!is_zero(me)) {                            // PathTracer.h:165-167
                         push_ext = true;
                         ext_d = wi;
@@ -335,6 +345,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                             float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
                             float pb;
                             f3 fc = bsdf_eval(mat, s, ld, &pb);
+                            if (guided) pb = strategy_pdf(ld, pb);
                             if (pe > 0.0f && !is_zero(fc)) {
                                 float mis = (pe * pe) / (pb * pb + pe * pe);
                                 f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
@@ -385,6 +396,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                             float mis;
                             float pb;
                             f3 fc = bsdf_eval(mat, s, sd, &pb);
+                            if (guided) pb = strategy_pdf(sd, pb);
                             if (pdfB >= 0.0f) {
                                 float cosL = fabsf(dot(-sd, lsn));
                                 pb *= cosL / distSq;
@@ -438,6 +450,23 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
             qout.d[ei] = f4(ext_d, __uint_as_float(id));
             qout.w[ei] = f4(ext_w, __uint_as_float(ext_v | (fl << 24)));
             qout.e[ei] = f4(ext_e, ext_pe);
+            if (GUIDE && gr.recs) {
+                const uint32_t ri = gr.rec_base + ei;
+                const bool keep = ri < gr.rec_cap && ri >= gr.rec_base;
+                qout.rec[ei] = keep ? ri : INVALID;
+                if (keep) {
+                    TrainRec& rr = gr.recs[ri];
+                    const f3 T = ext_w * ext_e;
+                    rr.q[0] = s.p.x; rr.q[1] = s.p.y; rr.q[2] = s.p.z;
+                    rr.q[3] = s.sn.x; rr.q[4] = s.sn.y; rr.q[5] = s.sn.z;
+                    rr.q[6] = s.wo.x; rr.q[7] = s.wo.y; rr.q[8] = s.wo.z;
+                    rr.dir[0] = ext_d.x; rr.dir[1] = ext_d.y; rr.dir[2] = ext_d.z;
+                    rr.pmix = ext_pdf;
+                    rr.T[0] = T.x; rr.T[1] = T.y; rr.T[2] = T.z;
+                    rr.id = id;
+                    rr.valid = guided ? 1u : 0u;
+                }
+            }
         }
         if (push_sh) {
             sq.o[si] = f4(sh_o, sh_tmax);
@@ -446,6 +475,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
         }
     }
 }
+
+template __global__ void k_shade<false>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue,
+                                        uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
+template __global__ void k_shade<true>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue,
+                                       uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
 
 // Any-hit for queued shadow rays; unoccluded contributions go straight to the
 // path radiance, or to pending[] when the path continues (see k_shade).

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "guide.h"
 #include "kernels.h"
 
 using namespace xt;
@@ -56,6 +57,8 @@ struct RenderBuffers {
     float4* hits = nullptr;
     float4* radiance = nullptr;
     float4* pending = nullptr;
+    uint32_t* rec[2] = {};           // training-record index per path-queue entry (allocated on demand)
+    float* lobes = nullptr;          // decoded guide mixtures per path-queue entry (allocated on demand)
     uint32_t* counts = nullptr;      // [0], [1]: path queues, [2]: shadow queue
     uint32_t* host_counts = nullptr; // pinned
     unsigned long long* visits = nullptr;  // node, tri fetches: extension [0..1], shadow [2..3]
@@ -65,8 +68,37 @@ struct RenderBuffers {
         }
         hipFree(sq.o); hipFree(sq.d); hipFree(sq.c);
         hipFree(hits); hipFree(radiance); hipFree(pending); hipFree(counts); hipFree(visits);
+        hipFree(rec[0]); hipFree(rec[1]); hipFree(lobes);
         if (host_counts) hipHostFree(host_counts);
         *this = RenderBuffers();
+    }
+};
+
+// Guiding network and training state of a scene.
+struct GuideState {
+    bool ready = false;
+    xt_guide_config cfg{};
+    GuideBounds gb{};
+    float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    float* P = nullptr;              // fp32 master parameters
+    float* m = nullptr;              // Adam moments
+    float* v = nullptr;
+    float* partial = nullptr;        // per-block gradient rows of k_guide_grad
+    uint32_t partial_rows = 0;
+    void* net_mem = nullptr;         // bf16 weight copies + fp32 biases
+    GuideNet net{};
+    uint64_t step = 0;               // Adam steps taken
+    TrainRec* recs = nullptr;
+    GuideSample* samples = nullptr;
+    uint32_t rec_cap = 0;
+    uint32_t* d_count = nullptr;     // finalised samples
+    float* d_wsum = nullptr;         // their weight sum
+    float* host_io = nullptr;        // pinned: count + weight sum read-back
+    void release() {
+        hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem);
+        hipFree(recs); hipFree(samples); hipFree(d_count); hipFree(d_wsum);
+        if (host_io) hipHostFree(host_io);
+        *this = GuideState();
     }
 };
 
@@ -97,6 +129,8 @@ struct xt_scene {
     float4* d_qhits = nullptr;
     uint8_t* d_qocc = nullptr;
     size_t qout_cap = 0;
+    float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
+    GuideState guide;
 };
 
 extern "C" {
@@ -137,6 +171,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
     hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc);
     s->rb.release();
+    s->guide.release();
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return XT_OK;
@@ -340,6 +375,16 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     s->bvh_depth = bvh.depth;
     s->bytes_nodes = (uint64_t)bvh.num_nodes * 128;
     s->bytes_tris = (uint64_t)nprim * 48;
+    for (int a = 0; a < 3; ++a) {
+        s->bounds_lo[a] = INFINITY;
+        s->bounds_hi[a] = -INFINITY;
+    }
+    for (const BuildTri& t : bt)
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) {
+                s->bounds_lo[a] = std::min(s->bounds_lo[a], t.p[k][a]);
+                s->bounds_hi[a] = std::max(s->bounds_hi[a], t.p[k][a]);
+            }
     s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     *out = s;
     return XT_OK;
@@ -434,7 +479,7 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 struct StageTimer {
     hipStream_t st;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];   // extend, shade, shadow, other, guide infer, guide train
     hipEvent_t mark() {
         hipEvent_t e;
         hipEventCreate(&e);
@@ -481,6 +526,102 @@ static uint32_t path_capacity() {
                                             std::to_string(__LINE__) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// ---------------------------------------------------------------- guiding (host side)
+
+static uint64_t splitmix64(uint64_t& x) {   // Steele, Lea, Flood: SplitMix64
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static void guide_init_params(uint32_t seed, std::vector<float>& P) {
+    P.assign(G_NPARAMS, 0.0f);
+    uint64_t st = 0x5EEDull * 0x100000001ull + seed;
+    auto uni = [&](float bound) { return bound * (2.0f * (float)((splitmix64(st) >> 40) * 0x1.0p-24) - 1.0f); };
+    for (int i = 0; i < G_HID * G_IN; ++i) P[G_OFF_W0 + i] = uni(std::sqrt(6.0f / G_IN));
+    for (int i = 0; i < G_HID * G_HID; ++i) P[G_OFF_W1 + i] = uni(std::sqrt(6.0f / G_HID));
+    for (int i = 0; i < G_HID * G_HID; ++i) P[G_OFF_W2 + i] = uni(std::sqrt(6.0f / G_HID));
+    for (int i = 0; i < G_OUT * G_HID; ++i) P[G_OFF_W3 + i] = uni(0.1f * std::sqrt(6.0f / G_HID));
+}
+
+static xt_status guide_prepare(xt_scene* s) {
+    GuideState& g = s->guide;
+    hipLaunchKernelGGL(k_guide_prepare, dim3((G_HID * G_IN + 255) / 256), dim3(256), 0, s->stream, g.P, g.net);
+    if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "k_guide_prepare launch failed");
+    return XT_OK;
+}
+
+static xt_status guide_need(xt_scene* s) {
+    if (!s) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!s->guide.ready) return fail(XT_ERR_INVALID_ARGUMENT, "guiding not initialised (xt_guide_init)");
+    XT_HIP(hipSetDevice(s->dev->ordinal));
+    return XT_OK;
+}
+
+static uint32_t guide_grad_blocks(const xt_scene* s, uint32_t n) {
+    return std::max(1u, std::min((n + 63) / 64, s->guide.partial_rows));
+}
+
+// Adam steps over d_samples[0..n) in minibatches, weights scaled by weight_scale.
+static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t n, float weight_scale,
+                             uint32_t max_steps, uint64_t* steps_taken) {
+    GuideState& g = s->guide;
+    const uint32_t M = std::max(64u, g.cfg.minibatch);
+    const uint32_t steps = std::min<uint32_t>((n + M - 1) / M, std::max(1u, max_steps));
+    for (uint32_t k = 0; k < steps; ++k) {
+        const uint32_t off = k * M, cnt = std::min(M, n - off);
+        const uint32_t grid = guide_grad_blocks(s, cnt);
+        hipLaunchKernelGGL(k_guide_grad, dim3(grid), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, d_samples + off, cnt,
+                           weight_scale, g.cfg.weight_clip, g.partial);
+        g.step++;
+        const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
+        const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
+        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 255) / 256), dim3(256), 0, s->stream, g.P, g.m, g.v, g.partial,
+                           grid, g.cfg.learning_rate, g.cfg.beta1, g.cfg.beta2, g.cfg.adam_eps, b1t, b2t,
+                           1.0f / (float)cnt);
+        xt_status st = guide_prepare(s);
+        if (st != XT_OK) return st;
+        if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "guide training launch failed");
+    }
+    if (steps_taken) *steps_taken += steps;
+    return XT_OK;
+}
+
+static xt_status guide_ensure_training(xt_scene* s) {
+    GuideState& g = s->guide;
+    if (g.recs) return XT_OK;
+    g.rec_cap = std::max(1024u, g.cfg.max_records);
+    XT_HIP(dalloc(&g.recs, g.rec_cap));
+    XT_HIP(dalloc(&g.samples, g.rec_cap));
+    return XT_OK;
+}
+
+// Host samples -> device GuideSample array (grows a scratch buffer).
+static xt_status guide_upload_samples(xt_scene* s, const float* queries, const float* dirs, const float* weights,
+                                      uint32_t n, GuideSample** out) {
+    GuideState& g = s->guide;
+    if (g.rec_cap < n || !g.samples) {
+        hipFree(g.recs);
+        hipFree(g.samples);
+        g.recs = nullptr;
+        g.samples = nullptr;
+        g.rec_cap = std::max(n, g.cfg.max_records);
+        XT_HIP(dalloc(&g.recs, g.rec_cap));
+        XT_HIP(dalloc(&g.samples, g.rec_cap));
+    }
+    std::vector<GuideSample> hs(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 9; ++k) hs[i].q[k] = queries[9 * (size_t)i + k];
+        for (int k = 0; k < 3; ++k) hs[i].dir[k] = dirs[3 * (size_t)i + k];
+        hs[i].weight = weights[i];
+    }
+    XT_HIP(hipMemcpyAsync(g.samples, hs.data(), n * sizeof(GuideSample), hipMemcpyHostToDevice, s->stream));
+    XT_HIP(hipStreamSynchronize(s->stream));
+    *out = g.samples;
+    return XT_OK;
+}
+
 xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* film, int film_f64,
                     int film_on_device) {
     if (!s || !cam || !p || !film) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -488,6 +629,11 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     if (p->integrator > XT_INTEGRATOR_NAIVE) return fail(XT_ERR_INVALID_ARGUMENT, "unknown integrator");
     if ((uint64_t)p->width * p->height >= (1ull << 31)) return fail(XT_ERR_TOO_LARGE, "image too large");
     if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(XT_ERR_INVALID_ARGUMENT, "rr_prob must be in (0, 1]");
+    const bool guide = (p->flags & XT_FLAG_GUIDE) != 0;
+    const bool train = guide && (p->flags & XT_FLAG_GUIDE_TRAIN) != 0;
+    if (guide && p->integrator != XT_INTEGRATOR_STANDARD)
+        return fail(XT_ERR_INVALID_ARGUMENT, "guiding needs the standard integrator");
+    if (guide && !s->guide.ready) return fail(XT_ERR_INVALID_ARGUMENT, "guiding not initialised (xt_guide_init)");
     XT_HIP(hipSetDevice(s->dev->ordinal));
     const uint64_t npix_all = (uint64_t)p->width * p->height;
     const size_t film_bytes = npix_all * 3 * (film_f64 ? 8 : 4);
@@ -496,6 +642,18 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix_all * p->spp);
     xt_status st = ensure_buffers(s, cap);
     if (st != XT_OK) return st;
+    RenderBuffers& rbg = s->rb;
+    if (guide && !rbg.lobes) XT_HIP(dalloc(&rbg.lobes, (size_t)G_LOBE_FIELDS * rbg.capacity));
+    if (train) {
+        if (!rbg.rec[0]) {
+            XT_HIP(dalloc(&rbg.rec[0], rbg.capacity));
+            XT_HIP(dalloc(&rbg.rec[1], rbg.capacity));
+        }
+        st = guide_ensure_training(s);
+        if (st != XT_OK) return st;
+    }
+    for (int k = 0; k < 2; ++k) rbg.q[k].rec = train ? rbg.rec[k] : nullptr;
+    GuideState& g = s->guide;
     void* d_film = film;
     if (!film_on_device) {
         XT_HIP(hipMalloc(&d_film, film_bytes));
@@ -547,6 +705,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             s->stats.camera_rays += wp.num_paths;
             uint32_t live = wp.num_paths;
             int cur = 0;
+            uint32_t rec_base = 0;
             while (live > 0) {
                 s->stats.extension_rays += live;
                 s->stats.extend_launches++;
@@ -566,10 +725,33 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 XT_LAUNCHED(result);
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
                 XT_LAUNCHED(result);
+                GuideRender gr{};
+                if (guide) {
+                    a = tm.mark();
+                    const uint32_t ib = std::max(1u, std::min((live + 64 * GI_WAVES - 1) / (64 * GI_WAVES),
+                                                              (uint32_t)s->dev->num_cu));
+                    hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(64 * GI_WAVES), 0, s->stream, g.net, g.gb, wp, s->ds,
+                                       rb.q[cur], rb.counts + cur, rb.hits, rb.lobes, rb.capacity);
+                    XT_LAUNCHED(result);
+                    tm.add(4, a, tm.mark());
+                    gr.lobes = rb.lobes;
+                    gr.stride = rb.capacity;
+                    gr.prob = g.cfg.guide_prob;
+                    if (train) {
+                        gr.recs = g.recs;
+                        gr.rec_base = rec_base;
+                        gr.rec_cap = g.rec_cap;
+                    }
+                }
                 a = tm.mark();
-                hipLaunchKernelGGL(k_shade, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
-                                   rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
-                                   rb.radiance, rb.pending);
+                if (guide)
+                    hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
+                                       rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
+                                       rb.counts + 2, rb.radiance, rb.pending, gr);
+                else
+                    hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
+                                       rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
+                                       rb.counts + 2, rb.radiance, rb.pending, gr);
                 XT_LAUNCHED(result);
                 b = tm.mark();
                 tm.add(1, a, b);
@@ -579,6 +761,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     break;
                 }
                 const uint32_t next = rb.host_counts[1 - cur], nsh = rb.host_counts[2];
+                if (train) rec_base = (uint32_t)std::min<uint64_t>((uint64_t)rec_base + next, g.rec_cap);
                 if (nsh > 0) {
                     s->stats.shadow_rays += nsh;
                     a = tm.mark();
@@ -599,6 +782,31 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             hipLaunchKernelGGL(k_accumulate, dim3(blocks256(np)), dim3(256), 0, s->stream, wp, rb.radiance, d_film, film_f64);
             XT_LAUNCHED(result);
             tm.add(3, a, tm.mark());
+            if (train && rec_base > 0 && result == XT_OK) {
+                // finished paths -> training samples -> Adam steps
+                a = tm.mark();
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, g.d_count, 1u);
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, (uint32_t*)g.d_wsum, 1u);
+                hipLaunchKernelGGL(k_guide_finalize, dim3((rec_base + 255) / 256), dim3(256), 0, s->stream, g.recs, rec_base,
+                                   rb.radiance, g.samples, g.d_count, g.d_wsum);
+                XT_LAUNCHED(result);
+                hipMemcpyAsync(g.host_io, g.d_count, 4, hipMemcpyDeviceToHost, s->stream);
+                hipMemcpyAsync(g.host_io + 1, g.d_wsum, 4, hipMemcpyDeviceToHost, s->stream);
+                if (hipStreamSynchronize(s->stream) != hipSuccess) {
+                    result = fail(XT_ERR_DEVICE, "guide finalisation failed");
+                    break;
+                }
+                uint32_t ns;
+                std::memcpy(&ns, g.host_io, 4);
+                const float wsum = g.host_io[1];
+                if (ns > 0 && wsum > 0.0f && std::isfinite(wsum)) {
+                    xt_status ts = guide_train(s, g.samples, ns, (float)ns / wsum, g.cfg.max_steps_per_batch,
+                                               &s->stats.guide_steps);
+                    if (ts != XT_OK && result == XT_OK) result = ts;
+                    s->stats.guide_samples += ns;
+                }
+                tm.add(5, a, tm.mark());
+            }
         }
     }
     hipEvent_t t_end = tm.mark();
@@ -613,6 +821,8 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
         s->stats.ms_shade = tm.total(1);
         s->stats.ms_shadow = tm.total(2);
         s->stats.ms_other = tm.total(3);
+        s->stats.ms_guide_infer = tm.total(4);
+        s->stats.ms_guide_train = tm.total(5);
         if (s->counting) {
             unsigned long long v[4] = {0, 0, 0, 0};
             if (hipMemcpy(v, rb.visits, sizeof v, hipMemcpyDeviceToHost) == hipSuccess) {
@@ -644,6 +854,190 @@ xt_status xt_scene_set_counting(xt_scene* s, int on) {
 xt_status xt_render_get_stats(const xt_scene* s, xt_render_stats* out) {
     if (!s || !out) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = s->stats;
+    return XT_OK;
+}
+
+void xt_guide_config_default(xt_guide_config* c) {
+    if (!c) return;
+    c->guide_prob = 0.5f;
+    c->learning_rate = 1e-2f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.999f;
+    c->adam_eps = 1e-8f;
+    c->minibatch = 65536;
+    c->max_steps_per_batch = 32;
+    c->weight_clip = 20.0f;
+    c->seed = 1;
+    c->max_records = 1u << 23;
+}
+
+xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
+    if (!s) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    xt_guide_config c;
+    xt_guide_config_default(&c);
+    if (cfg) c = *cfg;
+    if (!(c.guide_prob >= 0.0f && c.guide_prob < 1.0f))
+        return fail(XT_ERR_INVALID_ARGUMENT, "guide_prob must be in [0, 1)");
+    if (!(c.learning_rate > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f))
+        return fail(XT_ERR_INVALID_ARGUMENT, "bad Adam settings");
+    XT_HIP(hipSetDevice(s->dev->ordinal));
+    GuideState& g = s->guide;
+    g.release();
+    g.cfg = c;
+    for (int a = 0; a < 3; ++a) {
+        float lo = s->bounds_lo[a], hi = s->bounds_hi[a];
+        if (!(hi > lo)) {   // empty or flat scene
+            lo = std::isfinite(lo) ? lo - 0.5f : -0.5f;
+            hi = lo + 1.0f;
+        }
+        g.lo[a] = lo;
+        g.hi[a] = hi;
+    }
+    g.gb.lo = mk(g.lo[0], g.lo[1], g.lo[2]);
+    g.gb.inv_ext = mk(1.0f / (g.hi[0] - g.lo[0]), 1.0f / (g.hi[1] - g.lo[1]), 1.0f / (g.hi[2] - g.lo[2]));
+    g.partial_rows = (uint32_t)s->dev->num_cu;
+    xt_status st = XT_OK;
+    auto chk = [&](hipError_t e) {
+        if (e != hipSuccess && st == XT_OK)
+            st = fail(e == hipErrorOutOfMemory ? XT_ERR_OUT_OF_MEMORY : XT_ERR_DEVICE,
+                      std::string("guide allocation failed: ") + hipGetErrorString(e));
+    };
+    chk(dalloc(&g.P, G_NPARAMS));
+    chk(dalloc(&g.m, G_NPARAMS));
+    chk(dalloc(&g.v, G_NPARAMS));
+    chk(dalloc(&g.partial, (size_t)g.partial_rows * G_NPARAMS));
+    chk(dalloc(&g.d_count, 1));
+    chk(dalloc(&g.d_wsum, 1));
+    chk(hipHostMalloc((void**)&g.host_io, 2 * sizeof(float), hipHostMallocDefault));
+    // bf16 copies and biases in one allocation, 256-byte aligned pieces
+    const size_t sizes[12] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
+                              2 * G_IN * G_HID, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_HID * 64,
+                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD};
+    size_t off[12], total = 0;
+    for (int k = 0; k < 12; ++k) {
+        off[k] = total;
+        total += (sizes[k] + 255) / 256 * 256;
+    }
+    chk(hipMalloc(&g.net_mem, total));
+    if (st != XT_OK) {
+        g.release();
+        return st;
+    }
+    char* b = (char*)g.net_mem;
+    g.net.w0 = (__bf16*)(b + off[0]); g.net.w1 = (__bf16*)(b + off[1]);
+    g.net.w2 = (__bf16*)(b + off[2]); g.net.w3 = (__bf16*)(b + off[3]);
+    g.net.w0t = (__bf16*)(b + off[4]); g.net.w1t = (__bf16*)(b + off[5]);
+    g.net.w2t = (__bf16*)(b + off[6]); g.net.w3t = (__bf16*)(b + off[7]);
+    g.net.b0 = (float*)(b + off[8]); g.net.b1 = (float*)(b + off[9]);
+    g.net.b2 = (float*)(b + off[10]); g.net.b3 = (float*)(b + off[11]);
+    g.ready = true;
+    std::vector<float> P;
+    guide_init_params(c.seed, P);
+    return xt_guide_set_params(s, P.data());
+}
+
+xt_status xt_guide_get_params(xt_scene* s, float* params) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (!params) return fail(XT_ERR_INVALID_ARGUMENT, "params is NULL");
+    XT_HIP(hipStreamSynchronize(s->stream));
+    XT_HIP(hipMemcpy(params, s->guide.P, G_NPARAMS * sizeof(float), hipMemcpyDeviceToHost));
+    return XT_OK;
+}
+
+xt_status xt_guide_set_params(xt_scene* s, const float* params) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (!params) return fail(XT_ERR_INVALID_ARGUMENT, "params is NULL");
+    GuideState& g = s->guide;
+    XT_HIP(hipStreamSynchronize(s->stream));
+    XT_HIP(hipMemcpy(g.P, params, G_NPARAMS * sizeof(float), hipMemcpyHostToDevice));
+    XT_HIP(hipMemset(g.m, 0, G_NPARAMS * sizeof(float)));
+    XT_HIP(hipMemset(g.v, 0, G_NPARAMS * sizeof(float)));
+    g.step = 0;
+    st = guide_prepare(s);
+    if (st != XT_OK) return st;
+    XT_HIP(hipStreamSynchronize(s->stream));
+    return XT_OK;
+}
+
+xt_status xt_guide_get_bounds(xt_scene* s, float lo[3], float hi[3]) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (!lo || !hi) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = s->guide.lo[a];
+        hi[a] = s->guide.hi[a];
+    }
+    return XT_OK;
+}
+
+xt_status xt_guide_forward(xt_scene* s, const float* queries, uint32_t n, float* outputs) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (n == 0) return XT_OK;
+    if (!queries || !outputs) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    float *dq = nullptr, *dout = nullptr;
+    XT_HIP(dalloc(&dq, (size_t)n * 9));
+    hipError_t e = dalloc(&dout, (size_t)n * G_OUT);
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, queries, (size_t)n * 9 * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        const uint32_t grid = std::max(1u, std::min((n + 64 * GI_WAVES - 1) / (64 * GI_WAVES), (uint32_t)s->dev->num_cu));
+        hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(64 * GI_WAVES), 0, s->stream, s->guide.net, s->guide.gb, dq,
+                           n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(outputs, dout, (size_t)n * G_OUT * 4, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(dq);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("xt_guide_forward: ") + hipGetErrorString(e));
+    return XT_OK;
+}
+
+xt_status xt_guide_gradient(xt_scene* s, const float* queries, const float* dirs, const float* weights, uint32_t n,
+                            float weight_scale, float weight_clip, float* grad) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (!grad || (n && (!queries || !dirs || !weights))) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    GuideSample* ds = nullptr;
+    if (n) {
+        st = guide_upload_samples(s, queries, dirs, weights, n, &ds);
+        if (st != XT_OK) return st;
+    }
+    GuideState& g = s->guide;
+    const uint32_t grid = guide_grad_blocks(s, n);
+    hipLaunchKernelGGL(k_guide_grad, dim3(grid), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, ds, n, weight_scale,
+                       weight_clip, g.partial);
+    XT_HIP(hipGetLastError());
+    std::vector<float> rows((size_t)grid * G_NPARAMS);
+    XT_HIP(hipMemcpyAsync(rows.data(), g.partial, rows.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    XT_HIP(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < G_NPARAMS; ++i) {
+        float acc = 0.0f;
+        for (uint32_t b = 0; b < grid; ++b) acc += rows[(size_t)b * G_NPARAMS + i];
+        grad[i] = acc;
+    }
+    return XT_OK;
+}
+
+xt_status xt_guide_train_step(xt_scene* s, const float* queries, const float* dirs, const float* weights, uint32_t n) {
+    xt_status st = guide_need(s);
+    if (st != XT_OK) return st;
+    if (n == 0) return XT_OK;
+    if (!queries || !dirs || !weights) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    double wsum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) wsum += weights[i];
+    if (!(wsum > 0.0) || !std::isfinite(wsum)) return fail(XT_ERR_INVALID_ARGUMENT, "weights must have a positive sum");
+    GuideSample* ds = nullptr;
+    st = guide_upload_samples(s, queries, dirs, weights, n, &ds);
+    if (st != XT_OK) return st;
+    const uint32_t saved = s->guide.cfg.minibatch;
+    s->guide.cfg.minibatch = std::max(saved, n);   // one step over all n samples
+    st = guide_train(s, ds, n, (float)(n / wsum), 1, nullptr);
+    s->guide.cfg.minibatch = saved;
+    if (st != XT_OK) return st;
+    XT_HIP(hipStreamSynchronize(s->stream));
     return XT_OK;
 }
 

@@ -330,4 +330,35 @@ XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, const Frame& fr, float
     return to_world(f, a, b * dp.c, b * dp.s);
 }
 
+__device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
+
+// SurfaceIntersection at a hit (TriangleIndexed.h:133-172, then the identity
+// pass of Object.h:106-144 where Transform::asNormal normalises n, wo and
+// shading.n once more).
+__device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 d, float b0, float b1) {
+    const float4* tr = sc.tris + (size_t)slot * TRI_F4;
+    float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
+    f3 p0 = xyz(v0), p1 = xyz(v1), p2 = xyz(v2);
+    Surf s;
+    s.slot = slot;
+    s.mat = __float_as_uint(v1.w);
+    uint32_t flags = __float_as_uint(v2.w);
+    s.p = lerp3(p0, p1, p2, b0, b1);
+    f3 wo = normalize(-d);
+    f3 n;
+    if (flags & TRI_FLAG_NORMALS) {
+        const float4* nr = sc.normals + (size_t)slot * TRI_F4;
+        n = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), b0, b1));
+    } else {
+        n = normalize(cross(p1 - p0, p2 - p0));
+    }
+    f3 sn = face_forward(n, wo);
+    s.n = normalize(n);
+    s.wo = normalize(wo);
+    s.sn = normalize(sn);
+    return s;
+}
+
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
 }  // namespace xt

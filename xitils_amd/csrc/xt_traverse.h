@@ -93,7 +93,6 @@ struct TraceResult {
     float b0, b1;
 };
 
-__device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
 // Four child slab tests; returns entry distances (inf = miss).  The near and
 // far planes of each axis are picked by address (the ray's direction signs),

@@ -391,3 +391,82 @@ def furnace_object(albedo=0.6, width=32, height=32):
     cam = look_at((0.9, 0.7, -1.6), (0.0, 0.0, 0.0))
     s.camera = Camera(cam, math.radians(45.0), height / width)
     return s
+
+
+# --------------------------------------------------------------------------- occluded room
+
+def _box_faces(lo, hi, inward=False):
+    """12 triangles of an axis-aligned box; normals outward (or inward)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    c = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float64)
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2)]
+    idx = []
+    for a, b, cc, d in quads:
+        t = [(a, b, cc), (a, cc, d)]
+        if inward:
+            t = [(a, cc, b), (a, d, cc)]
+        idx += t
+    return c.astype(np.float32), np.array(idx, np.int32)
+
+
+def occluded_room(width=1280, height=720, n_tris=500_000, seed=4, light_radiance=500.0):
+    """Closed 8 x 4 x 3 room (x, z horizontal, y up) lit only through a 0.05-wide
+    vertical slit in the x = 8 wall, behind which an exterior area light of
+    radiance 500 faces the slit from a closed light box; ~n_tris triangles of
+    seeded boxes and tessellated spheres inside; 50% Lambert, 50% GGX (Metal,
+    alpha ~ U[0.1, 0.4]); no sky."""
+    s = SceneDesc(name="occluded_room")
+    rng = np.random.default_rng(seed)
+    wall = s.add_material(diffuse(0.7))
+    light = s.add_material(emission(light_radiance, light_radiance, light_radiance))
+    X, Y, Z = 8.0, 4.0, 3.0
+    # room shell, inward normals, except the x = X wall which has the slit
+    slit_z0, slit_z1 = 1.475, 1.525
+    verts, idx = [], []
+
+    def add_quad(p0, p1, p2, p3):
+        b = sum(v.shape[0] for v in verts)
+        verts.append(np.array([p0, p1, p2, p3], np.float32))
+        idx.append(np.array([(0, 1, 2), (0, 2, 3)], np.int32) + b)
+    add_quad((0, 0, 0), (0, 0, Z), (X, 0, Z), (X, 0, 0))          # floor, +y
+    add_quad((0, Y, 0), (X, Y, 0), (X, Y, Z), (0, Y, Z))          # ceiling, -y
+    add_quad((0, 0, 0), (0, Y, 0), (0, Y, Z), (0, 0, Z))          # x = 0, +x
+    add_quad((0, 0, 0), (X, 0, 0), (X, Y, 0), (0, Y, 0))          # z = 0, +z
+    add_quad((0, 0, Z), (0, Y, Z), (X, Y, Z), (X, 0, Z))          # z = Z, -z
+    # x = X wall (normal -x) with a full-height slit z in [slit_z0, slit_z1]
+    add_quad((X, 0, 0), (X, 0, slit_z0), (X, Y, slit_z0), (X, Y, 0))
+    add_quad((X, 0, slit_z1), (X, 0, Z), (X, Y, Z), (X, Y, slit_z1))
+    s.objects.append(ObjectDesc(np.concatenate(verts), np.concatenate(idx), material=wall))
+    # exterior light box behind the slit: the emitter at x = X + 1 faces -x
+    lb_v, lb_i = _box_faces((X + 0.001, 0.0, slit_z0 - 0.5), (X + 1.0, Y, slit_z1 + 0.5), inward=True)
+    keep = [k for k in range(12) if k not in (4, 5, 6, 7)]   # open toward the slit; x = X + 1 is the emitter
+    s.objects.append(ObjectDesc(lb_v, lb_i[keep], material=wall))
+    lv, li = quad((X + 1.0, 0.0, slit_z0 - 0.5), (X + 1.0, 0.0, slit_z1 + 0.5),
+                  (X + 1.0, Y, slit_z1 + 0.5), (X + 1.0, Y, slit_z0 - 0.5))   # normal -x
+    s.objects.append(ObjectDesc(lv, li, material=light))
+    # clutter: boxes and spheres, half Lambert, half GGX
+    lam = [s.add_material(diffuse(*rng.uniform(0.2, 0.8, 3))) for _ in range(8)]
+    ggx = [s.add_material(metal(float(np.sqrt(rng.uniform(0.1, 0.4))), tuple(rng.uniform(0.5, 0.95, 3))))
+           for _ in range(8)]
+    tris = 0
+    k = 0
+    while tris < n_tris:
+        mat = (lam if k % 2 == 0 else ggx)[int(rng.integers(0, 8))]
+        c = rng.uniform((0.5, 0.0, 0.3), (X - 1.5, 1.5, Z - 0.3))
+        if k % 3 == 2:
+            r = float(rng.uniform(0.08, 0.35))
+            nu = int(rng.integers(48, 160))
+            p, i, nrm = uv_sphere((c[0], r + c[1] * 0.5, c[2]), r, nu, nu // 2)
+            s.objects.append(ObjectDesc(p, i, material=mat, normals=nrm))
+            tris += i.shape[0]
+        else:
+            size = rng.uniform(0.1, 0.6, 3)
+            bv, bi = box_mesh(tuple(size), (c[0], size[1] / 2, c[2]), float(rng.uniform(0, 90)))
+            s.objects.append(ObjectDesc(bv, bi, material=mat))
+            tris += 12
+        k += 1
+    cam = look_at((0.6, 2.2, 0.4), (6.0, 0.8, 2.2))
+    s.camera = Camera(cam, math.radians(70.0), height / width)
+    return s
