@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["heightfield", "guiding"], default="heightfield",
+                    help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room")
+    ap.add_argument("--budget", type=float, default=2.0, help="guiding: seconds of rendering per method")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="guiding: samples per pixel of the FP64 reference")
+    ap.add_argument("--room-tris", type=int, default=500_000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 1 GiB streaming kernel (out = in * 1) to calibrate rocprof byte counters")
     return ap.parse_args()
@@ -98,12 +103,61 @@ def cpu_baseline(desc, args, max_depth, flags):
     }
 
 
+def guiding_main(args):
+    """Equal-time relMSE, guided vs plain, on the 1280x720 occluded room
+    (8x4x3 m room lit through a 5 cm door slit by an exterior light of
+    radiance 500; 500k triangles of seeded boxes and spheres, 50/50 Lambert /
+    GGX alpha U[0.1, 0.4]) against a 16,384-spp FP64 reference."""
+    import torch  # noqa: F401  (load torch's HIP runtime before the library opens the device)
+    from xitils_amd import api, equal_time, quality, scenes
+    W, H = 1280, 720
+    max_depth = args.max_depth
+    log(f"[bench] building occluded room ({args.room_tris} triangles)")
+    desc = scenes.occluded_room(W, H, n_tris=args.room_tris)
+    dev = api.Device(0)
+    scene = api.Scene(dev, desc)
+    info = scene.info()
+    log(f"[bench] scene {info}")
+    scene.guide_init(seed=1)
+    t0 = time.perf_counter()
+    ref = quality.reference_image(scene, W, H, spp=args.ref_spp, max_depth=max_depth)
+    ref_s = time.perf_counter() - t0
+    log(f"[bench] reference {args.ref_spp} spp in {ref_s:.1f} s ({W * H * args.ref_spp / ref_s / 1e6:.1f} Mpaths/s)")
+    res = equal_time.compare(scene, W, H, args.budget, ref, log=log, max_depth=max_depth)
+    out = {
+        "metric": "equal-time relMSE guided vs plain GPU PT (16k-spp oracle)",
+        "value": round(res["relmse_ratio_equal_time"], 4),
+        "unit": "relMSE(plain) / relMSE(guided) at equal wall time",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(args.budget * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 render, bf16/fp32 MLP",
+        "data": "synthetic",
+        "config": {"workload": f"occluded_room_{W}x{H}_{info['num_prims']}tris_depth{max_depth}",
+                   "budget_s": args.budget, "reference_spp": args.ref_spp, "guide_prob": 0.5,
+                   "train_fraction": 0.25, "learning_rate": 1e-2, "parallelism": "1 GPU"},
+        "time_speedup": res["time_speedup"],
+        "guided_seconds_to_plain_relmse": res["guided_seconds_to_plain_relmse"],
+        "plain": res["plain"],
+        "guided": res["guided"],
+        "reference_seconds": round(ref_s, 2),
+        "curves": res["curves"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only, "
               f"see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "guiding":
+        return guiding_main(args)
     import numpy as np
     import torch
     from xitils_amd import api, scenes

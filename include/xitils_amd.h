@@ -196,6 +196,35 @@ xt_status xt_render_get_stats(const xt_scene* scene, xt_render_stats* out);
  * (algorithmic-traffic accounting for the roofline).  Results are identical. */
 xt_status xt_scene_set_counting(xt_scene* scene, int on);
 
+/* ---- VonMisesFisherDistribution<N> (VonMisesFisherDistribution.h:9-217) -----
+ * Mixtures of num_lobes (1..16) lobes in the reference's encoding
+ * (encode/decode, :37-58): per lobe mu.x, mu.y, mu.z, kappa, alpha, the last
+ * lobe's alpha omitted (5 * num_lobes - 1 floats); decoding sets the last
+ * alpha to 1 minus the running sum.  Arrays are host pointers, or device
+ * pointers when on_device != 0.  The reference draws its uniforms from a
+ * Sampler (Sampler.h:17-19); here the caller passes them. */
+
+/* VonMisesFisherDistribution::eval (:60-66): pdf[i] of direction i under
+ * distribution dist_of_dir[i] (NULL: distribution 0); an index >= num_dists
+ * gives NaN. */
+xt_status xt_vmf_eval(xt_device* dev, uint32_t num_lobes, const float* encoded, uint32_t num_dists,
+                      const uint32_t* dist_of_dir, const float* dirs, uint32_t n, float* pdf, int on_device);
+/* VonMisesFisherDistribution::sample (:162-201): one direction per 4
+ * uniforms (lobe choice, disk radius, disk angle, W), the reference's
+ * sampler including its unit-disk tangent draw and the W = 1 rule at
+ * kappa = 0. */
+xt_status xt_vmf_sample(xt_device* dev, uint32_t num_lobes, const float* encoded, uint32_t num_dists,
+                        const uint32_t* dist_of_sample, const float* uniforms, uint32_t n, float* dirs, int on_device);
+/* VonMisesFisherDistribution::approximateBySEM (:68-160) for num_fits
+ * independent sample sets, set f = samples[offsets[f] .. offsets[f+1]) (xyz
+ * unit vectors), with 2 * num_lobes initialisation uniforms per set (u, v per
+ * lobe, :89-98).  Writes the encoded fits and, if steps is not NULL, the EM
+ * steps each took.  An empty set yields the default distribution (the
+ * reference divides by zero there). */
+xt_status xt_vmf_fit(xt_device* dev, uint32_t num_lobes, const float* samples, uint32_t num_samples,
+                     const uint32_t* offsets, uint32_t num_fits, const float* init_uniforms, float* encoded,
+                     int32_t* steps, int on_device);
+
 /* ---- learned path guiding --------------------------------------------------
  * No reference counterpart: the reference ships only the vMF mixture
  * (VonMisesFisherDistribution.h:9-217).  A 3x64 MLP maps (position, shading

@@ -63,6 +63,9 @@ def lib():
         L.orc_camera_ray.argtypes = [f32p, C.c_float, C.c_float, f32p]
         L.orc_sample_light.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, f32p]
         L.orc_ref_bvh_info.argtypes = [C.c_void_p, C.c_int, i32p]
+        L.orc_vmf_eval.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
+        L.orc_vmf_sample.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
+        L.orc_vmf_fit.argtypes = [C.c_int, f32p, u32p, C.c_int, f32p, f32p, i32p, C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -188,3 +191,69 @@ def camera_ray(cam18, fx, fy):
     out = np.zeros(8, np.float32)
     lib().orc_camera_ray(np.asarray(cam18, np.float32), fx, fy, out)
     return out
+
+
+# ---- VonMisesFisherDistribution<N> (VonMisesFisherDistribution.h:9-217), 5N-1 float encoding
+
+def vmf_encode(mu, kappa, alpha):
+    """(N,3) mu, (N,) kappa, (N,) alpha -> 5N-1 floats (VonMisesFisherDistribution.h:37-45)."""
+    mu = np.asarray(mu, np.float32).reshape(-1, 3)
+    n = mu.shape[0]
+    e = np.zeros(5 * n - 1, np.float32)
+    for k in range(n):
+        e[5 * k:5 * k + 3] = mu[k]
+        e[5 * k + 3] = kappa[k]
+        if k < n - 1:
+            e[5 * k + 4] = alpha[k]
+    return e
+
+
+def vmf_decode(enc):
+    """5N-1 floats -> (mu, kappa, alpha); the last alpha is 1 - running sum
+    in float32 (VonMisesFisherDistribution.h:47-58)."""
+    enc = np.asarray(enc, np.float32)
+    n = (enc.shape[-1] + 1) // 5
+    mu = np.zeros((n, 3), np.float32)
+    kappa = np.zeros(n, np.float32)
+    alpha = np.zeros(n, np.float32)
+    acc = np.float32(0)
+    for k in range(n):
+        mu[k] = enc[5 * k:5 * k + 3]
+        kappa[k] = enc[5 * k + 3]
+        alpha[k] = enc[5 * k + 4] if k < n - 1 else np.float32(1) - acc
+        acc = np.float32(acc + alpha[k])
+    return mu, kappa, alpha
+
+
+def vmf_eval(enc, dirs):
+    enc = np.ascontiguousarray(enc, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros(d.shape[0], np.float32)
+    lib().orc_vmf_eval((enc.shape[0] + 1) // 5, enc, d, d.shape[0], out)
+    return out
+
+
+def vmf_sample(enc, u4):
+    """Directions from the reference's sampler with uniforms (n, 4) = (lobe
+    choice, disk radius, disk angle, W) per sample."""
+    enc = np.ascontiguousarray(enc, np.float32)
+    u = np.ascontiguousarray(u4, np.float32).reshape(-1, 4)
+    out = np.zeros((u.shape[0], 3), np.float32)
+    lib().orc_vmf_sample((enc.shape[0] + 1) // 5, enc, u, u.shape[0], out)
+    return out
+
+
+def vmf_fit(num_lobes, samples, offsets, init_u, nthreads=0, fixed_steps=None):
+    """approximateBySEM (VonMisesFisherDistribution.h:68-160) for each sample
+    set samples[offsets[f]:offsets[f+1]]; init_u is (num_fits, 2N) uniforms.
+    fixed_steps (per set, > 0) runs exactly that many EM steps without the
+    convergence test.  Returns (encoded (num_fits, 5N-1), steps (num_fits,))."""
+    s = np.ascontiguousarray(samples, np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    nf = off.shape[0] - 1
+    iu = np.ascontiguousarray(init_u, np.float32).reshape(nf, 2 * num_lobes)
+    enc = np.zeros((nf, 5 * num_lobes - 1), np.float32)
+    steps = np.zeros(nf, np.int32)
+    fs = None if fixed_steps is None else np.ascontiguousarray(fixed_steps, np.int32)
+    lib().orc_vmf_fit(num_lobes, s, off, nf, iu, enc, steps, None if fs is None else fs.ctypes.data, nthreads)
+    return enc, steps

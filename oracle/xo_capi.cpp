@@ -4,6 +4,7 @@
 // RenderTarget<T>::render (RenderTarget.h:110-127) is restated as an OpenMP
 // `schedule(dynamic, 1)` loop over 16x16 tiles, then samples, then pixels.
 #include "xo_core.hpp"
+#include "xo_vmf.hpp"
 
 #include <omp.h>
 
@@ -243,6 +244,40 @@ void orc_ref_bvh_info(void* s, int obj, int32_t* out2) {
         if (b.nodes[n].prim < 0) { st.push_back({b.nodes[n].child[0], d + 1}); st.push_back({b.nodes[n].child[1], d + 1}); }
     }
     out2[1] = maxd;
+}
+
+// ---- VonMisesFisherDistribution<N> (encoded as 5N - 1 floats)
+void orc_vmf_eval(int N, const float* enc, const float* dirs, int n, float* out) {
+    Vmf v(N);
+    v.decode(enc);
+    for (int i = 0; i < n; ++i) out[i] = v.eval(V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
+void orc_vmf_sample(int N, const float* enc, const float* u4, int n, float* out) {
+    Vmf v(N);
+    v.decode(enc);
+    for (int i = 0; i < n; ++i) {
+        V3 d = v.sample(u4 + 4 * i);
+        out[3 * i] = d.x;
+        out[3 * i + 1] = d.y;
+        out[3 * i + 2] = d.z;
+    }
+}
+
+// Fits num_fits independent sample sets (offsets[f]..offsets[f+1]) in parallel.
+void orc_vmf_fit(int N, const float* samples, const uint32_t* offsets, int num_fits, const float* init_u,
+                 float* out_enc, int32_t* out_steps, const int32_t* fixed_steps, int nthreads) {
+    const int E = 5 * N - 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int f = 0; f < num_fits; ++f) {
+        std::vector<V3> smp;
+        for (uint32_t i = offsets[f]; i < offsets[f + 1]; ++i)
+            smp.push_back(V3(samples[3 * i], samples[3 * i + 1], samples[3 * i + 2]));
+        int steps = 0;
+        Vmf v = Vmf::fit(N, smp, init_u + 2 * N * f, &steps, fixed_steps ? fixed_steps[f] : 0);
+        v.encode(out_enc + (size_t)E * f);
+        out_steps[f] = steps;
+    }
 }
 
 }  // extern "C"

@@ -24,7 +24,7 @@ EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", 
             "xt_scene_info", "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats",
             "xt_scene_set_counting", "xt_guide_config_default", "xt_guide_init", "xt_guide_get_params",
             "xt_guide_set_params", "xt_guide_get_bounds", "xt_guide_forward", "xt_guide_gradient",
-            "xt_guide_train_step"]
+            "xt_guide_train_step", "xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit"]
 
 XT_FLAG_UNIFORM_BARY = 1
 XT_FLAG_ENV_NEE = 2
@@ -120,6 +120,14 @@ def lib():
         L.xt_guide_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                         C.c_float, C.c_void_p]
         L.xt_guide_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+        L.xt_vmf_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_void_p, C.c_int]
+        L.xt_vmf_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                    C.c_uint32, C.c_void_p, C.c_int]
+        L.xt_vmf_fit.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_int]
+        for f in ("xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit"):
+            getattr(L, f).restype = C.c_int
         for f in ("xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy", "xt_scene_info",
                   "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting",
                   "xt_guide_init", "xt_guide_get_params", "xt_guide_set_params", "xt_guide_get_bounds",
@@ -150,6 +158,45 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+def vmf_eval(device, encoded, dirs, dist_of_dir=None):
+    """pdf of each direction under its encoded mixture (VonMisesFisherDistribution::eval)."""
+    enc = np.ascontiguousarray(encoded, np.float32)
+    enc2 = enc.reshape(-1, enc.shape[-1])
+    nl = (enc2.shape[1] + 1) // 5
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    idx = None if dist_of_dir is None else np.ascontiguousarray(dist_of_dir, np.uint32)
+    out = np.zeros(d.shape[0], np.float32)
+    check(lib().xt_vmf_eval(device.h, nl, enc2.ctypes.data, enc2.shape[0], None if idx is None else idx.ctypes.data,
+                            d.ctypes.data, d.shape[0], out.ctypes.data, 0))
+    return out
+
+
+def vmf_sample(device, encoded, uniforms, dist_of_sample=None):
+    """Directions from the reference's vMF sampler, 4 uniforms per sample."""
+    enc = np.ascontiguousarray(encoded, np.float32)
+    enc2 = enc.reshape(-1, enc.shape[-1])
+    nl = (enc2.shape[1] + 1) // 5
+    u = np.ascontiguousarray(uniforms, np.float32).reshape(-1, 4)
+    idx = None if dist_of_sample is None else np.ascontiguousarray(dist_of_sample, np.uint32)
+    out = np.zeros((u.shape[0], 3), np.float32)
+    check(lib().xt_vmf_sample(device.h, nl, enc2.ctypes.data, enc2.shape[0], None if idx is None else idx.ctypes.data,
+                              u.ctypes.data, u.shape[0], out.ctypes.data, 0))
+    return out
+
+
+def vmf_fit(device, num_lobes, samples, offsets, init_uniforms):
+    """approximateBySEM per sample set; returns (encoded (F, 5N-1), steps (F,))."""
+    s = np.ascontiguousarray(samples, np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    nf = off.shape[0] - 1
+    iu = np.ascontiguousarray(init_uniforms, np.float32).reshape(nf, 2 * num_lobes)
+    enc = np.zeros((nf, 5 * num_lobes - 1), np.float32)
+    steps = np.zeros(nf, np.int32)
+    check(lib().xt_vmf_fit(device.h, num_lobes, s.ctypes.data, s.shape[0], off.ctypes.data, nf, iu.ctypes.data,
+                           enc.ctypes.data, steps.ctypes.data, 0))
+    return enc, steps
 
 
 def env_struct(e: "_sc.Env") -> xt_env:

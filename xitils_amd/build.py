@@ -18,7 +18,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libxitils_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 
-SOURCES = ["runtime.hip", "kernels.hip", "guide.hip", "bvh.cpp"]
+SOURCES = ["runtime.hip", "kernels.hip", "guide.hip", "vmf.hip", "bvh.cpp"]
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

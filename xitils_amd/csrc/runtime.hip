@@ -16,39 +16,20 @@
 #include "bvh.h"
 #include "guide.h"
 #include "kernels.h"
+#include "runtime.h"
 
 using namespace xt;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-xt_status fail(xt_status s, const std::string& msg) {
+xt_status xt_fail(xt_status s, const std::string& msg) {
     g_last_error = msg;
     return s;
 }
 
-#define XT_HIP(call)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (call);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail(XT_ERR_DEVICE, std::string(#call) + " failed: " + hipGetErrorString(e_));       \
-    } while (0)
-
-template <class T>
-hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) return hipSuccess;
-    return hipMalloc((void**)p, count * sizeof(T));
-}
-
-}  // namespace
-
-struct xt_device {
-    int ordinal = 0;
-    int num_cu = 0;
-    std::string arch;
-};
+static xt_status fail(xt_status s, const std::string& msg) { return xt_fail(s, msg); }
 
 struct RenderBuffers {
     uint32_t capacity = 0;

@@ -1,0 +1,92 @@
+"""Equal-time comparison of guided and plain path tracing.
+
+Both renderers accumulate progressively into a device film (each call adds
+`spp` samples at the next sample index, the way the reference's
+RenderTarget::render is called once per frame, RenderTarget.h:110-127) until a
+wall-clock budget is spent.  After each call the running mean is compared
+with a converged reference (relMSE, quality.rel_mse); the comparison happens
+outside the timed region.  The guided renderer trains its network online
+during the first `train_fraction` of its budget (one sample per pixel per
+call, Adam steps after each call) and samples with the frozen network after
+that; its training samples stay in the film.
+
+The film is a torch tensor: import torch before the library opens its device,
+so that both share one HIP runtime.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import api
+from .quality import rel_mse
+
+
+def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=0.25, spp_per_call=4,
+                train_spp_per_call=1, seed=1, flags=0, max_depth=16, max_spp=1 << 20, log=None):
+    """Returns the error curve [(seconds, spp, relMSE)] of one progressive run."""
+    import torch
+    film = torch.zeros((height, width, 3), dtype=torch.float32, device="cuda:0")
+    curve = []
+    t = 0.0
+    spp = 0
+    steps = samples = 0
+    infer_ms = train_ms = 0.0
+    while t < budget_s and spp < max_spp:
+        training = guided and t < train_fraction * budget_s
+        f = flags | (api.XT_FLAG_GUIDE if guided else 0) | (api.XT_FLAG_GUIDE_TRAIN if training else 0)
+        n = train_spp_per_call if training else spp_per_call
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.render_device(film.data_ptr(), width, height, n, spp_offset=spp, seed=seed, flags=f, max_depth=max_depth)
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+        spp += n
+        st = scene.stats()
+        steps += st["guide_steps"]
+        samples += st["guide_samples"]
+        infer_ms += st["ms_guide_infer"]
+        train_ms += st["ms_guide_train"]
+        img = film.cpu().numpy().astype(np.float64) / spp
+        curve.append((t, spp, rel_mse(img, ref)))
+    if log:
+        log(f"[equal-time] guided={guided}: {spp} spp in {t:.3f} s, relMSE {curve[-1][2]:.4e}, "
+            f"adam steps {steps}, samples {samples}, infer {infer_ms:.1f} ms, train {train_ms:.1f} ms")
+    return curve, {"adam_steps": steps, "train_samples": samples, "ms_guide_infer": infer_ms,
+                   "ms_guide_train": train_ms}
+
+
+def time_to_reach(curve, target):
+    """First time the curve's relMSE drops to `target` (log-linear between
+    checkpoints), or None."""
+    prev = None
+    for t, _, e in curve:
+        if e <= target:
+            if prev is None or prev[1] <= e:
+                return t
+            t0, e0 = prev
+            a = (math.log(e0) - math.log(target)) / (math.log(e0) - math.log(e))
+            return t0 + a * (t - t0)
+        prev = (t, e)
+    return None
+
+
+def compare(scene, width, height, budget_s, ref, *, log=None, **kw):
+    """Runs plain then guided for the same budget; returns a summary dict."""
+    plain, _ = progressive(scene, width, height, budget_s, ref, guided=False, log=log, **kw)
+    guided, gstats = progressive(scene, width, height, budget_s, ref, guided=True, log=log, **kw)
+    e_plain = plain[-1][2]
+    e_guided = guided[-1][2]
+    t_reach = time_to_reach(guided, e_plain)
+    return {
+        "budget_s": budget_s,
+        "plain": {"spp": plain[-1][1], "seconds": round(plain[-1][0], 4), "relmse": e_plain},
+        "guided": {"spp": guided[-1][1], "seconds": round(guided[-1][0], 4), "relmse": e_guided, **gstats},
+        "relmse_ratio_equal_time": e_plain / e_guided,
+        "guided_seconds_to_plain_relmse": t_reach,
+        "time_speedup": (plain[-1][0] / t_reach) if t_reach else None,
+        "curves": {"plain": [(round(a, 4), b, c) for a, b, c in plain[:: max(1, len(plain) // 40)]],
+                   "guided": [(round(a, 4), b, c) for a, b, c in guided[:: max(1, len(guided) // 40)]]},
+    }
