@@ -245,6 +245,7 @@ typedef struct xt_guide_config {
     float weight_clip;           /* cap on sample weights normalised to mean 1 (20) */
     uint32_t seed;               /* parameter initialisation */
     uint32_t max_records;        /* training records kept per render batch (8 Mi) */
+    uint32_t train_target;       /* 0: incident radiance L_i, 1: product f cos L_i (default) */
 } xt_guide_config;
 
 void xt_guide_config_default(xt_guide_config* cfg);

@@ -84,7 +84,8 @@ class xt_render_stats(C.Structure):
 class xt_guide_config(C.Structure):
     _fields_ = [("guide_prob", C.c_float), ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("adam_eps", C.c_float), ("minibatch", C.c_uint32), ("max_steps_per_batch", C.c_uint32),
-                ("weight_clip", C.c_float), ("seed", C.c_uint32), ("max_records", C.c_uint32)]
+                ("weight_clip", C.c_float), ("seed", C.c_uint32), ("max_records", C.c_uint32),
+                ("train_target", C.c_uint32)]
 
 
 _lib = None

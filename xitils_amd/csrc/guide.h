@@ -16,7 +16,26 @@ struct GuideNet {
     __bf16 *w0, *w1, *w2, *w3;
     __bf16 *w0t, *w1t, *w2t, *w3t;
     float *b0, *b1, *b2, *b3;
+    struct GuideChain* chain;   // inference copy (below)
 };
+
+// Inference copy of the network, laid out as the kernels hold it in LDS.
+// Hidden activations stay in registers between layers: the MFMA output
+// fragment of one layer is the input fragment of the next once the input
+// columns of the next weight matrix are permuted within each block of 32
+// (unit_of_kpos); layer 0 columns follow the per-lane feature order
+// (feature_of_kpos) and layer 3 rows put whole lobes in each lane
+// (output_of_row).  Rows are padded (104, 72 bf16) so 16-byte fragment
+// reads hit distinct LDS banks.
+constexpr int GC_LD0 = 104, GC_LD = 72;
+struct GuideChain {
+    __bf16 w0[G_HID * GC_LD0];
+    __bf16 w1[G_HID * GC_LD];
+    __bf16 w2[G_HID * GC_LD];
+    __bf16 w3[G_OUT_PAD * GC_LD];
+    float b0[G_HID], b1[G_HID], b2[G_HID], b3[G_OUT_PAD];
+};
+static_assert(sizeof(GuideChain) % 16 == 0, "GuideChain is copied in 16-byte pieces");
 
 // One training sample: query (p, n, wo), sampled direction, weight.
 struct GuideSample {
@@ -27,16 +46,16 @@ struct GuideSample {
 
 __global__ void k_guide_prepare(const float* P, GuideNet net);
 __global__ void k_guide_forward(GuideNet net, GuideBounds gb, const float* qin, uint32_t n, float* out);
-__global__ void k_guide_infer(GuideNet net, GuideBounds gb, WaveParams wp, DevScene sc, PathQueue q,
-                              const uint32_t* count, const float4* hits, float* lobes, uint32_t stride);
+__global__ void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue q, const uint32_t* count,
+                              const float4* hits, float4* lobes4, float2* lobes2, uint32_t stride);
 __global__ void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
                              float weight_clip, float* partial);
 __global__ void k_guide_adam(float* P, float* m, float* v, const float* partial, uint32_t nparts, float lr, float b1,
                              float b2, float eps, float b1t, float b2t, float grad_scale);
-__global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4* radiance, GuideSample* out,
+__global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4* radiance, uint32_t target, GuideSample* out,
                                  uint32_t* count, float* weight_sum);
 
-constexpr int GI_WAVES = 4;     // waves per block of k_guide_forward / k_guide_infer
+constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
 constexpr int GG_BLOCK = 64;    // threads per block of k_guide_grad
 
 }  // namespace xt

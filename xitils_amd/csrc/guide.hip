@@ -63,23 +63,29 @@ __device__ __forceinline__ void dense64(const __bf16* W, const float* bias, cons
     }
 }
 
-// Four layers for the 64 queries whose features sit in X ([64][96] bf16);
-// H is a [64][64] bf16 buffer; the raw outputs land in Y ([64][48] fp32,
-// which may alias X).
-__device__ __forceinline__ void mlp64(const __bf16* w0, const __bf16* w1, const __bf16* w2, const __bf16* w3,
-                                      const float* b0, const float* b1, const float* b2, const float* b3, __bf16* X,
-                                      __bf16* H, float* Y) {
-    dense64<G_IN, G_HID, true, false>(w0, b0, X, H);
-    wave_sync();
-    dense64<G_HID, G_HID, true, false>(w1, b1, H, X);
-    wave_sync();
-    dense64<G_HID, G_HID, true, false>(w2, b2, X, H);
-    wave_sync();
-    dense64<G_HID, G_OUT_PAD, false, true>(w3, b3, H, Y);
-    wave_sync();
-}
-
 // ---------------------------------------------------------------- parameters
+
+// GuideChain layout maps.  An MFMA output fragment leaves lane l = 16g + r
+// with units 16b + 4g + i (i = 0..3) of query r; the next layer's input
+// fragment wants k positions 32c + 8g + j.  Position p of a hidden layer's
+// input therefore holds unit unit_of_kpos(p).
+__host__ __device__ __forceinline__ int unit_of_kpos(int p) {
+    const int c = p >> 5, g = (p >> 3) & 3, j = p & 7;
+    return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+// Layer 0: lane group g < 3 computes the 24 frequency features of axis g
+// (sin, cos of 2^k pi u, k = 0..11), group 3 the normal and view direction.
+__host__ __device__ __forceinline__ int feature_of_kpos(int p) {   // -1: zero column
+    const int c = p >> 5, g = (p >> 3) & 3, j = p & 7, s = 8 * c + j;
+    if (g < 3) return 24 * g + s;
+    return s < 6 ? 72 + s : -1;
+}
+// Layer 3: output row m lands in lane group g as slot s = 4b + i; slots
+// 0..9 of group g carry lobes 2g and 2g + 1 (5 raw outputs each).
+__host__ __device__ __forceinline__ int output_of_row(int m) {      // -1: unused row
+    const int b = m >> 4, g = (m >> 2) & 3, i = m & 3, s = 4 * b + i;
+    return s < 10 ? 10 * g + s : -1;
+}
 
 // fp32 master parameters -> bf16 copies: natural [out][in] for the forward
 // pass and transposed [in][out] (w3t padded to [64][64]) for the backward
@@ -113,81 +119,201 @@ __global__ void k_guide_prepare(const float* __restrict__ P, GuideNet net) {
         net.b2[i] = P[G_OFF_B2 + i];
     }
     if (i < G_OUT_PAD) net.b3[i] = P[G_OFF_B3 + i];
+    // inference copy (GuideChain)
+    GuideChain& ch = *net.chain;
+    if (i < G_HID * GC_LD0) {
+        const int m = i / GC_LD0, p = i % GC_LD0, f = p < G_IN ? feature_of_kpos(p) : -1;
+        ch.w0[i] = (__bf16)(f >= 0 ? P[G_OFF_W0 + m * G_IN + f] : 0.0f);
+    }
+    if (i < G_HID * GC_LD) {
+        const int m = i / GC_LD, p = i % GC_LD, u = p < G_HID ? unit_of_kpos(p) : -1;
+        ch.w1[i] = (__bf16)(u >= 0 ? P[G_OFF_W1 + m * G_HID + u] : 0.0f);
+        ch.w2[i] = (__bf16)(u >= 0 ? P[G_OFF_W2 + m * G_HID + u] : 0.0f);
+    }
+    if (i < G_OUT_PAD * GC_LD) {
+        const int m = i / GC_LD, p = i % GC_LD, u = p < G_HID ? unit_of_kpos(p) : -1, o = output_of_row(m);
+        ch.w3[i] = (__bf16)(u >= 0 && o >= 0 ? P[G_OFF_W3 + o * G_HID + u] : 0.0f);
+    }
+    if (i < G_HID) {
+        ch.b0[i] = P[G_OFF_B0 + i];
+        ch.b1[i] = P[G_OFF_B1 + i];
+        ch.b2[i] = P[G_OFF_B2 + i];
+    }
+    if (i < G_OUT_PAD) {
+        const int o = output_of_row(i);
+        ch.b3[i] = o >= 0 ? P[G_OFF_B3 + o] : 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------- inference
-
-constexpr int GI_WAVE_LDS = 64 * G_IN * 2 + 64 * 64 * 2;   // X|Y region (12 KB) + H region (8 KB)
-
-struct InferLds {
-    __bf16 w0[G_HID * G_IN], w1[G_HID * G_HID], w2[G_HID * G_HID], w3[G_OUT_PAD * G_HID];
-    float b[G_HID * 3 + G_OUT_PAD];
-    unsigned char wave[GI_WAVES][GI_WAVE_LDS];
-};
-
-__device__ __forceinline__ void load_net_lds(const GuideNet& net, InferLds& s) {
-    for (int i = threadIdx.x; i < G_HID * G_IN; i += blockDim.x) s.w0[i] = net.w0[i];
-    for (int i = threadIdx.x; i < G_HID * G_HID; i += blockDim.x) {
-        s.w1[i] = net.w1[i];
-        s.w2[i] = net.w2[i];
-    }
-    for (int i = threadIdx.x; i < G_OUT_PAD * G_HID; i += blockDim.x) s.w3[i] = net.w3[i];
-    for (int i = threadIdx.x; i < G_HID; i += blockDim.x) {
-        s.b[i] = net.b0[i];
-        s.b[G_HID + i] = net.b1[i];
-        s.b[2 * G_HID + i] = net.b2[i];
-    }
-    for (int i = threadIdx.x; i < G_OUT_PAD; i += blockDim.x) s.b[3 * G_HID + i] = net.b3[i];
-    __syncthreads();
-}
 
 __device__ __forceinline__ void zero_row(__bf16* x) {
     for (int k = 0; k < G_IN; ++k) x[k] = (__bf16)0.0f;
 }
 
-// Raw network outputs for n queries (p, n, wo as 9 floats each): out[q * 40 + o].
-__global__ __launch_bounds__(64 * GI_WAVES) void k_guide_forward(GuideNet net, GuideBounds gb, const float* __restrict__ qin,
-                                                                 uint32_t n, float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) InferLds s;
-    load_net_lds(net, s);
-    const int wave = threadIdx.x >> 6, l = __lane_id();
-    __bf16* X = (__bf16*)s.wave[wave];
-    __bf16* H = (__bf16*)(s.wave[wave] + 64 * G_IN * 2);
-    float* Y = (float*)X;
-    const float* sb = s.b;
-    for (uint32_t base = (blockIdx.x * GI_WAVES + wave) * 64u; base < n; base += gridDim.x * GI_WAVES * 64u) {
-        const uint32_t q = base + l;
-        if (q < n) {
-            const float* in = qin + (size_t)q * 9;
-            guide_encode(gb, mk(in[0], in[1], in[2]), mk(in[3], in[4], in[5]), mk(in[6], in[7], in[8]), X + l * G_IN);
-        } else {
-            zero_row(X + l * G_IN);
+__device__ __forceinline__ void load_chain(const GuideChain* src, GuideChain& dst) {
+    const uint4* s = (const uint4*)src;
+    uint4* d = (uint4*)&dst;
+    for (int k = threadIdx.x; k < (int)(sizeof(GuideChain) / 16); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+}
+
+// This lane's 24 input features of one query (layer-0 input fragment, k
+// blocks 0..2): group g < 3 the frequency features of axis g, group 3 the
+// shading normal and view direction.  Same arithmetic as guide_encode.
+__device__ __forceinline__ void chain_features(const GuideBounds& gb, f3 p, f3 n, f3 wo, bf16x8 (&x)[3]) {
+    const int g = __lane_id() >> 4;
+    float v[24];
+    if (g < 3) {
+        const float PI = 3.14159265358979323846f;
+        const float pa = g == 0 ? p.x : (g == 1 ? p.y : p.z);
+        const float lo = g == 0 ? gb.lo.x : (g == 1 ? gb.lo.y : gb.lo.z);
+        const float inv = g == 0 ? gb.inv_ext.x : (g == 1 ? gb.inv_ext.y : gb.inv_ext.z);
+        const float ua = fminf(fmaxf((pa - lo) * inv, 0.0f), 1.0f);
+        float sn = sinf(PI * ua), cs = cosf(PI * ua);
+#pragma unroll
+        for (int k = 0; k < G_BANDS; ++k) {
+            v[2 * k] = sn;
+            v[2 * k + 1] = cs;
+            const float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
+            sn = s2;
+            cs = c2;
         }
-        wave_sync();
-        mlp64(s.w0, s.w1, s.w2, s.w3, sb, sb + G_HID, sb + 2 * G_HID, sb + 3 * G_HID, X, H, Y);
-        if (q < n)
-            for (int o = 0; o < G_OUT; ++o) out[(size_t)q * G_OUT + o] = Y[l * G_OUT_PAD + o];
-        wave_sync();
+    } else {
+        v[0] = n.x; v[1] = n.y; v[2] = n.z;
+        v[3] = wo.x; v[4] = wo.y; v[5] = wo.z;
+#pragma unroll
+        for (int k = 6; k < 24; ++k) v[k] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[c][j] = (__bf16)v[8 * c + j];
+}
+
+// relu(acc) of two adjacent 16-unit output tiles -> one input fragment
+__device__ __forceinline__ bf16x8 relu_pack(floatx4 a, floatx4 b) {
+    bf16x8 h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h[i] = (__bf16)fmaxf(a[i], 0.0f);
+        h[4 + i] = (__bf16)fmaxf(b[i], 0.0f);
+    }
+    return h;
+}
+
+// The four layers for NT tiles of 16 queries, activations in registers.
+// y[t][b][i] = raw output row 16b + 4g + i (GuideChain order) of query 16t + r.
+template <int NT>
+__device__ __forceinline__ void chain_mlp(const GuideChain& s, const bf16x8 (&x)[NT][3], floatx4 (&y)[NT][3]) {
+    const int g = __lane_id() >> 4;
+    floatx4 acc[NT][4];
+    bf16x8 h[NT][2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const floatx4 bias = *(const floatx4*)&s.b0[16 * b + 4 * g];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bf16x8 a = frag(s.w0, GC_LD0, 16 * b, 32 * c);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, x[t][c], acc[t][b]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        h[t][0] = relu_pack(acc[t][0], acc[t][1]);
+        h[t][1] = relu_pack(acc[t][2], acc[t][3]);
+    }
+    const __bf16* wl[2] = {s.w1, s.w2};
+    const float* bl[2] = {s.b1, s.b2};
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const floatx4 bias = *(const floatx4*)&bl[layer][16 * b + 4 * g];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const bf16x8 a = frag(wl[layer], GC_LD, 16 * b, 32 * c);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, h[t][c], acc[t][b]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            h[t][0] = relu_pack(acc[t][0], acc[t][1]);
+            h[t][1] = relu_pack(acc[t][2], acc[t][3]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const floatx4 bias = *(const floatx4*)&s.b3[16 * b + 4 * g];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) y[t][b] = bias;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bf16x8 a = frag(s.w3, GC_LD, 16 * b, 32 * c);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) y[t][b] = mfma(a, h[t][c], y[t][b]);
+        }
+    }
+}
+
+// Slot k (0..9) of this lane's two lobes: raw output 10g + k of the query.
+__device__ __forceinline__ float out_slot(const floatx4 (&y)[3], int k) { return y[k >> 2][k & 3]; }
+
+// Raw network outputs for n queries (p, n, wo as 9 floats each): out[q * 40 + o].
+__global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, GuideBounds gb, const float* __restrict__ qin,
+                                                       uint32_t n, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) GuideChain s;
+    load_chain(net.chain, s);
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u; base < n; base += nwaves * 64u) {
+        float qv[9];
+        const uint32_t qi = base + l;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) qv[k] = qi < n ? qin[(size_t)qi * 9 + k] : 0.0f;
+        bf16x8 x[4][3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float v[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) v[k] = __shfl(qv[k], 16 * t + r);
+            chain_features(gb, mk(v[0], v[1], v[2]), mk(v[3], v[4], v[5]), mk(v[6], v[7], v[8]), x[t]);
+        }
+        floatx4 y[4][3];
+        chain_mlp<4>(s, x, y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t q = base + 16 * t + r;
+            if (q < n)
+#pragma unroll
+                for (int k = 0; k < 10; ++k) out[(size_t)q * G_OUT + 10 * g + k] = out_slot(y[t], k);
+        }
     }
 }
 
 // Decoded mixtures for the extension-queue entries of one bounce that the
-// shade kernel may guide (hit, not emissive, not a delta material).
-__global__ __launch_bounds__(64 * GI_WAVES) void k_guide_infer(GuideNet net, GuideBounds gb, WaveParams wp, DevScene sc,
-                                                               PathQueue pq, const uint32_t* __restrict__ count,
-                                                               const float4* __restrict__ hits, float* __restrict__ lobes,
-                                                               uint32_t stride) {
-    __shared__ __attribute__((aligned(16))) InferLds s;
-    load_net_lds(net, s);
+// shade kernel may guide (hit, not emissive, not a delta material):
+// lobes4[j * stride + i] = (mu, kappa), lobes2[j * stride + i] =
+// (alpha, alpha * vmf_norm(kappa)) of lobe j.
+__global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue pq,
+                                                     const uint32_t* __restrict__ count, const float4* __restrict__ hits,
+                                                     float4* __restrict__ lobes4, float2* __restrict__ lobes2,
+                                                     uint32_t stride) {
+    __shared__ __attribute__((aligned(16))) GuideChain s;
+    load_chain(net.chain, s);
     const uint32_t n = *count;
-    const int wave = threadIdx.x >> 6, l = __lane_id();
-    __bf16* X = (__bf16*)s.wave[wave];
-    __bf16* H = (__bf16*)(s.wave[wave] + 64 * G_IN * 2);
-    float* Y = (float*)X;
-    const float* sb = s.b;
-    for (uint32_t base = (blockIdx.x * GI_WAVES + wave) * 64u; base < n; base += gridDim.x * GI_WAVES * 64u) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + l;
         bool use = false;
+        f3 P = mk(0, 0, 0), N = mk(0, 0, 0), WO = mk(0, 0, 0);
         if (i < n) {
             const float4 h = hits[i];
             const uint32_t slot = __float_as_uint(h.y);
@@ -196,32 +322,52 @@ __global__ __launch_bounds__(64 * GI_WAVES) void k_guide_infer(GuideNet net, Gui
                 const uint32_t mt = sc.mats[mat_id].type;
                 if (mt != XT_MAT_EMISSION && !mat_is_delta(mt)) {
                     const Surf sf = make_surf(sc, slot, xyz(pq.d[i]), h.z, h.w);
-                    guide_encode(gb, sf.p, sf.sn, sf.wo, X + l * G_IN);
+                    P = sf.p;
+                    N = sf.sn;
+                    WO = sf.wo;
                     use = true;
                 }
             }
         }
-        if (!use) zero_row(X + l * G_IN);
-        // whole waves only: skip the products when no lane has a query
-        if (__ballot(use) != 0) {
-            wave_sync();
-            mlp64(s.w0, s.w1, s.w2, s.w3, sb, sb + G_HID, sb + 2 * G_HID, sb + 3 * G_HID, X, H, Y);
-            if (use) {
-                VmfLobe lb[G_LOBES];
-                guide_decode(Y + l * G_OUT_PAD, lb);
+        if (__ballot(use) == 0) continue;
+        bf16x8 x[4][3];
 #pragma unroll
-                for (int j = 0; j < G_LOBES; ++j) {
-                    float* f = lobes + (size_t)(6 * j) * stride + i;
-                    f[0] = lb[j].mu.x;
-                    f[(size_t)stride] = lb[j].mu.y;
-                    f[(size_t)2 * stride] = lb[j].mu.z;
-                    f[(size_t)3 * stride] = lb[j].kappa;
-                    f[(size_t)4 * stride] = lb[j].alpha;
-                    f[(size_t)5 * stride] = lb[j].alpha * vmf_norm(lb[j].kappa);
+        for (int t = 0; t < 4; ++t) {
+            const int src = 16 * t + r;
+            chain_features(gb, mk(__shfl(P.x, src), __shfl(P.y, src), __shfl(P.z, src)),
+                           mk(__shfl(N.x, src), __shfl(N.y, src), __shfl(N.z, src)),
+                           mk(__shfl(WO.x, src), __shfl(WO.y, src), __shfl(WO.z, src)), x[t]);
+        }
+        floatx4 y[4][3];
+        chain_mlp<4>(s, x, y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int src = 16 * t + r;
+            const bool useq = __shfl((int)use, src) != 0;
+            // softmax over the 8 alpha logits held by the 4 lanes of this query
+            const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
+            float mx = fmaxf(la, lb);
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float ea = expf(la - mx), eb = expf(lb - mx);
+            float sum = ea + eb;
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            if (useq) {
+                const uint32_t q = base + src;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
+                    const float len = length(m);
+                    const f3 mu = len > 1e-12f ? div(m, len) : mk(0.0f, 0.0f, 1.0f);
+                    const float kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
+                    const float alpha = (k == 0 ? ea : eb) / sum;
+                    const size_t o = (size_t)(2 * g + k) * stride + q;
+                    lobes4[o] = make_float4(mu.x, mu.y, mu.z, kappa);
+                    lobes2[o] = make_float2(alpha, alpha * vmf_norm(kappa));
                 }
             }
         }
-        wave_sync();
     }
 }
 
@@ -387,21 +533,27 @@ __global__ void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float
     P[i] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// Training records -> samples: incident radiance along the sampled direction
-// (L_path - pre) / T, weight = luminance / pmix; keeps positive finite
-// weights, appended to out (order not fixed), and sums the weights.
+// Training records -> samples.  C = (L_path - pre) / W estimates
+// f cos L_i / pdf along the sampled direction; the weight is
+//   target 0 (radiance): luminance(L_i) / pdf = luminance(C / E) / pdf
+//   target 1 (product):  luminance(f cos L_i) / pdf = luminance(C)
+// Positive finite weights are appended to out (order not fixed) and summed.
 __global__ void k_guide_finalize(const TrainRec* __restrict__ recs, uint32_t n, const float4* __restrict__ radiance,
-                                 GuideSample* __restrict__ out, uint32_t* __restrict__ count, float* __restrict__ weight_sum) {
+                                 uint32_t target, GuideSample* __restrict__ out, uint32_t* __restrict__ count,
+                                 float* __restrict__ weight_sum) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float w = 0.0f;
     bool keep = false;
     if (i < n && recs[i].valid) {
         const TrainRec& r = recs[i];
         const float4 L = radiance[r.id];
-        float Li[3] = {L.x - r.pre[0], L.y - r.pre[1], L.z - r.pre[2]};
-        for (int c = 0; c < 3; ++c) Li[c] = r.T[c] > 0.0f ? fmaxf(Li[c], 0.0f) / r.T[c] : 0.0f;
-        const float lum = 0.2126f * Li[0] + 0.7152f * Li[1] + 0.0722f * Li[2];
-        w = lum / r.pmix;
+        float C[3] = {L.x - r.pre[0], L.y - r.pre[1], L.z - r.pre[2]};
+        for (int c = 0; c < 3; ++c) {
+            C[c] = r.W[c] > 0.0f ? fmaxf(C[c], 0.0f) / r.W[c] : 0.0f;
+            if (target == 0) C[c] = r.E[c] > 0.0f ? C[c] / r.E[c] : 0.0f;
+        }
+        const float lum = 0.2126f * C[0] + 0.7152f * C[1] + 0.0722f * C[2];
+        w = target == 0 ? lum / r.pmix : lum;
         keep = r.pmix > 0.0f && w > 0.0f && isfinite(w);
     }
     const uint64_t mask = __ballot(keep);

@@ -59,28 +59,30 @@ struct ShadowQueue {
 };
 
 // Per-vertex training record written while rendering: the query, the
-// sampled direction and its mixture pdf, the path throughput after the
-// vertex (T) and the path radiance collected up to and including the
-// vertex's own light sample (pre).  After the path ends, the incident
-// radiance along dir is (L_path - pre) / T per channel.
+// sampled direction and its strategy pdf, the path throughput before the
+// vertex (W), the vertex's sample weight f cos / pdf (E) and the path
+// radiance collected up to and including the vertex's own light sample
+// (pre).  After the path ends, C = (L_path - pre) / W is the vertex's
+// estimate of f cos L_i / pdf along dir, and L_i = C / E.
 struct TrainRec {
     float q[9];
     float dir[3];
     float pmix;
-    float T[3];
+    float W[3];
+    float E[3];
     float pre[3];
     uint32_t id;      // path id
     uint32_t valid;   // 1 = guided vertex
-    float pad;
 };
 
-// Decoded mixture per path-queue entry, SoA: field f of entry i at
-// lobes[f * stride + i]; per lobe j fields 6j.. = mu.x, mu.y, mu.z, kappa,
-// alpha, alpha * vmf_norm(kappa).
+// Decoded mixture per path-queue entry, SoA per lobe j:
+// lobes4[j * stride + i] = (mu, kappa), lobes2[j * stride + i] =
+// (alpha, alpha * vmf_norm(kappa)); 192 bytes per entry.
 constexpr int G_LOBE_FIELDS = 6 * G_LOBES;
 
 struct GuideRender {
-    const float* lobes;    // null: guiding off
+    const float4* lobes4;  // null: guiding off
+    const float2* lobes2;
     uint32_t stride;
     float prob;            // probability of sampling the guide (one-sample MIS with the BSDF)
     TrainRec* recs;        // null: no training records

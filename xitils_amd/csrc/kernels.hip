@@ -301,15 +301,17 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     // Guided vertex: one-sample MIS of the learned mixture (probability
                     // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
                     // pdf wherever the BSDF pdf is positive, and zero elsewhere.
-                    guided = GUIDE && gr.lobes != nullptr && !mat_is_delta(mat.type);
-                    auto lobe = [&](int f) { return gr.lobes[(size_t)f * gr.stride + i]; };
+                    guided = GUIDE && gr.lobes4 != nullptr && !mat_is_delta(mat.type);
+                    auto l4 = [&](int j) { return gr.lobes4[(size_t)j * gr.stride + i]; };
+                    auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
                     auto strategy_pdf = [&](f3 dir, float pb) {
-                        return pb > 0.0f ? gr.prob * guide_pdf_fields(lobe, dir) + (1.0f - gr.prob) * pb : 0.0f;
+                        return pb > 0.0f ? gr.prob * guide_pdf_lobes(l4, l2, dir) + (1.0f - gr.prob) * pb : 0.0f;
                     };
                     if (guided) {
                         const uint4 blk3 = rng.block(v, 3);
                         if (Rng::to_unit(blk3.x) < gr.prob) {
-                            wi = guide_sample_fields(lobe, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z), Rng::to_unit(blk3.w));
+                            wi = guide_sample_lobes(l4, l2, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z),
+                                                    Rng::to_unit(blk3.w));
                         } else {
                             bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
                                         &pdfB);
@@ -456,13 +458,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                 qout.rec[ei] = keep ? ri : INVALID;
                 if (keep) {
                     TrainRec& rr = gr.recs[ri];
-                    const f3 T = ext_w * ext_e;
                     rr.q[0] = s.p.x; rr.q[1] = s.p.y; rr.q[2] = s.p.z;
                     rr.q[3] = s.sn.x; rr.q[4] = s.sn.y; rr.q[5] = s.sn.z;
                     rr.q[6] = s.wo.x; rr.q[7] = s.wo.y; rr.q[8] = s.wo.z;
                     rr.dir[0] = ext_d.x; rr.dir[1] = ext_d.y; rr.dir[2] = ext_d.z;
                     rr.pmix = ext_pdf;
-                    rr.T[0] = T.x; rr.T[1] = T.y; rr.T[2] = T.z;
+                    rr.W[0] = ext_w.x; rr.W[1] = ext_w.y; rr.W[2] = ext_w.z;
+                    rr.E[0] = ext_e.x; rr.E[1] = ext_e.y; rr.E[2] = ext_e.z;
                     rr.id = id;
                     rr.valid = guided ? 1u : 0u;
                 }

@@ -528,7 +528,7 @@ static void guide_init_params(uint32_t seed, std::vector<float>& P) {
 
 static xt_status guide_prepare(xt_scene* s) {
     GuideState& g = s->guide;
-    hipLaunchKernelGGL(k_guide_prepare, dim3((G_HID * G_IN + 255) / 256), dim3(256), 0, s->stream, g.P, g.net);
+    hipLaunchKernelGGL(k_guide_prepare, dim3((G_HID * GC_LD0 + 255) / 256), dim3(256), 0, s->stream, g.P, g.net);
     if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "k_guide_prepare launch failed");
     return XT_OK;
 }
@@ -709,13 +709,16 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 GuideRender gr{};
                 if (guide) {
                     a = tm.mark();
-                    const uint32_t ib = std::max(1u, std::min((live + 64 * GI_WAVES - 1) / (64 * GI_WAVES),
-                                                              (uint32_t)s->dev->num_cu));
-                    hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(64 * GI_WAVES), 0, s->stream, g.net, g.gb, wp, s->ds,
-                                       rb.q[cur], rb.counts + cur, rb.hits, rb.lobes, rb.capacity);
+                    const uint32_t ib = std::max(1u, std::min((live + GI_BLOCK - 1) / GI_BLOCK,
+                                                              (uint32_t)s->dev->num_cu * 4));
+                    float4* l4 = (float4*)rb.lobes;
+                    float2* l2 = (float2*)(rb.lobes + (size_t)4 * G_LOBES * rb.capacity);
+                    hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
+                                       rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity);
                     XT_LAUNCHED(result);
                     tm.add(4, a, tm.mark());
-                    gr.lobes = rb.lobes;
+                    gr.lobes4 = l4;
+                    gr.lobes2 = l2;
                     gr.stride = rb.capacity;
                     gr.prob = g.cfg.guide_prob;
                     if (train) {
@@ -769,7 +772,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, g.d_count, 1u);
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, (uint32_t*)g.d_wsum, 1u);
                 hipLaunchKernelGGL(k_guide_finalize, dim3((rec_base + 255) / 256), dim3(256), 0, s->stream, g.recs, rec_base,
-                                   rb.radiance, g.samples, g.d_count, g.d_wsum);
+                                   rb.radiance, g.cfg.train_target, g.samples, g.d_count, g.d_wsum);
                 XT_LAUNCHED(result);
                 hipMemcpyAsync(g.host_io, g.d_count, 4, hipMemcpyDeviceToHost, s->stream);
                 hipMemcpyAsync(g.host_io + 1, g.d_wsum, 4, hipMemcpyDeviceToHost, s->stream);
@@ -850,6 +853,7 @@ void xt_guide_config_default(xt_guide_config* c) {
     c->weight_clip = 20.0f;
     c->seed = 1;
     c->max_records = 1u << 23;
+    c->train_target = 1;
 }
 
 xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
@@ -859,6 +863,7 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     if (cfg) c = *cfg;
     if (!(c.guide_prob >= 0.0f && c.guide_prob < 1.0f))
         return fail(XT_ERR_INVALID_ARGUMENT, "guide_prob must be in [0, 1)");
+    if (c.train_target > 1) return xt_fail(XT_ERR_INVALID_ARGUMENT, "train_target must be 0 or 1");
     if (!(c.learning_rate > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f))
         return fail(XT_ERR_INVALID_ARGUMENT, "bad Adam settings");
     XT_HIP(hipSetDevice(s->dev->ordinal));
@@ -891,11 +896,11 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     chk(dalloc(&g.d_wsum, 1));
     chk(hipHostMalloc((void**)&g.host_io, 2 * sizeof(float), hipHostMallocDefault));
     // bf16 copies and biases in one allocation, 256-byte aligned pieces
-    const size_t sizes[12] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
+    const size_t sizes[13] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
                               2 * G_IN * G_HID, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_HID * 64,
-                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD};
-    size_t off[12], total = 0;
-    for (int k = 0; k < 12; ++k) {
+                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD, sizeof(GuideChain)};
+    size_t off[13], total = 0;
+    for (int k = 0; k < 13; ++k) {
         off[k] = total;
         total += (sizes[k] + 255) / 256 * 256;
     }
@@ -911,6 +916,7 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     g.net.w2t = (__bf16*)(b + off[6]); g.net.w3t = (__bf16*)(b + off[7]);
     g.net.b0 = (float*)(b + off[8]); g.net.b1 = (float*)(b + off[9]);
     g.net.b2 = (float*)(b + off[10]); g.net.b3 = (float*)(b + off[11]);
+    g.net.chain = (GuideChain*)(b + off[12]);
     g.ready = true;
     std::vector<float> P;
     guide_init_params(c.seed, P);
@@ -963,8 +969,8 @@ xt_status xt_guide_forward(xt_scene* s, const float* queries, uint32_t n, float*
     hipError_t e = dalloc(&dout, (size_t)n * G_OUT);
     if (e == hipSuccess) e = hipMemcpyAsync(dq, queries, (size_t)n * 9 * 4, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) {
-        const uint32_t grid = std::max(1u, std::min((n + 64 * GI_WAVES - 1) / (64 * GI_WAVES), (uint32_t)s->dev->num_cu));
-        hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(64 * GI_WAVES), 0, s->stream, s->guide.net, s->guide.gb, dq,
+        const uint32_t grid = std::max(1u, std::min((n + GI_BLOCK - 1) / GI_BLOCK, (uint32_t)s->dev->num_cu * 4));
+        hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(GI_BLOCK), 0, s->stream, s->guide.net, s->guide.gb, dq,
                            n, dout);
         e = hipGetLastError();
     }

@@ -169,30 +169,32 @@ XT_HD void guide_loss_grad(const float* y, f3 w, float weight, Store store) {
     }
 }
 
-// Mixture pdf from the stored per-lobe fields (mu, kappa, alpha, alpha * norm).
-template <class Load>
-XT_HD float guide_pdf_fields(Load ld, f3 w) {
+// Mixture pdf from stored lobes: l4(j) = (mu, kappa), l2(j) = (alpha,
+// alpha * vmf_norm(kappa)).
+template <class L4, class L2>
+XT_HD float guide_pdf_lobes(L4 l4, L2 l2, f3 w) {
     float p = 0.0f;
     for (int j = 0; j < G_LOBES; ++j) {
-        f3 mu = mk(ld(6 * j), ld(6 * j + 1), ld(6 * j + 2));
-        p += ld(6 * j + 5) * expf(ld(6 * j + 3) * (dot(w, mu) - 1.0f));
+        const float4 a = l4(j);
+        p += l2(j).y * expf(a.w * (dot(w, mk(a.x, a.y, a.z)) - 1.0f));
     }
     return p;
 }
 
 // Lobe by alpha with usel, then Jakob's sampler with (u1, u2).
-template <class Load>
-XT_HD f3 guide_sample_fields(Load ld, float usel, float u1, float u2) {
+template <class L4, class L2>
+XT_HD f3 guide_sample_lobes(L4 l4, L2 l2, float usel, float u1, float u2) {
     int j = 0;
     float acc = 0.0f;
     for (; j < G_LOBES - 1; ++j) {
-        acc += ld(6 * j + 4);
+        acc += l2(j).x;
         if (usel < acc) break;
     }
+    const float4 a = l4(j);
     VmfLobe l;
-    l.mu = mk(ld(6 * j), ld(6 * j + 1), ld(6 * j + 2));
-    l.kappa = ld(6 * j + 3);
-    l.alpha = ld(6 * j + 4);
+    l.mu = mk(a.x, a.y, a.z);
+    l.kappa = a.w;
+    l.alpha = 0.0f;
     return vmf_sample(l, u1, u2);
 }
 
