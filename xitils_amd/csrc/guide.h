@@ -37,12 +37,15 @@ struct GuideChain {
 };
 static_assert(sizeof(GuideChain) % 16 == 0, "GuideChain is copied in 16-byte pieces");
 
-// One training sample: query (p, n, wo), sampled direction, weight.
+// One training sample (64 B, four float4): query (p, n, wo), sampled
+// direction, weight.
 struct GuideSample {
     float q[9];
     float dir[3];
     float weight;
+    float pad[3];
 };
+static_assert(sizeof(GuideSample) == 64, "GuideSample is read as four float4");
 
 __global__ void k_guide_prepare(const float* P, GuideNet net);
 __global__ void k_guide_forward(GuideNet net, GuideBounds gb, const float* qin, uint32_t n, float* out);
@@ -57,5 +60,6 @@ __global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4*
 
 constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
 constexpr int GG_BLOCK = 64;    // threads per block of k_guide_grad
+constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize
 
 }  // namespace xt

@@ -41,16 +41,25 @@ template <int IN, int OUT, bool RELU, bool F32OUT>
 __device__ __forceinline__ void dense64(const __bf16* W, const float* bias, const __bf16* in, void* out,
                                         __bf16* out_t = nullptr) {
     const int l = __lane_id(), g = l >> 4, r = l & 15;
-#pragma unroll 1
-    for (int b = 0; b < OUT / 16; ++b) {
+    // all weight fragments of the layer first: one memory round trip per layer
+    bf16x8 wf[OUT / 16][IN / 32];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+    for (int b = 0; b < OUT / 16; ++b)
+#pragma unroll
+        for (int c = 0; c < IN / 32; ++c) wf[b][c] = frag(W, IN, 16 * b, 32 * c);
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        bf16x8 xf[IN / 32];
+#pragma unroll
+        for (int c = 0; c < IN / 32; ++c) xf[c] = frag(in, IN, 16 * t, 32 * c);
+        const int q = 16 * t + r;
+#pragma unroll
+        for (int b = 0; b < OUT / 16; ++b) {
             floatx4 acc;
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i] = bias[16 * b + 4 * g + i];
 #pragma unroll
-            for (int c = 0; c < IN / 32; ++c) acc = mfma(frag(W, IN, 16 * b, 32 * c), frag(in, IN, 16 * t, 32 * c), acc);
-            const int q = 16 * t + r;
+            for (int c = 0; c < IN / 32; ++c) acc = mfma(wf[b][c], xf[c], acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int o = 16 * b + 4 * g + i;
@@ -420,19 +429,31 @@ template <int IN, int OUTP>
 __device__ __forceinline__ void back_layer(const __bf16* Wt, const __bf16* dZ, const __bf16* Ht, __bf16* dZp,
                                            __bf16* dZpt) {
     const int l = __lane_id(), g = l >> 4, r = l & 15;
-#pragma unroll 1
+    bf16x8 wf[IN / 16][OUTP / 32];
+#pragma unroll
     for (int b = 0; b < IN / 16; ++b)
+#pragma unroll
+        for (int c = 0; c < OUTP / 32; ++c) wf[b][c] = frag(Wt, OUTP, 16 * b, 32 * c);
 #pragma unroll 1
-        for (int t = 0; t < 4; ++t) {
-            floatx4 acc = prod_tile<OUTP>(Wt, OUTP, 16 * b, dZ, OUTP, 16 * t);
+    for (int t = 0; t < 4; ++t) {
+        bf16x8 zf[OUTP / 32];
+#pragma unroll
+        for (int c = 0; c < OUTP / 32; ++c) zf[c] = frag(dZ, OUTP, 16 * t, 32 * c);
+        const int q = 16 * t + r;
+#pragma unroll
+        for (int b = 0; b < IN / 16; ++b) {
+            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < OUTP / 32; ++c) acc = mfma(wf[b][c], zf[c], acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int u = 16 * b + 4 * g + i, q = 16 * t + r;
+                const int u = 16 * b + 4 * g + i;
                 const float v = (float)Ht[u * 64 + q] > 0.0f ? acc[i] : 0.0f;
                 dZp[q * IN + u] = (__bf16)v;
                 dZpt[u * 64 + q] = (__bf16)v;
             }
         }
+    }
 }
 
 struct GradLds {
@@ -459,9 +480,14 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t si = base + l;
         const bool valid = si < n;
+        float4 s0 = make_float4(0, 0, 0, 0), s1 = s0, s2 = s0, s3 = s0;
         if (valid) {
-            const float* qq = smp[si].q;
-            guide_encode(gb, mk(qq[0], qq[1], qq[2]), mk(qq[3], qq[4], qq[5]), mk(qq[6], qq[7], qq[8]), X + l * G_IN);
+            const float4* sv = (const float4*)&smp[si];
+            s0 = sv[0];
+            s1 = sv[1];
+            s2 = sv[2];
+            s3 = sv[3];
+            guide_encode(gb, mk(s0.x, s0.y, s0.z), mk(s0.w, s1.x, s1.y), mk(s1.z, s1.w, s2.x), X + l * G_IN);
         } else {
             zero_row(X + l * G_IN);
         }
@@ -484,9 +510,8 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
                 dYt[o * 64 + l] = (__bf16)0.0f;
             }
             if (valid) {
-                const GuideSample& sm = smp[si];
-                const float w = fminf(sm.weight * weight_scale, weight_clip);
-                guide_loss_grad(Y + l * G_OUT_PAD, mk(sm.dir[0], sm.dir[1], sm.dir[2]), w, [&](int o, float v) {
+                const float w = fminf(s3.x * weight_scale, weight_clip);
+                guide_loss_grad(Y + l * G_OUT_PAD, mk(s2.y, s2.z, s2.w), w, [&](int o, float v) {
                     dY[l * 64 + o] = (__bf16)v;
                     dYt[o * 64 + l] = (__bf16)v;
                 });
@@ -514,15 +539,24 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
     for (int i = l; i < G_NPARAMS; i += 64) out[i] = s.acc[i];
 }
 
-// Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the
-// in-order sum of nparts partial rows times grad_scale; b1t, b2t = beta^t.
-__global__ void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                             const float* __restrict__ partial, uint32_t nparts, float lr, float b1, float b2, float eps,
-                             float b1t, float b2t, float grad_scale) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G_NPARAMS) return;
+// Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the sum
+// of nparts partial rows (8 interleaved row groups per parameter, then the
+// groups in order) times grad_scale; b1t, b2t = beta^t.  32 parameters per
+// 256-thread block.
+__global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                                                    const float* __restrict__ partial, uint32_t nparts, float lr, float b1,
+                                                    float b2, float eps, float b1t, float b2t, float grad_scale) {
+    __shared__ float red[8][32];
+    const int pl = threadIdx.x & 31, rg = threadIdx.x >> 5;
+    const int i = blockIdx.x * 32 + pl;
+    float gp = 0.0f;
+    if (i < G_NPARAMS)
+        for (uint32_t b = rg; b < nparts; b += 8) gp += partial[(size_t)b * G_NPARAMS + i];
+    red[rg][pl] = gp;
+    __syncthreads();
+    if (rg != 0 || i >= G_NPARAMS) return;
     float g = 0.0f;
-    for (uint32_t b = 0; b < nparts; ++b) g += partial[(size_t)b * G_NPARAMS + i];
+    for (int r = 0; r < 8; ++r) g += red[r][pl];
     g *= grad_scale;
     if (!isfinite(g)) g = 0.0f;
     const float mi = b1 * m[i] + (1.0f - b1) * g;
@@ -538,44 +572,64 @@ __global__ void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float
 //   target 0 (radiance): luminance(L_i) / pdf = luminance(C / E) / pdf
 //   target 1 (product):  luminance(f cos L_i) / pdf = luminance(C)
 // Positive finite weights are appended to out (order not fixed) and summed.
-__global__ void k_guide_finalize(const TrainRec* __restrict__ recs, uint32_t n, const float4* __restrict__ radiance,
+__global__ __launch_bounds__(GF_BLOCK) void k_guide_finalize(const TrainRec* __restrict__ recs, uint32_t n,
+                                                            const float4* __restrict__ radiance,
                                  uint32_t target, GuideSample* __restrict__ out, uint32_t* __restrict__ count,
                                  float* __restrict__ weight_sum) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float w = 0.0f;
     bool keep = false;
-    if (i < n && recs[i].valid) {
-        const TrainRec& r = recs[i];
-        const float4 L = radiance[r.id];
-        float C[3] = {L.x - r.pre[0], L.y - r.pre[1], L.z - r.pre[2]};
+    float4 e = make_float4(0, 0, 0, 0), d = e;
+    if (i < n) e = recs[i].e;
+    const uint32_t tag = __float_as_uint(e.w);
+    if (i < n && (tag & TREC_GUIDED)) {
+        d = recs[i].d;
+        const float4 pre = recs[i].f;
+        const float4 L = radiance[tag & ~TREC_GUIDED];
+        const float Wt[3] = {d.y, d.z, d.w}, Et[3] = {e.x, e.y, e.z};
+        float C[3] = {L.x - pre.x, L.y - pre.y, L.z - pre.z};
         for (int c = 0; c < 3; ++c) {
-            C[c] = r.W[c] > 0.0f ? fmaxf(C[c], 0.0f) / r.W[c] : 0.0f;
-            if (target == 0) C[c] = r.E[c] > 0.0f ? C[c] / r.E[c] : 0.0f;
+            C[c] = Wt[c] > 0.0f ? fmaxf(C[c], 0.0f) / Wt[c] : 0.0f;
+            if (target == 0) C[c] = Et[c] > 0.0f ? C[c] / Et[c] : 0.0f;
         }
         const float lum = 0.2126f * C[0] + 0.7152f * C[1] + 0.0722f * C[2];
-        w = target == 0 ? lum / r.pmix : lum;
-        keep = r.pmix > 0.0f && w > 0.0f && isfinite(w);
+        w = target == 0 ? lum / d.x : lum;
+        keep = d.x > 0.0f && w > 0.0f && isfinite(w);
     }
+    // one counter atomic per block: wave counts and weight sums through LDS
+    __shared__ uint32_t wcount[GF_BLOCK / 64];
+    __shared__ float wsums[GF_BLOCK / 64];
+    __shared__ uint32_t block_base;
+    const int l = __lane_id(), wave = threadIdx.x >> 6;
     const uint64_t mask = __ballot(keep);
-    if (mask == 0) return;
-    const int l = __lane_id();
-    const uint32_t nk = (uint32_t)__popcll(mask);
     const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << l) - 1ull));
-    const int leader = __ffsll((unsigned long long)mask) - 1;
     float wsum = keep ? w : 0.0f;
     for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
-    uint32_t base = 0;
-    if (l == leader) {
-        base = atomicAdd(count, nk);
-        atomicAdd(weight_sum, wsum);
+    if (l == 0) {
+        wcount[wave] = (uint32_t)__popcll(mask);
+        wsums[wave] = wsum;
     }
-    base = __shfl(base, leader);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        float ws = 0.0f;
+        for (int k = 0; k < GF_BLOCK / 64; ++k) {
+            const uint32_t c = wcount[k];
+            wcount[k] = tot;
+            tot += c;
+            ws += wsums[k];
+        }
+        block_base = tot ? atomicAdd(count, tot) : 0u;
+        if (tot) atomicAdd(weight_sum, ws);
+    }
+    __syncthreads();
+    const uint32_t base = block_base + wcount[wave];
     if (keep) {
-        const TrainRec& r = recs[i];
-        GuideSample& o = out[base + rank];
-        for (int k = 0; k < 9; ++k) o.q[k] = r.q[k];
-        for (int k = 0; k < 3; ++k) o.dir[k] = r.dir[k];
-        o.weight = w;
+        float4* o = (float4*)&out[base + rank];
+        o[0] = recs[i].a;
+        o[1] = recs[i].b;
+        o[2] = recs[i].c;
+        o[3] = make_float4(w, 0.0f, 0.0f, 0.0f);
     }
 }
 

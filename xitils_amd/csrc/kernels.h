@@ -58,22 +58,20 @@ struct ShadowQueue {
     float4* c;
 };
 
-// Per-vertex training record written while rendering: the query, the
-// sampled direction and its strategy pdf, the path throughput before the
-// vertex (W), the vertex's sample weight f cos / pdf (E) and the path
-// radiance collected up to and including the vertex's own light sample
-// (pre).  After the path ends, C = (L_path - pre) / W is the vertex's
-// estimate of f cos L_i / pdf along dir, and L_i = C / E.
+// Per-vertex training record written while rendering (96 B, six float4):
+//   a = (p.xyz, n.x)  b = (n.yz, wo.xy)  c = (wo.z, dir.xyz)
+//   d = (pdf, W.xyz)  e = (E.xyz, path id | 1 << 31 if the vertex was guided)
+//   f = (pre.xyz, 0)
+// p, n, wo: the query; dir and pdf: the sampled direction and its strategy
+// pdf; W: path throughput before the vertex; E: the vertex's sample weight
+// f cos / pdf; pre: the path radiance collected up to and including the
+// vertex's own light sample (written when the next vertex is shaded).
+// After the path ends, C = (L_path - pre) / W is the vertex's estimate of
+// f cos L_i / pdf along dir, and L_i = C / E.
 struct TrainRec {
-    float q[9];
-    float dir[3];
-    float pmix;
-    float W[3];
-    float E[3];
-    float pre[3];
-    uint32_t id;      // path id
-    uint32_t valid;   // 1 = guided vertex
+    float4 a, b, c, d, e, f;
 };
+constexpr uint32_t TREC_GUIDED = 1u << 31;
 
 // Decoded mixture per path-queue entry, SoA per lobe j:
 // lobes4[j * stride + i] = (mu, kappa), lobes2[j * stride + i] =

@@ -215,13 +215,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
             if (GUIDE && gr.recs) {
                 // the vertex that sampled this ray: radiance up to and including its light sample
                 const uint32_t rin = qin.rec[i];
-                if (rin != INVALID) {
-                    TrainRec& rr = gr.recs[rin];
-                    const f3 pre = rad + pend;
-                    rr.pre[0] = pre.x;
-                    rr.pre[1] = pre.y;
-                    rr.pre[2] = pre.z;
-                }
+                if (rin != INVALID) gr.recs[rin].f = f4(rad + pend, 0.0f);
             }
             const uint32_t slot = __float_as_uint(h.y);
             const bool hit = slot != INVALID;
@@ -458,15 +452,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                 qout.rec[ei] = keep ? ri : INVALID;
                 if (keep) {
                     TrainRec& rr = gr.recs[ri];
-                    rr.q[0] = s.p.x; rr.q[1] = s.p.y; rr.q[2] = s.p.z;
-                    rr.q[3] = s.sn.x; rr.q[4] = s.sn.y; rr.q[5] = s.sn.z;
-                    rr.q[6] = s.wo.x; rr.q[7] = s.wo.y; rr.q[8] = s.wo.z;
-                    rr.dir[0] = ext_d.x; rr.dir[1] = ext_d.y; rr.dir[2] = ext_d.z;
-                    rr.pmix = ext_pdf;
-                    rr.W[0] = ext_w.x; rr.W[1] = ext_w.y; rr.W[2] = ext_w.z;
-                    rr.E[0] = ext_e.x; rr.E[1] = ext_e.y; rr.E[2] = ext_e.z;
-                    rr.id = id;
-                    rr.valid = guided ? 1u : 0u;
+                    rr.a = make_float4(s.p.x, s.p.y, s.p.z, s.sn.x);
+                    rr.b = make_float4(s.sn.y, s.sn.z, s.wo.x, s.wo.y);
+                    rr.c = make_float4(s.wo.z, ext_d.x, ext_d.y, ext_d.z);
+                    rr.d = make_float4(ext_pdf, ext_w.x, ext_w.y, ext_w.z);
+                    rr.e = f4(ext_e, __uint_as_float(id | (guided ? TREC_GUIDED : 0u)));
                 }
             }
         }

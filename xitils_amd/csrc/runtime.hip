@@ -558,7 +558,7 @@ static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t
         g.step++;
         const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
         const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
-        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 255) / 256), dim3(256), 0, s->stream, g.P, g.m, g.v, g.partial,
+        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(256), 0, s->stream, g.P, g.m, g.v, g.partial,
                            grid, g.cfg.learning_rate, g.cfg.beta1, g.cfg.beta2, g.cfg.adam_eps, b1t, b2t,
                            1.0f / (float)cnt);
         xt_status st = guide_prepare(s);
@@ -771,7 +771,8 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 a = tm.mark();
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, g.d_count, 1u);
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, (uint32_t*)g.d_wsum, 1u);
-                hipLaunchKernelGGL(k_guide_finalize, dim3((rec_base + 255) / 256), dim3(256), 0, s->stream, g.recs, rec_base,
+                hipLaunchKernelGGL(k_guide_finalize, dim3((rec_base + GF_BLOCK - 1) / GF_BLOCK), dim3(GF_BLOCK), 0, s->stream,
+                                   g.recs, rec_base,
                                    rb.radiance, g.cfg.train_target, g.samples, g.d_count, g.d_wsum);
                 XT_LAUNCHED(result);
                 hipMemcpyAsync(g.host_io, g.d_count, 4, hipMemcpyDeviceToHost, s->stream);
