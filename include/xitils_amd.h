@@ -80,8 +80,18 @@ typedef struct xt_object_desc {
     const uint32_t* tri_materials; /* num_triangles material indices, or NULL */
 } xt_object_desc;
 
-/* SkySphereUniform (SkySphere.h:36-46) plus an optional sun disk.  Without a
- * sky the scene has Scene::skySphere == nullptr (escapes add nothing). */
+enum {
+    XT_ENV_MAP_NONE = 0,
+    XT_ENV_MAP_ANGULAR = 1,      /* SkySphereFromImage: Debevec angular map (SkySphere.h:14-34) */
+    XT_ENV_MAP_LATLONG = 2       /* latitude-longitude map, importance-sampled under XT_FLAG_ENV_NEE */
+};
+
+/* SkySphereUniform (SkySphere.h:36-46) or SkySphereFromImage (SkySphere.h:
+ * 14-34), plus an optional sun disk.  Without a sky the scene has
+ * Scene::skySphere == nullptr (escapes add nothing).  An image sky replaces
+ * the constant colour; it is looked up with Texture::rgb's bilinear filter
+ * (Texture.h:83-103).  Lat-long maps: u = (atan2(d.z, d.x) + pi) / 2pi,
+ * v = acos(d.y) / pi (y up, row 0 at the zenith), wrapping in u. */
 typedef struct xt_env {
     int32_t has_sky;
     float sky[3];
@@ -89,6 +99,9 @@ typedef struct xt_env {
     float sun_one_minus_cos;/* 1 - cos(half angle); 0 = no sun */
     float sun_sin2;         /* sin^2(half angle) */
     float sun_radiance[3];
+    uint32_t map_kind;      /* XT_ENV_MAP_* */
+    uint32_t map_width, map_height;
+    const float* map_rgb;   /* map_width * map_height * 3 floats, row 0 first (copied) */
 } xt_env;
 
 typedef struct xt_scene_desc {

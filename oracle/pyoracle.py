@@ -43,6 +43,10 @@ def lib():
                                            C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_scene_add_object.restype = C.c_int
         L.orc_scene_set_env.argtypes = [C.c_void_p, C.c_int, f32p, f32p, C.c_float, C.c_float, f32p]
+        L.orc_scene_set_env_map.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, f32p]
+        L.orc_env_radiance.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+        L.orc_env_map_pdf.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+        L.orc_env_map_sample.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
         L.orc_scene_build.argtypes = [C.c_void_p]
         L.orc_scene_num_prims.argtypes = [C.c_void_p]
         L.orc_scene_num_prims.restype = C.c_uint32
@@ -100,6 +104,9 @@ class OracleScene:
         omc, sin2 = e.sun_terms()
         L.orc_scene_set_env(self.h, int(e.has_sky), np.array(e.sky, np.float32), np.array(e.sun_dir, np.float32),
                             float(omc), float(sin2), np.array(e.sun_radiance, np.float32))
+        if getattr(e, "map_kind", 0) and e.map is not None:
+            m = np.ascontiguousarray(e.map, np.float32)
+            L.orc_scene_set_env_map(self.h, int(e.map_kind), m.shape[1], m.shape[0], m.reshape(-1))
         L.orc_scene_build(self.h)
 
     def __del__(self):
@@ -137,6 +144,24 @@ class OracleScene:
         lib().orc_render(self.h, cam, ip, fp, _ptr(film32), _ptr(film64), stats, nthreads)
         self.last_stats = stats
         return film32 if film32 is not None else film64
+
+    def env_radiance(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros_like(d)
+        lib().orc_env_radiance(self.h, d, d.shape[0], out)
+        return out
+
+    def env_map_pdf(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(d.shape[0], np.float32)
+        lib().orc_env_map_pdf(self.h, d, d.shape[0], out)
+        return out
+
+    def env_map_sample(self, u2):
+        u = np.ascontiguousarray(u2, np.float32).reshape(-1, 2)
+        out = np.zeros((u.shape[0], 4), np.float32)
+        lib().orc_env_map_sample(self.h, u, u.shape[0], out)
+        return out[:, :3], out[:, 3]
 
     def sample_light(self, seed, pixel, sample, slot, uniform_bary=False):
         out = np.zeros(12, np.float32)

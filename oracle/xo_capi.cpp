@@ -60,6 +60,40 @@ void orc_scene_set_env(void* s, int has_sky, const float* sky_rgb, const float* 
     sc->env.sunL = V3(sun_rgb[0], sun_rgb[1], sun_rgb[2]);
 }
 
+// Image sky: kind 1 = angular map (SkySphereFromImage), 2 = lat-long.
+void orc_scene_set_env_map(void* s, int kind, int width, int height, const float* rgb) {
+    Env& e = ((Scene*)s)->env;
+    e.mapKind = kind;
+    e.mapW = width;
+    e.mapH = height;
+    e.map.assign(rgb, rgb + 3 * (size_t)width * height);
+    e.buildMapDistribution();
+}
+
+// SkySphere::getRadiance (+ sun) for n ray directions.
+void orc_env_radiance(void* s, const float* dirs, int n, float* out) {
+    const Env& e = ((Scene*)s)->env;
+    for (int i = 0; i < n; ++i) {
+        V3 L = e.radiance(V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    }
+}
+
+// Lat-long distribution: solid-angle pdf of unit directions, and samples
+// (direction + pdf) from uniform pairs.
+void orc_env_map_pdf(void* s, const float* dirs, int n, float* out) {
+    const Env& e = ((Scene*)s)->env;
+    for (int i = 0; i < n; ++i) out[i] = e.mapPdf(V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+void orc_env_map_sample(void* s, const float* u2, int n, float* out4) {
+    const Env& e = ((Scene*)s)->env;
+    for (int i = 0; i < n; ++i) {
+        float pdf;
+        V3 d = e.sampleMap(u2[2 * i], u2[2 * i + 1], &pdf);
+        out4[4 * i] = d.x; out4[4 * i + 1] = d.y; out4[4 * i + 2] = d.z; out4[4 * i + 3] = pdf;
+    }
+}
+
 int orc_scene_build(void* s) {
     ((Scene*)s)->build();
     return 0;

@@ -581,16 +581,153 @@ struct Env {
     float sunOmc = 0.0f;
     float sunSin2 = 0.0f;
     V3 sunL = V3(0.0f);
+    // Image sky (replaces the constant colour): 1 = angular map as
+    // SkySphereFromImage (SkySphere.h:14-34), 2 = latitude-longitude map
+    // (new: u = (atan2(d.z, d.x) + pi) / 2pi, v = acos(d.y) / pi, y up, row 0
+    // at the zenith), with a 2D piecewise-constant importance distribution.
+    int mapKind = 0;
+    int mapW = 0, mapH = 0;
+    std::vector<float> map;                      // rgb, row-major, row 0 first
+    std::vector<float> margCdf, condCdf, pdfPix; // lat-long distribution (buildMapDistribution)
+
     bool hasSun() const { return !sunL.isZero() && sunOmc > 0.0f; }
     // Inside the disk: d . s > 0 and |d x s|^2 <= sin^2(half angle).
     bool inSun(const V3& dUnit) const {
         return dot(dUnit, sunDir) > 0.0f && cross(dUnit, sunDir).lengthSq() <= sunSin2;
     }
-    // SkySphereUniform::getRadiance (SkySphere.h:40-42) plus the sun disk.
+    // Texture::rgb(x, y) (Texture.h:76-80) with warpClamp (Texture.h:161-171);
+    // the lat-long map wraps in x.
+    V3 texel(int x, int y) const {
+        if (mapKind == 2) {
+            x %= mapW;
+            if (x < 0) x += mapW;
+        } else {
+            x = std::min(std::max(x, 0), mapW - 1);
+        }
+        y = std::min(std::max(y, 0), mapH - 1);
+        const float* t = &map[3 * ((size_t)y * mapW + x)];
+        return V3(t[0], t[1], t[2]);
+    }
+    // Texture::rgb(uv), bilinear filter (Texture.h:83-103)
+    V3 bilinear(float u, float v) const {
+        int x0 = (int)floorf(u * mapW - 0.5f);
+        int y0 = (int)floorf(v * mapH - 0.5f);
+        int x1 = x0 + 1, y1 = y0 + 1;
+        float wx1 = (u * mapW - 0.5f) - x0;
+        float wy1 = (v * mapH - 0.5f) - y0;
+        float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
+        return wx0 * wy0 * texel(x0, y0) + wx0 * wy1 * texel(x0, y1) + wx1 * wy0 * texel(x1, y0) +
+               wx1 * wy1 * texel(x1, y1);
+    }
+    // lat-long coordinates of a unit direction
+    static void latlongUV(const V3& d, float* u, float* v) {
+        *u = (atan2f(d.z, d.x) + Pi) / (2.0f * Pi);
+        *v = acosf(std::min(std::max(d.y, -1.0f), 1.0f)) / Pi;
+    }
+    V3 skyRadiance(const V3& wi) const {
+        if (mapKind == 1) {                         // SkySphere.h:20-33
+            V3 d = normalize(-wi);
+            float sq = sqrtf(d.x * d.x + d.y * d.y);
+            if (sq == 0.0f) return V3(0.0f);
+            float r = (float)((1.0 / M_PI) * (double)acosf(d.z) / (double)sq);
+            float u = (d.x * r + 1.0f) / 2.0f;
+            float v = (d.y * r + 1.0f) / 2.0f;
+            return bilinear(u, v);
+        }
+        if (mapKind == 2) {
+            float u, v;
+            latlongUV(normalize(wi), &u, &v);
+            return bilinear(u, v);
+        }
+        return sky;                                 // SkySphere.h:40-42
+    }
+    // SkySphere::getRadiance plus the sun disk.
     V3 radiance(const V3& d) const {
-        V3 L = sky;
+        V3 L = skyRadiance(d);
         if (hasSun() && inSun(normalize(d))) L += sunL;
         return L;
+    }
+
+    // Lat-long importance distribution: texel weight = the largest luminance
+    // of its 3x3 neighbourhood (the bilinear lookup inside a pixel reads
+    // those texels) times sin(theta) at the row centre; rows by a marginal
+    // CDF, columns by per-row CDFs, sums in double.  pdfPix folds in the
+    // Jacobian W H / (2 pi^2); the solid-angle pdf is pdfPix / sin(theta).
+    void buildMapDistribution() {
+        margCdf.clear(); condCdf.clear(); pdfPix.clear();
+        if (mapKind != 2 || mapW <= 0 || mapH <= 0) return;
+        const int W = mapW, H = mapH;
+        auto lum = [&](int x, int y) {
+            V3 t = texel(x, y);
+            return 0.2126 * t.x + 0.7152 * t.y + 0.0722 * t.z;
+        };
+        std::vector<double> w((size_t)W * H);
+        for (int y = 0; y < H; ++y) {
+            double st = std::sin(M_PI * (y + 0.5) / H);
+            for (int x = 0; x < W; ++x) {
+                double m = 0.0;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) m = std::max(m, lum(x + dx, y + dy));
+                w[(size_t)y * W + x] = m * st;
+            }
+        }
+        std::vector<double> row(H, 0.0);
+        double total = 0.0;
+        condCdf.assign((size_t)H * (W + 1), 0.0f);
+        for (int y = 0; y < H; ++y) {
+            double acc = 0.0;
+            for (int x = 0; x < W; ++x) acc += w[(size_t)y * W + x];
+            row[y] = acc;
+            total += acc;
+            double run = 0.0;
+            for (int x = 0; x < W; ++x) {
+                condCdf[(size_t)y * (W + 1) + x] = acc > 0.0 ? (float)(run / acc) : (float)x / (float)W;
+                run += w[(size_t)y * W + x];
+            }
+            condCdf[(size_t)y * (W + 1) + W] = 1.0f;
+        }
+        margCdf.assign(H + 1, 0.0f);
+        double run = 0.0;
+        for (int y = 0; y < H; ++y) {
+            margCdf[y] = total > 0.0 ? (float)(run / total) : (float)y / (float)H;
+            run += row[y];
+        }
+        margCdf[H] = 1.0f;
+        pdfPix.assign((size_t)W * H, 0.0f);
+        for (size_t k = 0; k < w.size(); ++k)
+            pdfPix[k] = total > 0.0 ? (float)(w[k] / total * W * H / (2.0 * M_PI * M_PI)) : 0.0f;
+    }
+    // largest i in [0, n) with cdf[i] <= u (cdf[0] = 0, cdf[n] = 1)
+    static int cdfFind(const float* cdf, int n, float u) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cdf[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
+    V3 sampleMap(float u1, float u2, float* pdf) const {
+        const int W = mapW, H = mapH;
+        int y = cdfFind(margCdf.data(), H, u1);
+        float dy = margCdf[y + 1] - margCdf[y];
+        float fy = dy > 0.0f ? (u1 - margCdf[y]) / dy : 0.5f;
+        const float* c = &condCdf[(size_t)y * (W + 1)];
+        int x = cdfFind(c, W, u2);
+        float dx = c[x + 1] - c[x];
+        float fx = dx > 0.0f ? (u2 - c[x]) / dx : 0.5f;
+        float u = (x + fx) / (float)W, v = (y + fy) / (float)H;
+        float phi = 2.0f * Pi * u - Pi, theta = Pi * v;
+        float st = sinf(theta);
+        *pdf = st > 0.0f ? pdfPix[(size_t)y * W + x] / st : 0.0f;
+        return V3(st * cosf(phi), cosf(theta), st * sinf(phi));
+    }
+    float mapPdf(const V3& dUnit) const {
+        float u, v;
+        latlongUV(dUnit, &u, &v);
+        int x = std::min((int)(u * mapW), mapW - 1), y = std::min((int)(v * mapH), mapH - 1);
+        float st = sqrtf(std::max(0.0f, 1.0f - dUnit.y * dUnit.y));
+        return st > 0.0f ? pdfPix[(size_t)y * mapW + x] / st : 0.0f;
     }
 };
 
@@ -842,17 +979,21 @@ struct EnvNee {
     float pSelect;     // probability of sampling the environment
     float pSun;        // within env: probability of the sun cone
     EnvNee(const Env& e, bool enabled, bool haveArea) : env(e) {
-        on = enabled && e.hasSky && (e.hasSun() || !e.sky.isZero());
+        bool sky = !e.sky.isZero() || e.mapKind != 0;
+        on = enabled && e.hasSky && (e.hasSun() || sky);
         pSelect = on ? (haveArea ? 0.5f : 1.0f) : 0.0f;
-        bool sky = !e.sky.isZero();
         pSun = e.hasSun() ? (sky ? 0.5f : 1.0f) : 0.0f;
     }
     float sunPdf() const { return 1.0f / (2.0f * Pi * env.sunOmc); }
-    // solid-angle pdf of the env mixture (before pSelect)
+    // solid-angle pdf of the env mixture (before pSelect): sun cone, then the
+    // lat-long map distribution or a cosine lobe about n
     float pdf(const V3& dUnit, const V3& n) const {
         float p = 0.0f;
         if (pSun > 0.0f && env.inSun(dUnit)) p += pSun * sunPdf();
-        if (pSun < 1.0f) p += (1.0f - pSun) * (float)((double)clampPositive(dot(dUnit, n)) / M_PI);
+        if (pSun < 1.0f) {
+            if (env.mapKind == 2) p += (1.0f - pSun) * env.mapPdf(dUnit);
+            else p += (1.0f - pSun) * (float)((double)clampPositive(dot(dUnit, n)) / M_PI);
+        }
         return p;
     }
     V3 sample(const V3& n, float usel, float u1, float u2) const {
@@ -864,6 +1005,10 @@ struct EnvNee {
             float phi = 2.0f * Pi * u2;
             Basis b(env.sunDir);
             return b.toGlobal(cosT, sinT * cosf(phi), sinT * sinf(phi));
+        }
+        if (env.mapKind == 2) {
+            float pdfMap;
+            return env.sampleMap(u1, u2, &pdfMap);
         }
         return sampleCosHemisphere(n, u1, u2);
     }

@@ -52,7 +52,9 @@ class xt_object_desc(C.Structure):
 
 class xt_env(C.Structure):
     _fields_ = [("has_sky", C.c_int32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
-                ("sun_one_minus_cos", C.c_float), ("sun_sin2", C.c_float), ("sun_radiance", C.c_float * 3)]
+                ("sun_one_minus_cos", C.c_float), ("sun_sin2", C.c_float), ("sun_radiance", C.c_float * 3),
+                ("map_kind", C.c_uint32), ("map_width", C.c_uint32), ("map_height", C.c_uint32),
+                ("map_rgb", C.c_void_p)]
 
 
 class xt_scene_desc(C.Structure):
@@ -200,9 +202,16 @@ def vmf_fit(device, num_lobes, samples, offsets, init_uniforms):
     return enc, steps
 
 
-def env_struct(e: "_sc.Env") -> xt_env:
+def env_struct(e: "_sc.Env", keep=None) -> xt_env:
     omc, sin2 = e.sun_terms()
     s = xt_env()
+    if e.map_kind and e.map is not None:
+        m = np.ascontiguousarray(e.map, np.float32)
+        if keep is not None:
+            keep.append(m)
+        s.map_kind = int(e.map_kind)
+        s.map_height, s.map_width = int(m.shape[0]), int(m.shape[1])
+        s.map_rgb = m.ctypes.data
     s.has_sky = int(e.has_sky)
     s.sky[:] = [float(x) for x in e.sky]
     s.sun_dir[:] = [float(x) for x in e.sun_dir]
@@ -259,7 +268,7 @@ class Scene:
         sd.num_objects = len(desc.objects)
         sd.materials = mats.ctypes.data
         sd.num_materials = mats.shape[0]
-        sd.env = env_struct(desc.env)
+        sd.env = env_struct(desc.env, keep)
         h = C.c_void_p()
         check(lib().xt_scene_create(device.h, C.byref(sd), C.byref(h)))
         self.h = h

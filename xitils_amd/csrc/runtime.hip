@@ -96,6 +96,10 @@ struct xt_scene {
     uint2* d_light_objs = nullptr;
     uint32_t* d_light_slots = nullptr;
     uint32_t* d_ovf = nullptr;        // traversal stack spill columns
+    float4* d_map = nullptr;          // image sky texels
+    float* d_map_pdf = nullptr;       // lat-long importance distribution
+    float* d_map_marg = nullptr;
+    float* d_map_cond = nullptr;
     uint32_t ovf_depth = 0;           // entries per lane beyond the LDS part
     uint32_t trace_grid_max = 0;      // blocks of the traversal kernels
     uint32_t bvh_nodes = 0, bvh_depth = 0;
@@ -151,6 +155,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_nodes); hipFree(s->d_tris); hipFree(s->d_normals); hipFree(s->d_mats);
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
     hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc);
+    hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
     s->rb.release();
     s->guide.release();
     if (s->stream) hipStreamDestroy(s->stream);
@@ -158,11 +163,74 @@ xt_status xt_scene_destroy(xt_scene* s) {
     return XT_OK;
 }
 
+// Lat-long importance distribution of an image sky: texel weight = the
+// largest luminance of its 3x3 neighbourhood (the bilinear lookup inside a
+// texel reads those texels) times sin(theta) at the row centre; a marginal
+// row CDF and per-row column CDFs, summed in double; pdf = weight / total *
+// W H / (2 pi^2), divided by sin(theta) at lookup.
+struct MapDistribution {
+    std::vector<float> pdf, marg, cond;
+};
+static MapDistribution build_map_distribution(const float* rgb, int W, int H) {
+    auto lum = [&](int x, int y) {
+        x %= W;
+        if (x < 0) x += W;
+        y = std::min(std::max(y, 0), H - 1);
+        const float* t = rgb + 3 * ((size_t)y * W + x);
+        return 0.2126 * t[0] + 0.7152 * t[1] + 0.0722 * t[2];
+    };
+    std::vector<double> w((size_t)W * H);
+    for (int y = 0; y < H; ++y) {
+        const double st = std::sin(M_PI * (y + 0.5) / H);
+        for (int x = 0; x < W; ++x) {
+            double m = 0.0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) m = std::max(m, lum(x + dx, y + dy));
+            w[(size_t)y * W + x] = m * st;
+        }
+    }
+    MapDistribution d;
+    std::vector<double> row(H, 0.0);
+    double total = 0.0;
+    d.cond.assign((size_t)H * (W + 1), 0.0f);
+    for (int y = 0; y < H; ++y) {
+        double acc = 0.0;
+        for (int x = 0; x < W; ++x) acc += w[(size_t)y * W + x];
+        row[y] = acc;
+        total += acc;
+        double run = 0.0;
+        for (int x = 0; x < W; ++x) {
+            d.cond[(size_t)y * (W + 1) + x] = acc > 0.0 ? (float)(run / acc) : (float)x / (float)W;
+            run += w[(size_t)y * W + x];
+        }
+        d.cond[(size_t)y * (W + 1) + W] = 1.0f;
+    }
+    d.marg.assign(H + 1, 0.0f);
+    double run = 0.0;
+    for (int y = 0; y < H; ++y) {
+        d.marg[y] = total > 0.0 ? (float)(run / total) : (float)y / (float)H;
+        run += row[y];
+    }
+    d.marg[H] = 1.0f;
+    d.pdf.assign((size_t)W * H, 0.0f);
+    for (size_t k = 0; k < w.size(); ++k)
+        d.pdf[k] = total > 0.0 ? (float)(w[k] / total * W * H / (2.0 * M_PI * M_PI)) : 0.0f;
+    return d;
+}
+
 xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** out) {
     if (!dev || !desc || !out) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (desc->num_objects > 0 && !desc->objects) return fail(XT_ERR_INVALID_ARGUMENT, "objects is NULL");
     if (desc->num_materials == 0 || !desc->materials) return fail(XT_ERR_INVALID_ARGUMENT, "no materials");
+    const xt_env& env_in = desc->env;
+    if (env_in.map_kind > XT_ENV_MAP_LATLONG) return fail(XT_ERR_INVALID_ARGUMENT, "unknown env map kind");
+    if (env_in.map_kind != XT_ENV_MAP_NONE) {
+        if (!env_in.map_rgb || env_in.map_width == 0 || env_in.map_height == 0)
+            return fail(XT_ERR_INVALID_ARGUMENT, "env map without texels");
+        if ((uint64_t)env_in.map_width * env_in.map_height > (1ull << 28))
+            return fail(XT_ERR_TOO_LARGE, "env map larger than 2^28 texels");
+    }
     for (uint32_t m = 0; m < desc->num_materials; ++m)
         if (desc->materials[m].type >= XT_MAT_COUNT_)
             return fail(XT_ERR_UNSUPPORTED_MATERIAL, "material " + std::to_string(m) + " has unknown type " +
@@ -352,6 +420,36 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     ds.env.sun_sin2 = e.sun_sin2;
     ds.env.sun_L = mk(e.sun_radiance[0], e.sun_radiance[1], e.sun_radiance[2]);
     ds.env.sun_frame = make_frame(ds.env.sun_dir);
+    ds.env.map_kind = (int)e.map_kind;
+    if (e.map_kind != XT_ENV_MAP_NONE) {
+        const int W = (int)e.map_width, H = (int)e.map_height;
+        std::vector<float4> tex((size_t)W * H);
+        for (size_t k = 0; k < tex.size(); ++k)
+            tex[k] = make_float4(e.map_rgb[3 * k], e.map_rgb[3 * k + 1], e.map_rgb[3 * k + 2], 0.0f);
+        auto up = [&](auto** dst, const auto& v) -> hipError_t {
+            hipError_t r = dalloc(dst, v.size());
+            if (r == hipSuccess) r = hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+            return r;
+        };
+        hipError_t r = up(&s->d_map, tex);
+        if (r == hipSuccess && e.map_kind == XT_ENV_MAP_LATLONG) {
+            MapDistribution md = build_map_distribution(e.map_rgb, W, H);
+            r = up(&s->d_map_pdf, md.pdf);
+            if (r == hipSuccess) r = up(&s->d_map_marg, md.marg);
+            if (r == hipSuccess) r = up(&s->d_map_cond, md.cond);
+        }
+        if (r != hipSuccess) {
+            xt_scene_destroy(s);
+            return fail(r == hipErrorOutOfMemory ? XT_ERR_OUT_OF_MEMORY : XT_ERR_DEVICE,
+                        std::string("env map upload failed: ") + hipGetErrorString(r));
+        }
+        ds.env.map_w = W;
+        ds.env.map_h = H;
+        ds.env.map = s->d_map;
+        ds.env.map_pdf = s->d_map_pdf;
+        ds.env.map_marg = s->d_map_marg;
+        ds.env.map_cond = s->d_map_cond;
+    }
     s->bvh_nodes = bvh.num_nodes;
     s->bvh_depth = bvh.depth;
     s->bytes_nodes = (uint64_t)bvh.num_nodes * 128;

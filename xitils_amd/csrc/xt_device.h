@@ -34,6 +34,13 @@ struct DevEnv {
     float sun_sin2;  // sin^2(half angle)
     f3 sun_L;
     Frame sun_frame; // make_frame(sun_dir), built once on the host
+    // image sky: XT_ENV_MAP_ANGULAR / XT_ENV_MAP_LATLONG (0: constant sky)
+    int map_kind;
+    int map_w, map_h;
+    const float4* map;       // texels (rgb, pad), row 0 first
+    const float* map_pdf;    // lat-long: per texel solid-angle pdf times sin(theta)
+    const float* map_marg;   // lat-long: row CDF (map_h + 1)
+    const float* map_cond;   // lat-long: per-row column CDFs (map_h x (map_w + 1))
 };
 
 struct DevScene {
@@ -286,11 +293,91 @@ XT_HD bool env_in_sun(const DevEnv& e, f3 du) {
     f3 c = cross(du, e.sun_dir);
     return dot(du, e.sun_dir) > 0.0f && dot(c, c) <= e.sun_sin2;
 }
-// SkySphereUniform::getRadiance (SkySphere.h:40-42) plus the sun disk.
+// Texture::rgb(x, y) (Texture.h:76-80) with warpClamp (Texture.h:161-171);
+// the lat-long map wraps in x.
+XT_HD f3 env_texel(const DevEnv& e, int x, int y) {
+    if (e.map_kind == XT_ENV_MAP_LATLONG) {
+        x %= e.map_w;
+        if (x < 0) x += e.map_w;
+    } else {
+        x = min(max(x, 0), e.map_w - 1);
+    }
+    y = min(max(y, 0), e.map_h - 1);
+    const float4 t = e.map[(size_t)y * e.map_w + x];
+    return mk(t.x, t.y, t.z);
+}
+// Texture::rgb(uv), bilinear filter (Texture.h:83-103)
+XT_HD f3 env_bilinear(const DevEnv& e, float u, float v) {
+    const int x0 = (int)floorf(u * e.map_w - 0.5f);
+    const int y0 = (int)floorf(v * e.map_h - 0.5f);
+    const float wx1 = (u * e.map_w - 0.5f) - x0;
+    const float wy1 = (v * e.map_h - 0.5f) - y0;
+    const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
+    return wx0 * wy0 * env_texel(e, x0, y0) + wx0 * wy1 * env_texel(e, x0, y0 + 1) + wx1 * wy0 * env_texel(e, x0 + 1, y0) +
+           wx1 * wy1 * env_texel(e, x0 + 1, y0 + 1);
+}
+// lat-long coordinates of a unit direction (y up, row 0 at the zenith)
+XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
+    const float PI = 3.14159265358979323846f;
+    *u = (atan2f(d.z, d.x) + PI) / (2.0f * PI);
+    *v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI;
+}
+// SkySphereFromImage::getRadiance (SkySphere.h:20-33), the lat-long map, or
+// SkySphereUniform::getRadiance (SkySphere.h:40-42).
+XT_HD f3 env_sky(const DevEnv& e, f3 wi) {
+    if (e.map_kind == XT_ENV_MAP_ANGULAR) {
+        const f3 d = normalize(-wi);
+        const float sq = sqrtf(d.x * d.x + d.y * d.y);
+        if (sq == 0.0f) return splat(0.0f);
+        const float r = (float)((1.0 / 3.14159265358979323846) * (double)acosf(d.z) / (double)sq);
+        return env_bilinear(e, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
+    }
+    if (e.map_kind == XT_ENV_MAP_LATLONG) {
+        float u, v;
+        env_latlong_uv(normalize(wi), &u, &v);
+        return env_bilinear(e, u, v);
+    }
+    return e.sky;
+}
+// the sky plus the sun disk
 XT_HD f3 env_radiance(const DevEnv& e, f3 d) {
-    f3 L = e.sky;
+    f3 L = env_sky(e, d);
     if (env_has_sun(e) && env_in_sun(e, normalize(d))) L += e.sun_L;
     return L;
+}
+// largest i in [0, n) with cdf[i] <= u (cdf[0] = 0, cdf[n] = 1)
+XT_HD int cdf_find(const float* cdf, int n, float u) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// lat-long importance sample: row by the marginal CDF, column by the row's
+// CDF, the remainders as offsets inside the texel.
+XT_HD f3 env_map_sample(const DevEnv& e, float u1, float u2) {
+    const float PI = 3.14159265358979323846f;
+    const int W = e.map_w, H = e.map_h;
+    const int y = cdf_find(e.map_marg, H, u1);
+    const float dy = e.map_marg[y + 1] - e.map_marg[y];
+    const float fy = dy > 0.0f ? (u1 - e.map_marg[y]) / dy : 0.5f;
+    const float* c = e.map_cond + (size_t)y * (W + 1);
+    const int x = cdf_find(c, W, u2);
+    const float dx = c[x + 1] - c[x];
+    const float fx = dx > 0.0f ? (u2 - c[x]) / dx : 0.5f;
+    const float u = (x + fx) / (float)W, v = (y + fy) / (float)H;
+    const float phi = 2.0f * PI * u - PI, theta = PI * v;
+    const float st = sinf(theta);
+    return mk(st * cosf(phi), cosf(theta), st * sinf(phi));
+}
+XT_HD float env_map_pdf(const DevEnv& e, f3 du) {
+    float u, v;
+    env_latlong_uv(du, &u, &v);
+    const int x = min((int)(u * e.map_w), e.map_w - 1), y = min((int)(v * e.map_h), e.map_h - 1);
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - du.y * du.y));
+    return st > 0.0f ? e.map_pdf[(size_t)y * e.map_w + x] / st : 0.0f;
 }
 
 // Environment light strategy: pick env vs area lights, then sun cone vs
@@ -301,7 +388,7 @@ struct EnvNee {
 };
 XT_HD EnvNee env_nee_setup(const DevEnv& e, bool enabled, bool have_area) {
     EnvNee n;
-    bool sky = !is_zero(e.sky);
+    bool sky = !is_zero(e.sky) || e.map_kind != 0;
     n.on = enabled && e.has_sky && (env_has_sun(e) || sky);
     n.p_select = n.on ? (have_area ? 0.5f : 1.0f) : 0.0f;
     n.p_sun = env_has_sun(e) ? (sky ? 0.5f : 1.0f) : 0.0f;
@@ -311,7 +398,10 @@ XT_HD float env_nee_pdf(const DevEnv& e, const EnvNee& s, f3 du, f3 n) {
     const float PI = 3.14159265358979323846f;
     float p = 0.0f;
     if (s.p_sun > 0.0f && env_in_sun(e, du)) p += s.p_sun * (1.0f / (2.0f * PI * e.sun_omc));
-    if (s.p_sun < 1.0f) p += (1.0f - s.p_sun) * cos_pdf(clamp_pos(dot(du, n)));
+    if (s.p_sun < 1.0f) {
+        if (e.map_kind == XT_ENV_MAP_LATLONG) p += (1.0f - s.p_sun) * env_map_pdf(e, du);
+        else p += (1.0f - s.p_sun) * cos_pdf(clamp_pos(dot(du, n)));
+    }
     return p;
 }
 // Sun cone (uniform in solid angle) or cosine sky about the shading normal
@@ -319,6 +409,7 @@ XT_HD float env_nee_pdf(const DevEnv& e, const EnvNee& s, f3 du, f3 n) {
 // choice is a select, not a divergent branch.
 XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, const Frame& fr, float usel, float u1, float u2) {
     const bool sun = usel < s.p_sun;
+    if (!sun && e.map_kind == XT_ENV_MAP_LATLONG) return env_map_sample(e, u1, u2);
     const DiskPt dp = disk_pt(u1, u2);
     float uo = u1 * e.sun_omc;
     float a = sun ? 1.0f - uo : safe_sqrt(1.0f - u1);

@@ -81,6 +81,9 @@ class ObjectDesc:
         return int(self.indices.shape[0])
 
 
+ENV_MAP_NONE, ENV_MAP_ANGULAR, ENV_MAP_LATLONG = 0, 1, 2
+
+
 @dataclass
 class Env:
     has_sky: bool = False
@@ -88,6 +91,10 @@ class Env:
     sun_dir: tuple = (0.0, 1.0, 0.0)
     sun_half_angle_deg: float = 0.0
     sun_radiance: tuple = (0.0, 0.0, 0.0)
+    # image sky: ENV_MAP_ANGULAR (SkySphereFromImage) or ENV_MAP_LATLONG, with
+    # map an (H, W, 3) float32 array, row 0 first
+    map_kind: int = ENV_MAP_NONE
+    map: np.ndarray = None
 
     def sun_terms(self):
         """(omc, sin2) of the disk half angle, in float64 then rounded."""
@@ -390,6 +397,87 @@ def furnace_object(albedo=0.6, width=32, height=32):
     s.env = Env(True, (1.0, 1.0, 1.0))
     cam = look_at((0.9, 0.7, -1.6), (0.0, 0.0, 0.0))
     s.camera = Camera(cam, math.radians(45.0), height / width)
+    return s
+
+
+# --------------------------------------------------------------------------- image skies
+
+def latlong_directions(width, height):
+    """Unit direction at each lat-long texel centre (y up, row 0 at the zenith,
+    u = (atan2(d.z, d.x) + pi) / 2pi)."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    phi = 2 * np.pi * u - np.pi
+    theta = np.pi * v
+    P, T = np.meshgrid(phi, theta)
+    return np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+
+
+def procedural_sky_latlong(width=2048, height=1024, sun_dir=(0.4, 0.6, -0.7), sun_half_angle_deg=0.5,
+                           sun_radiance=2e4, seed=5):
+    """FP32 lat-long HDR: a zenith-to-horizon gradient, a dim ground and a sun
+    disk of the given radiance baked into the texels (the importance sampler
+    finds it; there is no analytic sun)."""
+    d = latlong_directions(width, height)
+    up = np.clip(d[..., 1], -1, 1)
+    t = np.clip(up, 0, 1)[..., None]
+    zen = np.array([0.25, 0.45, 1.0])
+    hor = np.array([1.0, 0.95, 0.85])
+    sky = (1 - t) * hor + t * zen
+    ground = np.array([0.12, 0.1, 0.08]) * np.ones_like(sky)
+    img = np.where((up >= 0)[..., None], sky, ground)
+    sd = np.asarray(sun_dir, np.float64)
+    sd /= np.linalg.norm(sd)
+    c = d @ sd
+    img = img + (c >= math.cos(math.radians(sun_half_angle_deg)))[..., None] * sun_radiance
+    rng = np.random.default_rng(seed)
+    img = img * (1 + 0.05 * rng.random(img.shape))      # fine texture so texels differ
+    return img.astype(np.float32)
+
+
+def procedural_angular_probe(size=256, seed=6):
+    """Angular-map light probe (the layout SkySphereFromImage reads, SkySphere.h:
+    20-33): texel (u, v) in [0, 1]^2 -> (a, b) = (2u - 1, 2v - 1), r = |(a, b)|,
+    theta = pi r about -z; outside the unit disk stays black."""
+    u = (np.arange(size) + 0.5) / size
+    A, B = np.meshgrid(2 * u - 1, 2 * u - 1)
+    r = np.sqrt(A * A + B * B)
+    th = np.pi * np.minimum(r, 1)
+    s = np.where(r > 0, np.sin(th) / np.maximum(r, 1e-12), 0.0)
+    d = np.stack([A * s, B * s, np.cos(th)], -1)
+    base = 0.5 + 0.5 * d[..., 1:2] * np.array([0.3, 0.6, 1.0]) + 0.3 * (d[..., 0:1] > 0.5)
+    spot = np.exp(40 * (d @ np.array([0.0, 0.6, -0.8]) - 1))[..., None] * np.array([30.0, 25.0, 20.0])
+    img = base + spot
+    img = img * (1 + 0.05 * np.random.default_rng(seed).random(img.shape))
+    img[r > 1] = 0.0
+    return img.astype(np.float32)
+
+
+def sky_scene(kind=ENV_MAP_LATLONG, width=64, height=48, map_size=None, albedo=0.6):
+    """A Lambert ground quad with a GGX sphere, a metal box and a diffuse
+    sphere under an image sky (angular probe or lat-long HDR); the camera
+    sees both objects and sky."""
+    s = SceneDesc(name=f"sky_{'angular' if kind == ENV_MAP_ANGULAR else 'latlong'}")
+    g = s.add_material(diffuse(albedo))
+    ggx = s.add_material(ggx_lambert((0.7, 0.4, 0.3), 0.25, 0.04))
+    met = s.add_material(metal(0.3, (0.9, 0.8, 0.6)))
+    dif = s.add_material(diffuse(0.3, 0.6, 0.4))
+    qv, qi = quad((-4, 0, -4), (-4, 0, 4), (4, 0, 4), (4, 0, -4))
+    s.objects.append(ObjectDesc(qv, qi, material=g))
+    p, i, n = uv_sphere((0.0, 0.6, 0.0), 0.6)
+    s.objects.append(ObjectDesc(p, i, material=ggx, normals=n))
+    bv, bi = box_mesh((0.7, 0.7, 0.7), (1.3, 0.35, 0.5), 20.0)
+    s.objects.append(ObjectDesc(bv, bi, material=met))
+    p, i, n = uv_sphere((-1.2, 0.4, 0.6), 0.4)
+    s.objects.append(ObjectDesc(p, i, material=dif, normals=n))
+    if kind == ENV_MAP_ANGULAR:
+        m = procedural_angular_probe(map_size or 128)
+    else:
+        ms = map_size or 256
+        m = procedural_sky_latlong(ms, ms // 2, sun_half_angle_deg=3.0, sun_radiance=300.0)
+    s.env = Env(True, (0.0, 0.0, 0.0), map_kind=kind, map=m)
+    cam = look_at((0.5, 1.6, -4.2), (0.0, 0.5, 0.0))
+    s.camera = Camera(cam, math.radians(60.0), height / width)
     return s
 
 
