@@ -260,6 +260,34 @@ struct SkySphereWithSun : SkySphereUniform {
     }
 };
 
+// SkySphereFromImage (SkySphere.h:14-34): an angular-map light probe.  The
+// reference loads the probe with stb_image (Texture.h:17-23); this API takes
+// the decoded RGB float texels (row 0 first), so any loader can feed it.
+struct SkySphereFromImage : SkySphere {
+    std::vector<float> rgb;
+    int width = 0, height = 0;
+    SkySphereFromImage(const float* texels, int w, int h) : rgb(texels, texels + 3 * (size_t)w * h), width(w), height(h) {}
+    xt_env env() const override {
+        xt_env e{};
+        e.has_sky = 1;
+        e.map_kind = XT_ENV_MAP_ANGULAR;
+        e.map_width = (uint32_t)width;
+        e.map_height = (uint32_t)height;
+        e.map_rgb = rgb.data();
+        return e;
+    }
+};
+// Latitude-longitude HDR sky (no reference counterpart), importance-sampled
+// as a light when the integrator sets XT_FLAG_ENV_NEE.
+struct SkySphereLatLong : SkySphereFromImage {
+    SkySphereLatLong(const float* texels, int w, int h) : SkySphereFromImage(texels, w, h) {}
+    xt_env env() const override {
+        xt_env e = SkySphereFromImage::env();
+        e.map_kind = XT_ENV_MAP_LATLONG;
+        return e;
+    }
+};
+
 struct Camera {
     virtual ~Camera() = default;
     virtual xt_camera record() const = 0;
