@@ -293,28 +293,34 @@ XT_HD bool env_in_sun(const DevEnv& e, f3 du) {
     f3 c = cross(du, e.sun_dir);
     return dot(du, e.sun_dir) > 0.0f && dot(c, c) <= e.sun_sin2;
 }
+// An image sky as the lookups see it (plain values, so the out-of-line
+// lookup below takes no address of a kernel argument).
+struct EnvMapRef {
+    int kind, w, h;
+    const float4* texels;
+};
 // Texture::rgb(x, y) (Texture.h:76-80) with warpClamp (Texture.h:161-171);
 // the lat-long map wraps in x.
-XT_HD f3 env_texel(const DevEnv& e, int x, int y) {
-    if (e.map_kind == XT_ENV_MAP_LATLONG) {
-        x %= e.map_w;
-        if (x < 0) x += e.map_w;
+XT_HD f3 env_texel(const EnvMapRef& m, int x, int y) {
+    if (m.kind == XT_ENV_MAP_LATLONG) {
+        x %= m.w;
+        if (x < 0) x += m.w;
     } else {
-        x = min(max(x, 0), e.map_w - 1);
+        x = min(max(x, 0), m.w - 1);
     }
-    y = min(max(y, 0), e.map_h - 1);
-    const float4 t = e.map[(size_t)y * e.map_w + x];
+    y = min(max(y, 0), m.h - 1);
+    const float4 t = m.texels[(size_t)y * m.w + x];
     return mk(t.x, t.y, t.z);
 }
 // Texture::rgb(uv), bilinear filter (Texture.h:83-103)
-XT_HD f3 env_bilinear(const DevEnv& e, float u, float v) {
-    const int x0 = (int)floorf(u * e.map_w - 0.5f);
-    const int y0 = (int)floorf(v * e.map_h - 0.5f);
-    const float wx1 = (u * e.map_w - 0.5f) - x0;
-    const float wy1 = (v * e.map_h - 0.5f) - y0;
+XT_HD f3 env_bilinear(const EnvMapRef& m, float u, float v) {
+    const int x0 = (int)floorf(u * m.w - 0.5f);
+    const int y0 = (int)floorf(v * m.h - 0.5f);
+    const float wx1 = (u * m.w - 0.5f) - x0;
+    const float wy1 = (v * m.h - 0.5f) - y0;
     const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
-    return wx0 * wy0 * env_texel(e, x0, y0) + wx0 * wy1 * env_texel(e, x0, y0 + 1) + wx1 * wy0 * env_texel(e, x0 + 1, y0) +
-           wx1 * wy1 * env_texel(e, x0 + 1, y0 + 1);
+    return wx0 * wy0 * env_texel(m, x0, y0) + wx0 * wy1 * env_texel(m, x0, y0 + 1) + wx1 * wy0 * env_texel(m, x0 + 1, y0) +
+           wx1 * wy1 * env_texel(m, x0 + 1, y0 + 1);
 }
 // lat-long coordinates of a unit direction (y up, row 0 at the zenith)
 XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
@@ -322,22 +328,25 @@ XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
     *u = (atan2f(d.z, d.x) + PI) / (2.0f * PI);
     *v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI;
 }
-// SkySphereFromImage::getRadiance (SkySphere.h:20-33), the lat-long map, or
-// SkySphereUniform::getRadiance (SkySphere.h:40-42).
-XT_HD f3 env_sky(const DevEnv& e, f3 wi) {
-    if (e.map_kind == XT_ENV_MAP_ANGULAR) {
+// SkySphereFromImage::getRadiance (SkySphere.h:20-33) or the lat-long map.
+// Kept out of line: inlined, the lookup's registers lower the occupancy of
+// every kernel that evaluates the sky.
+static __host__ __device__ __noinline__ f3 env_sky_map(int kind, int w, int h, const float4* texels, f3 wi) {
+    const EnvMapRef m{kind, w, h, texels};
+    if (kind == XT_ENV_MAP_ANGULAR) {
         const f3 d = normalize(-wi);
         const float sq = sqrtf(d.x * d.x + d.y * d.y);
         if (sq == 0.0f) return splat(0.0f);
         const float r = (float)((1.0 / 3.14159265358979323846) * (double)acosf(d.z) / (double)sq);
-        return env_bilinear(e, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
+        return env_bilinear(m, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
     }
-    if (e.map_kind == XT_ENV_MAP_LATLONG) {
-        float u, v;
-        env_latlong_uv(normalize(wi), &u, &v);
-        return env_bilinear(e, u, v);
-    }
-    return e.sky;
+    float u, v;
+    env_latlong_uv(normalize(wi), &u, &v);
+    return env_bilinear(m, u, v);
+}
+// the image sky, or SkySphereUniform::getRadiance (SkySphere.h:40-42)
+XT_HD f3 env_sky(const DevEnv& e, f3 wi) {
+    return e.map_kind != XT_ENV_MAP_NONE ? env_sky_map(e.map_kind, e.map_w, e.map_h, e.map, wi) : e.sky;
 }
 // the sky plus the sun disk
 XT_HD f3 env_radiance(const DevEnv& e, f3 d) {
