@@ -1084,17 +1084,15 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
                 V3 d = envn.sample(isect.sn, rng.uni(slot, DIM_ENV_SEL), rng.uni(slot, DIM_ENV_U1), rng.uni(slot, DIM_ENV_U2));
                 float pe = envn.pSelect * envn.pdf(d, isect.sn);
                 V3 fc = bsdfCos(*isect.mat, isect, d);
-                if (pe > 0.0f && !fc.isZero()) {
+                float pb = bsdfPdf(*isect.mat, isect, d);
+                float mis = (pe * pe) / (pb * pb + pe * pe);
+                if (pe > 0.0f && !fc.isZero() && mis > 0.0f) {   // a NaN pdf drops the sample
                     Ray sr;
                     sr.d = d;
                     sr.o = isect.p + rp.eps * sr.d;
                     sr.tMax = Infinity;
                     st->shadowRays++;
-                    if (!sc.intersectAny(sr)) {
-                        float pb = bsdfPdf(*isect.mat, isect, d);
-                        float mis = (pe * pe) / (pb * pb + pe * pe);
-                        radiance += weight * mis * fc * sc.env.radiance(d) / pe;
-                    }
+                    if (!sc.intersectAny(sr)) radiance += weight * mis * fc * sc.env.radiance(d) / pe;
                 }
             }
         } else if (canSample) {

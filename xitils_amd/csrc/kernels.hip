@@ -298,7 +298,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     guided = GUIDE && gr.lobes4 != nullptr && !mat_is_delta(mat.type);
                     auto l4 = [&](int j) { return gr.lobes4[(size_t)j * gr.stride + i]; };
                     auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
+                    // (a NaN BSDF pdf -- GGX D at a grazing half vector -- stays NaN, so
+                    // the light-sample MIS below drops that sample as the plain path does)
                     auto strategy_pdf = [&](f3 dir, float pb) {
+                        if (pb != pb) return pb;
                         return pb > 0.0f ? gr.prob * guide_pdf_lobes(l4, l2, dir) + (1.0f - gr.prob) * pb : 0.0f;
                     };
                     if (guided) {
@@ -313,6 +316,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                         float pb;
                         const f3 fc = bsdf_eval(mat, s, wi, &pb);
                         pdfB = strategy_pdf(wi, pb);
+                        if (!(pdfB > 0.0f)) pdfB = 0.0f;
                         me = pdfB > 0.0f ? div(fc, pdfB) : splat(0.0f);
                         if (!finite3(me)) me = splat(0.0f);
                     } else {
@@ -342,8 +346,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                             float pb;
                             f3 fc = bsdf_eval(mat, s, ld, &pb);
                             if (guided) pb = strategy_pdf(ld, pb);
-                            if (pe > 0.0f && !is_zero(fc)) {
-                                float mis = (pe * pe) / (pb * pb + pe * pe);
+                            const float mis = (pe * pe) / (pb * pb + pe * pe);
+                            if (pe > 0.0f && !is_zero(fc) && mis > 0.0f) {   // NaN mis: dropped
                                 f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
                                 push_sh = true;
                                 sh_d = ld;
@@ -437,6 +441,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                 }
             }
         }
+        // guided light samples: a non-finite contribution is dropped, never summed
+        if (GUIDE && push_sh && !finite3(sh_c)) push_sh = false;
         // ---- compacted appends (whole block participates)
         const uint2 slots = block_append2<SHADE_BLOCK>(count_out, push_ext, count_shadow, push_sh, append_lds);
         const uint32_t ei = slots.x, si = slots.y;
