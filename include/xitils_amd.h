@@ -204,6 +204,31 @@ xt_status xt_trace_any(xt_scene* scene, const xt_ray* rays, uint32_t n, uint8_t*
 xt_status xt_render(xt_scene* scene, const xt_camera* camera, const xt_render_params* params, void* film,
                     int film_is_f64, int film_on_device);
 xt_status xt_render_get_stats(const xt_scene* scene, xt_render_stats* out);
+/* One render of a batch: the arguments of one xt_render call. */
+typedef struct xt_batch_item {
+    xt_scene* scene;
+    xt_camera camera;
+    xt_render_params params;
+    void* film;
+    int32_t film_is_f64;
+    int32_t film_on_device;
+    float ms;                /* out: GPU time of this item on its scene's stream (HIP events) */
+} xt_batch_item;
+
+/* Many RenderTarget::render calls at once (no reference counterpart; the
+ * closest is the tile scheduler of RenderTarget.h:84-127).  Each item gives
+ * the film xt_render would give, bit for bit, but its bounce loop runs as a
+ * HIP graph replayed once per batch with no host round trip (every kernel
+ * reads its queue count from device memory), and items on different scenes
+ * run concurrently on their scenes' streams.  Needs a bounce cap
+ * (1 <= max_depth <= 1024) and no XT_FLAG_GUIDE_TRAIN.  Graphs are cached
+ * per scene, camera, parameters and film. */
+xt_status xt_render_batch(xt_batch_item* items, uint32_t n);
+/* Autotuner: renders one batch of (camera, params) as a graph under 9
+ * launch shapes and keeps the fastest for this scene's later renders;
+ * out = {traversal blocks per CU, shade blocks per CU}. */
+xt_status xt_scene_autotune(xt_scene* scene, const xt_camera* camera, const xt_render_params* params, uint32_t out[2]);
+
 /* Measurement hook (no reference counterpart): renders of this scene use the
  * traversal variant that counts node and triangle fetches into the stats
  * (algorithmic-traffic accounting for the roofline).  Results are identical. */

@@ -44,6 +44,7 @@ STRUCTS = {
     "xt_render_params": api.xt_render_params,
     "xt_render_stats": api.xt_render_stats,
     "xt_guide_config": api.xt_guide_config,
+    "xt_batch_item": api.xt_batch_item,
 }
 
 

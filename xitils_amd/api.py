@@ -24,7 +24,8 @@ EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", 
             "xt_scene_info", "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats",
             "xt_scene_set_counting", "xt_guide_config_default", "xt_guide_init", "xt_guide_get_params",
             "xt_guide_set_params", "xt_guide_get_bounds", "xt_guide_forward", "xt_guide_gradient",
-            "xt_guide_train_step", "xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit"]
+            "xt_guide_train_step", "xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit", "xt_render_batch",
+            "xt_scene_autotune"]
 
 XT_FLAG_UNIFORM_BARY = 1
 XT_FLAG_ENV_NEE = 2
@@ -83,6 +84,11 @@ class xt_render_stats(C.Structure):
                 ("guide_samples", C.c_uint64)]
 
 
+class xt_batch_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", xt_camera), ("params", xt_render_params), ("film", C.c_void_p),
+                ("film_is_f64", C.c_int32), ("film_on_device", C.c_int32), ("ms", C.c_float)]
+
+
 class xt_guide_config(C.Structure):
     _fields_ = [("guide_prob", C.c_float), ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("adam_eps", C.c_float), ("minibatch", C.c_uint32), ("max_steps_per_batch", C.c_uint32),
@@ -129,7 +135,9 @@ def lib():
                                     C.c_uint32, C.c_void_p, C.c_int]
         L.xt_vmf_fit.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_int]
-        for f in ("xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit"):
+        L.xt_render_batch.argtypes = [C.c_void_p, C.c_uint32]
+        L.xt_scene_autotune.argtypes = [C.c_void_p, C.POINTER(xt_camera), C.POINTER(xt_render_params), C.c_void_p]
+        for f in ("xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit", "xt_render_batch", "xt_scene_autotune"):
             getattr(L, f).restype = C.c_int
         for f in ("xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy", "xt_scene_info",
                   "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting",
@@ -200,6 +208,34 @@ def vmf_fit(device, num_lobes, samples, offsets, init_uniforms):
     check(lib().xt_vmf_fit(device.h, num_lobes, s.ctypes.data, s.shape[0], off.ctypes.data, nf, iu.ctypes.data,
                            enc.ctypes.data, steps.ctypes.data, 0))
     return enc, steps
+
+
+def render_batch(jobs):
+    """jobs: list of (scene, width, height, spp, film or None, kwargs) -> the
+    films (host numpy, accumulated like Scene.render) and per-item GPU ms.
+    Films may also be (device pointer, f64) tuples."""
+    items = (xt_batch_item * max(1, len(jobs)))()
+    films = []
+    for k, (sc, w, h, spp, film, kw) in enumerate(jobs):
+        kw = dict(kw)
+        cam = kw.pop("camera", None)
+        items[k].scene = sc.h
+        items[k].camera = camera_struct(cam or sc.desc.camera)
+        items[k].params = Scene.params(w, h, spp, **kw)
+        if isinstance(film, tuple):
+            items[k].film = film[0]
+            items[k].film_is_f64 = int(film[1])
+            items[k].film_on_device = 1
+            films.append(None)
+        else:
+            if film is None:
+                film = np.zeros((h, w, 3), np.float32)
+            items[k].film = film.ctypes.data
+            items[k].film_is_f64 = int(film.dtype == np.float64)
+            items[k].film_on_device = 0
+            films.append(film)
+    check(lib().xt_render_batch(items, len(jobs)))
+    return films, [items[k].ms for k in range(len(jobs))]
 
 
 def env_struct(e: "_sc.Env", keep=None) -> xt_env:
@@ -339,6 +375,15 @@ class Scene:
         cam = camera_struct(camera or self.desc.camera)
         p = self.params(width, height, spp, **kw)
         check(lib().xt_render(self.h, C.byref(cam), C.byref(p), C.c_void_p(film_ptr), int(f64), 1))
+
+    def autotune(self, width, height, spp, camera=None, **kw):
+        """Picks this scene's traversal / shade grid sizes for graph renders;
+        returns (traversal blocks per CU, shade blocks per CU)."""
+        cam = camera_struct(camera or self.desc.camera)
+        p = self.params(width, height, spp, **kw)
+        out = (C.c_uint32 * 2)()
+        check(lib().xt_scene_autotune(self.h, C.byref(cam), C.byref(p), out))
+        return int(out[0]), int(out[1])
 
     def set_counting(self, on):
         check(lib().xt_scene_set_counting(self.h, int(on)))

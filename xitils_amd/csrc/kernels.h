@@ -20,6 +20,16 @@ constexpr int SHADE_BLOCK = XT_SHADE_BLOCK;          // threads per block of the
 constexpr int TRACE_BLOCK = XT_TRACE_BLOCK;          // lanes per block of the traversal kernels
 constexpr int TRACE_LDS_DEPTH = XT_TRACE_LDS_DEPTH;  // LDS stack entries per lane
 
+// Where the current batch lies in the render: pixel range, paths, first
+// sample.  Host-driven renders pass it in WaveParams; graph replays read it
+// from device memory (WaveParams::cursor), advanced by k_advance_cursor.
+struct BatchCursor {
+    uint32_t first_pixel, num_pixels;
+    uint32_t num_paths;
+    uint32_t sample0;      // global index of the batch's first sample
+    uint32_t s_rel;        // the same, counted from the render's first sample
+};
+
 // Per-batch constants of the wavefront pipeline.
 struct WaveParams {
     uint32_t width, height;
@@ -32,6 +42,17 @@ struct WaveParams {
     float rr_prob, eps, shadow_margin;
     float tan_x, tan_y;
     float cam[16];
+    const BatchCursor* cursor;          // non-null: the batch fields above are read from here
+};
+
+__device__ __forceinline__ BatchCursor batch_of(const WaveParams& wp) {
+    if (wp.cursor) return *wp.cursor;
+    return BatchCursor{wp.first_pixel, wp.num_pixels, wp.num_paths, wp.sample0, 0u};
+}
+
+// Batch schedule of one render: S samples of pix_chunk pixels per batch.
+struct BatchPlan {
+    uint32_t npix, pix_chunk, S, spp, spp_offset;
 };
 
 // Extension-ray queue, SoA of float4 (64 B per entry):
@@ -103,5 +124,8 @@ __global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, flo
                          uint32_t* ovf, unsigned long long* visit_totals);
 __global__ void k_accumulate(WaveParams wp, const float4* radiance, void* film, int film_f64);
 __global__ void k_zero_u32(uint32_t* p, uint32_t n);
+__global__ void k_cursor_init(BatchCursor* c, BatchPlan plan);
+__global__ void k_advance_cursor(BatchCursor* c, BatchPlan plan);
+__global__ void k_count_from_cursor(uint32_t* count, const BatchCursor* c);
 
 }  // namespace xt

@@ -149,12 +149,13 @@ XT_INST_TRACE(true, true)
 // jittered film position of RenderTargetTile::GenerateFilmPosition
 // (RenderTarget.h:68-80).
 __global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, float4* __restrict__ radiance) {
-    const uint32_t P = wp.num_paths;
+    const BatchCursor bc = batch_of(wp);
+    const uint32_t P = bc.num_paths;
     for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < P; id += gridDim.x * blockDim.x) {
-        uint32_t s_local = id / wp.num_pixels;
-        uint32_t pixel = wp.first_pixel + (id - s_local * wp.num_pixels);
+        uint32_t s_local = id / bc.num_pixels;
+        uint32_t pixel = bc.first_pixel + (id - s_local * bc.num_pixels);
         uint32_t x = pixel % wp.width, y = pixel / wp.width;
-        Rng rng{make_uint2(wp.seed, wp.stream), pixel, wp.sample0 + s_local};
+        Rng rng{make_uint2(wp.seed, wp.stream), pixel, bc.sample0 + s_local};
         uint4 b0 = rng.block(0, 0);
         float ux = Rng::to_unit(b0.x), uy = Rng::to_unit(b0.y);
         float fx = (float)x, fy = (float)y;
@@ -271,9 +272,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
             Rng rng{key, 0u, 0u};
             if (live) {
                 // path id -> (pixel, sample) as in k_generate
-                uint32_t s_local = id / wp.num_pixels;
-                rng.pixel = wp.first_pixel + (id - s_local * wp.num_pixels);
-                rng.sample = wp.sample0 + s_local;
+                const BatchCursor bc = batch_of(wp);
+                uint32_t s_local = id / bc.num_pixels;
+                rng.pixel = bc.first_pixel + (id - s_local * bc.num_pixels);
+                rng.sample = bc.sample0 + s_local;
             }
             if (live && standard) {
                 if (v > 1) {                                       // PathTracer.h:239-245
@@ -544,9 +546,10 @@ template __global__ void k_shadow<true>(DevScene, ShadowQueue, const uint32_t*, 
 // film[pixel] += radiance of each of the batch's samples, in sample order.
 __global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, const float4* __restrict__ radiance, void* film,
                                                     int film_f64) {
-    const uint32_t np = wp.num_pixels, S = wp.num_paths / wp.num_pixels;
+    const BatchCursor bc = batch_of(wp);
+    const uint32_t np = bc.num_pixels, S = bc.num_paths / bc.num_pixels;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-        size_t o = 3 * (size_t)(wp.first_pixel + p);
+        size_t o = 3 * (size_t)(bc.first_pixel + p);
         if (film_f64) {
             double* f = (double*)film;
             double r = f[o], g = f[o + 1], b = f[o + 2];
@@ -573,6 +576,47 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, const float4*
             f[o + 2] = b;
         }
     }
+}
+
+// ---------------------------------------------------------------- device-side batch schedule
+
+__device__ __forceinline__ BatchCursor cursor_at(const BatchPlan& pl, uint32_t px0, uint32_t s0, uint32_t base) {
+    BatchCursor c;
+    c.first_pixel = px0;
+    c.num_pixels = min(pl.pix_chunk, pl.npix - px0);
+    c.s_rel = s0;
+    c.sample0 = base + s0;
+    c.num_paths = c.num_pixels * min(pl.S, pl.spp - s0);
+    return c;
+}
+
+// First batch of a render whose first sample is plan.spp_offset.
+__global__ void k_cursor_init(BatchCursor* c, BatchPlan plan) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = cursor_at(plan, 0, 0, plan.spp_offset);
+}
+
+// Next batch: the next pixel chunk, or the first chunk of the next S samples.
+// Past the last batch the cursor holds zero paths.
+__global__ void k_advance_cursor(BatchCursor* c, BatchPlan plan) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    BatchCursor cur = *c;
+    const uint32_t base = cur.sample0 - cur.s_rel;   // the render's first sample (plan.spp_offset is not used here)
+    uint32_t px0 = cur.first_pixel + plan.pix_chunk;
+    uint32_t s0 = cur.s_rel;
+    if (px0 >= plan.npix) {
+        px0 = 0;
+        s0 += plan.S;
+    }
+    if (s0 >= plan.spp) {
+        cur.num_paths = 0;
+        *c = cur;
+        return;
+    }
+    *c = cursor_at(plan, px0, s0, base);
+}
+
+__global__ void k_count_from_cursor(uint32_t* count, const BatchCursor* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = c->num_paths;
 }
 
 }  // namespace xt

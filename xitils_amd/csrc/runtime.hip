@@ -83,6 +83,24 @@ struct GuideState {
     }
 };
 
+// One render's bounce loop captured as a HIP graph: generate, max_depth + 1
+// rounds of extend / shade / shadow, accumulate, advance the batch cursor.
+// Every kernel reads its queue count from device memory, so one replay
+// renders one batch with no host round trip; replays walk the batches.
+struct GraphKey {
+    xt_camera cam;
+    xt_render_params p;
+    void* film;
+    int32_t f64;
+    uint32_t trace_grid, shade_grid, capacity;
+};
+struct RenderGraph {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    uint32_t nbatches = 0;
+    BatchPlan plan{};
+};
+
 struct xt_scene {
     xt_device* dev = nullptr;
     hipStream_t stream = nullptr;
@@ -101,7 +119,11 @@ struct xt_scene {
     float* d_map_marg = nullptr;
     float* d_map_cond = nullptr;
     uint32_t ovf_depth = 0;           // entries per lane beyond the LDS part
-    uint32_t trace_grid_max = 0;      // blocks of the traversal kernels
+    uint32_t trace_grid_max = 0;      // blocks of the traversal kernels (sizes the spill columns)
+    uint32_t trace_grid = 0;          // blocks launched (<= trace_grid_max; xt_scene_autotune)
+    uint32_t shade_grid = 0;          // blocks of the shade kernel (xt_scene_autotune)
+    BatchCursor* d_cursor = nullptr;  // device-side batch schedule of graph replays
+    std::vector<struct RenderGraph> graphs;   // instantiated bounce loops (xt_render_batch)
     uint32_t bvh_nodes = 0, bvh_depth = 0;
     uint64_t bytes_nodes = 0, bytes_tris = 0;
     double build_ms = 0;
@@ -156,6 +178,8 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
     hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc);
     hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
+    for (RenderGraph& g : s->graphs) hipGraphExecDestroy(g.exec);
+    hipFree(s->d_cursor);
     s->rb.release();
     s->guide.release();
     if (s->stream) hipStreamDestroy(s->stream);
@@ -400,6 +424,8 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     const uint32_t stack_bound = 3 * bvh.depth + 1;
     s->ovf_depth = stack_bound > (uint32_t)TRACE_LDS_DEPTH ? stack_bound - TRACE_LDS_DEPTH : 0;
     s->trace_grid_max = (uint32_t)dev->num_cu * 8;
+    s->trace_grid = s->trace_grid_max;
+    s->shade_grid = (uint32_t)dev->num_cu * (2048 / SHADE_BLOCK);
     if (s->ovf_depth) XT_HIP_S(dalloc(&s->d_ovf, (size_t)s->ovf_depth * s->trace_grid_max * TRACE_BLOCK));
 #undef XT_HIP_S
     DevScene& ds = s->ds;
@@ -482,7 +508,7 @@ xt_status xt_scene_info(const xt_scene* s, uint64_t out[6]) {
 
 static uint32_t trace_blocks(const xt_scene* s, size_t n) {
     size_t b = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
-    return (uint32_t)std::max<size_t>(1, std::min<size_t>(b, s->trace_grid_max));
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>(b, s->trace_grid));
 }
 
 static xt_status query(xt_scene* s, const xt_ray* rays, uint32_t n, void* out, int on_device, bool any) {
@@ -530,9 +556,17 @@ xt_status xt_trace_any(xt_scene* s, const xt_ray* rays, uint32_t n, uint8_t* occ
     return query(s, rays, n, occluded, on_device, true);
 }
 
+static void drop_graphs(xt_scene* s) {
+    if (!s->graphs.empty()) hipStreamSynchronize(s->stream);   // none may still be replaying
+    for (RenderGraph& g : s->graphs) hipGraphExecDestroy(g.exec);
+    s->graphs.clear();
+}
+
 static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
     RenderBuffers& rb = s->rb;
     if (rb.capacity >= capacity) return XT_OK;
+    drop_graphs(s);   // graphs hold the old buffer addresses
+    hipStreamSynchronize(s->stream);
     rb.release();
     size_t c = capacity;
     for (int k = 0; k < 2; ++k) {
@@ -722,7 +756,10 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     xt_status st = ensure_buffers(s, cap);
     if (st != XT_OK) return st;
     RenderBuffers& rbg = s->rb;
-    if (guide && !rbg.lobes) XT_HIP(dalloc(&rbg.lobes, (size_t)G_LOBE_FIELDS * rbg.capacity));
+    if (guide && !rbg.lobes) {
+        drop_graphs(s);
+        XT_HIP(dalloc(&rbg.lobes, (size_t)G_LOBE_FIELDS * rbg.capacity));
+    }
     if (train) {
         if (!rbg.rec[0]) {
             XT_HIP(dalloc(&rbg.rec[0], rbg.capacity));
@@ -759,7 +796,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     hipEvent_t t_begin = tm.mark();
     const uint32_t gblocks_max = (uint32_t)s->dev->num_cu * 8;
     auto blocks256 = [&](uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, gblocks_max)); };
-    const uint32_t shade_max = (uint32_t)s->dev->num_cu * (2048 / SHADE_BLOCK);
+    const uint32_t shade_max = s->shade_grid;
     auto shade_blocks = [&](uint64_t n) {
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + SHADE_BLOCK - 1) / SHADE_BLOCK, shade_max));
     };
@@ -926,6 +963,278 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
         hipFree(d_film);
     }
     return result;
+}
+
+// ---------------------------------------------------------------- graph-replayed renders
+
+static xt_status check_render_args(xt_scene* s, const xt_camera* cam, const xt_render_params* p) {
+    if (!s || !cam || !p) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (p->width == 0 || p->height == 0) return fail(XT_ERR_INVALID_ARGUMENT, "empty image");
+    if (p->integrator > XT_INTEGRATOR_NAIVE) return fail(XT_ERR_INVALID_ARGUMENT, "unknown integrator");
+    if ((uint64_t)p->width * p->height >= (1ull << 31)) return fail(XT_ERR_TOO_LARGE, "image too large");
+    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(XT_ERR_INVALID_ARGUMENT, "rr_prob must be in (0, 1]");
+    if (p->flags & XT_FLAG_GUIDE_TRAIN) return fail(XT_ERR_INVALID_ARGUMENT, "graph renders do not train (use xt_render)");
+    if ((p->flags & XT_FLAG_GUIDE) && (p->integrator != XT_INTEGRATOR_STANDARD || !s->guide.ready))
+        return fail(XT_ERR_INVALID_ARGUMENT, "guiding needs the standard integrator and xt_guide_init");
+    if (p->max_depth <= 0 || p->max_depth > 1024)
+        return fail(XT_ERR_INVALID_ARGUMENT, "graph renders need a bounce cap (max_depth in 1..1024)");
+    return XT_OK;
+}
+
+// The bounce loop of one batch, captured from s->stream into a graph.
+static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* d_film, int f64,
+                               RenderGraph* out) {
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix * p->spp);
+    xt_status st = ensure_buffers(s, cap);
+    if (st != XT_OK) return st;
+    RenderBuffers& rb = s->rb;
+    const bool guide = (p->flags & XT_FLAG_GUIDE) != 0;
+    if (guide && !rb.lobes) {
+        drop_graphs(s);
+        XT_HIP(dalloc(&rb.lobes, (size_t)G_LOBE_FIELDS * rb.capacity));
+    }
+    if (!s->d_cursor) XT_HIP(dalloc(&s->d_cursor, 1));
+    for (int k = 0; k < 2; ++k) rb.q[k].rec = nullptr;
+    BatchPlan plan;
+    plan.npix = (uint32_t)npix;
+    plan.S = npix <= cap ? (uint32_t)std::min<uint64_t>(cap / npix, p->spp) : 1u;
+    plan.pix_chunk = (uint32_t)std::min<uint64_t>(npix, cap);
+    plan.spp = p->spp;
+    plan.spp_offset = p->spp_offset;
+    const uint32_t maxpaths = plan.pix_chunk * plan.S;
+    WaveParams wp{};
+    wp.width = p->width;
+    wp.height = p->height;
+    wp.seed = p->seed;
+    wp.stream = p->stream;
+    wp.integrator = p->integrator;
+    wp.flags = p->flags;
+    wp.max_depth = p->max_depth;
+    wp.rr_start = p->rr_start;
+    wp.rr_prob = p->rr_prob;
+    wp.eps = p->ray_eps;
+    wp.shadow_margin = p->shadow_margin;
+    wp.tan_x = cam->tan_x;
+    wp.tan_y = cam->tan_y;
+    std::memcpy(wp.cam, cam->cam_to_world, sizeof(wp.cam));
+    wp.cursor = s->d_cursor;
+    const uint32_t gblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((maxpaths + 255) / 256, (uint64_t)s->dev->num_cu * 8));
+    const uint32_t ablocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((plan.pix_chunk + 255) / 256, (uint64_t)s->dev->num_cu * 8));
+    const uint32_t sblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((maxpaths + SHADE_BLOCK - 1) / SHADE_BLOCK, s->shade_grid));
+    const uint32_t tblocks = trace_blocks(s, maxpaths);
+    GuideState& g = s->guide;
+    float4* l4 = guide ? (float4*)rb.lobes : nullptr;
+    float2* l2 = guide ? (float2*)(rb.lobes + (size_t)4 * G_LOBES * rb.capacity) : nullptr;
+    const uint32_t ib = std::max(1u, std::min((maxpaths + GI_BLOCK - 1) / GI_BLOCK, (uint32_t)s->dev->num_cu * 4));
+    XT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, s->stream, wp, rb.q[0], rb.radiance);
+    hipLaunchKernelGGL(k_count_from_cursor, dim3(1), dim3(1), 0, s->stream, rb.counts + 0, s->d_cursor);
+    int cur = 0;
+    for (int it = 0; it <= p->max_depth; ++it) {
+        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
+                           rb.q[cur].o, rb.q[cur].d, 1, rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + (1 - cur), 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
+        GuideRender gr{};
+        if (guide) {
+            hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, rb.q[cur],
+                               rb.counts + cur, rb.hits, l4, l2, rb.capacity);
+            gr.lobes4 = l4;
+            gr.lobes2 = l2;
+            gr.stride = rb.capacity;
+            gr.prob = g.cfg.guide_prob;
+            hipLaunchKernelGGL(k_shade<true>, dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
+                               rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
+                               rb.radiance, rb.pending, gr);
+        } else {
+            hipLaunchKernelGGL(k_shade<false>, dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
+                               rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
+                               rb.radiance, rb.pending, gr);
+        }
+        hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.sq, rb.counts + 2,
+                           rb.radiance, rb.pending, s->d_ovf, nullptr);
+        cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(ablocks), dim3(256), 0, s->stream, wp, rb.radiance, d_film, f64);
+    hipLaunchKernelGGL(k_advance_cursor, dim3(1), dim3(1), 0, s->stream, s->d_cursor, plan);
+    hipError_t launch_err = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (launch_err != hipSuccess || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(XT_ERR_DEVICE, std::string("graph capture failed: ") +
+                                       hipGetErrorString(launch_err != hipSuccess ? launch_err : e));
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+    out->exec = exec;
+    out->plan = plan;
+    out->nbatches = ((p->spp + plan.S - 1) / plan.S) * (uint32_t)((npix + plan.pix_chunk - 1) / plan.pix_chunk);
+    return XT_OK;
+}
+
+static xt_status find_graph(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* d_film, int f64,
+                            RenderGraph** out) {
+    GraphKey key;
+    std::memset(&key, 0, sizeof key);
+    key.cam = *cam;
+    key.p = *p;
+    key.p.spp_offset = 0;   // only the cursor's start depends on it
+    key.film = d_film;
+    key.f64 = f64;
+    key.trace_grid = s->trace_grid;
+    key.shade_grid = s->shade_grid;
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    key.capacity = (uint32_t)std::min<uint64_t>(path_capacity(), npix * p->spp);
+    for (RenderGraph& g : s->graphs)
+        if (std::memcmp(&g.key, &key, sizeof key) == 0 && s->rb.capacity >= key.capacity) {
+            *out = &g;
+            return XT_OK;
+        }
+    if (s->graphs.size() >= 64) drop_graphs(s);
+    RenderGraph g;
+    g.key = key;
+    xt_status st = capture_graph(s, cam, p, d_film, f64, &g);
+    if (st != XT_OK) return st;
+    s->graphs.push_back(g);
+    *out = &s->graphs.back();
+    return XT_OK;
+}
+
+// Enqueues one render on s->stream: reset the cursor, replay the graph once
+// per batch.
+static xt_status enqueue_graph_render(xt_scene* s, const RenderGraph& g, uint32_t spp_offset) {
+    BatchPlan plan = g.plan;
+    plan.spp_offset = spp_offset;
+    hipLaunchKernelGGL(k_cursor_init, dim3(1), dim3(1), 0, s->stream, s->d_cursor, plan);
+    XT_HIP(hipGetLastError());
+    for (uint32_t b = 0; b < g.nbatches; ++b) XT_HIP(hipGraphLaunch(g.exec, s->stream));
+    return XT_OK;
+}
+
+xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
+    if (n && !items) return fail(XT_ERR_INVALID_ARGUMENT, "items is NULL");
+    for (uint32_t i = 0; i < n; ++i) {
+        xt_status st = check_render_args(items[i].scene, &items[i].camera, &items[i].params);
+        if (st != XT_OK) return fail(st, "item " + std::to_string(i) + ": " + xt_last_error());
+        if (!items[i].film) return fail(XT_ERR_INVALID_ARGUMENT, "item " + std::to_string(i) + ": film is NULL");
+    }
+    struct Pending {
+        void* d_film = nullptr;
+        bool owned = false;
+        size_t bytes = 0;
+        hipEvent_t a = nullptr, b = nullptr;
+    };
+    std::vector<Pending> pend(n);
+    xt_status result = XT_OK;
+    auto cleanup = [&]() {
+        for (Pending& q : pend) {
+            if (q.owned) hipFree(q.d_film);
+            if (q.a) hipEventDestroy(q.a);
+            if (q.b) hipEventDestroy(q.b);
+        }
+    };
+    for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
+        xt_batch_item& it = items[i];
+        xt_scene* s = it.scene;
+        it.ms = 0.0f;
+        if (hipSetDevice(s->dev->ordinal) != hipSuccess) {
+            result = fail(XT_ERR_DEVICE, "hipSetDevice failed");
+            break;
+        }
+        Pending& q = pend[i];
+        q.bytes = (size_t)it.params.width * it.params.height * 3 * (it.film_is_f64 ? 8 : 4);
+        q.d_film = it.film;
+        if (!it.film_on_device) {
+            if (hipMalloc(&q.d_film, q.bytes) != hipSuccess) {
+                result = fail(XT_ERR_OUT_OF_MEMORY, "film allocation failed");
+                break;
+            }
+            q.owned = true;
+            hipMemcpyAsync(q.d_film, it.film, q.bytes, hipMemcpyHostToDevice, s->stream);
+        }
+        if (it.params.spp == 0) continue;
+        RenderGraph* g = nullptr;
+        result = find_graph(s, &it.camera, &it.params, q.d_film, it.film_is_f64, &g);
+        if (result != XT_OK) break;
+        hipEventCreate(&q.a);
+        hipEventCreate(&q.b);
+        hipEventRecord(q.a, s->stream);
+        result = enqueue_graph_render(s, *g, it.params.spp_offset);
+        hipEventRecord(q.b, s->stream);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        xt_batch_item& it = items[i];
+        xt_scene* s = it.scene;
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess && result == XT_OK) result = fail(XT_ERR_DEVICE, std::string("batch render failed: ") + hipGetErrorString(e));
+        Pending& q = pend[i];
+        if (result == XT_OK && q.a) hipEventElapsedTime(&it.ms, q.a, q.b);
+        if (result == XT_OK && q.owned) {
+            e = hipMemcpy(it.film, q.d_film, q.bytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) result = fail(XT_ERR_DEVICE, "film copy failed");
+        }
+    }
+    cleanup();
+    return result;
+}
+
+// Times the graph-replayed render of one batch for a few launch shapes and
+// keeps the fastest (traversal blocks per CU x shade blocks per CU).
+xt_status xt_scene_autotune(xt_scene* s, const xt_camera* cam, const xt_render_params* p, uint32_t out[2]) {
+    xt_render_params q = *p;
+    xt_status st = check_render_args(s, cam, &q);
+    if (st != XT_OK) return st;
+    XT_HIP(hipSetDevice(s->dev->ordinal));
+    const uint64_t npix = (uint64_t)q.width * q.height;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix * std::max(1u, q.spp));
+    q.spp = std::max(1u, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cap / npix, 1), std::max(1u, p->spp)));
+    void* d_film = nullptr;
+    XT_HIP(hipMalloc(&d_film, npix * 3 * sizeof(float)));
+    const uint32_t ncu = (uint32_t)s->dev->num_cu;
+    const uint32_t tcand[3] = {2, 4, 8}, scand[3] = {2, 4, 8};
+    float best = INFINITY;
+    uint32_t bt = s->trace_grid, bs = s->shade_grid;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    for (uint32_t ti : tcand)
+        for (uint32_t si : scand) {
+            s->trace_grid = std::min(ti * ncu, s->trace_grid_max);
+            s->shade_grid = si * ncu;
+            RenderGraph* g = nullptr;
+            st = find_graph(s, cam, &q, d_film, 0, &g);
+            if (st != XT_OK) break;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2 && st == XT_OK; ++rep) {   // the first run warms the caches
+                hipEventRecord(a, s->stream);
+                st = enqueue_graph_render(s, *g, 0);
+                hipEventRecord(b, s->stream);
+                if (hipStreamSynchronize(s->stream) != hipSuccess) st = fail(XT_ERR_DEVICE, "autotune render failed");
+                hipEventElapsedTime(&ms, a, b);
+            }
+            if (st != XT_OK) break;
+            if (ms < best) {
+                best = ms;
+                bt = s->trace_grid;
+                bs = s->shade_grid;
+            }
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    hipFree(d_film);
+    s->trace_grid = bt;
+    s->shade_grid = bs;
+    drop_graphs(s);
+    if (st != XT_OK) return st;
+    if (out) {
+        out[0] = bt / ncu;
+        out[1] = bs / ncu;
+    }
+    return XT_OK;
 }
 
 xt_status xt_scene_set_counting(xt_scene* s, int on) {
