@@ -47,8 +47,13 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["heightfield", "guiding"], default="heightfield",
-                    help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room")
+    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch"], default="heightfield",
+                    help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room; "
+                         "batch: 64 small scenes rendered one by one and as one graph-replayed batch")
+    ap.add_argument("--scenes", type=int, default=64, help="batch: number of scenes")
+    ap.add_argument("--autotune", action="store_true", help="batch: autotune each scene's launch shapes first")
+    ap.add_argument("--batch-side", type=int, default=0, help="batch: override every scene's image side")
+    ap.add_argument("--batch-spp", type=int, default=0, help="batch: override every scene's spp")
     ap.add_argument("--budget", type=float, default=2.0, help="guiding: seconds of rendering per method")
     ap.add_argument("--ref-spp", type=int, default=16384, help="guiding: samples per pixel of the FP64 reference")
     ap.add_argument("--room-tris", type=int, default=500_000)
@@ -150,6 +155,91 @@ def guiding_main(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_main(args):
+    """Aggregate Mpaths/s and per-scene latency over the 64-scene batch
+    (master seed 3: unit rooms with tessellated spheres and boxes, 1e4..2e6
+    triangles, 128^2..1024^2 pixels, 4..256 spp, depth 4..32, one area light
+    each), rendered (a) one xt_render call after another and (b) as one
+    xt_render_batch call (HIP-graph bounce loops, concurrent streams)."""
+    import numpy as np
+    import torch
+    from xitils_amd import api, scenes
+    dev = api.Device(0)
+    t0 = time.perf_counter()
+    jobs = []
+    tris = 0
+    for k in range(args.scenes):
+        desc, w, h, spp, depth = scenes.batch_scene(k)
+        if args.batch_side:
+            w = h = args.batch_side
+        if args.batch_spp:
+            spp = args.batch_spp
+        sc = api.Scene(dev, desc)
+        tris += sc.info()["num_prims"]
+        film = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda:0")
+        jobs.append((sc, w, h, spp, depth, film))
+    build_s = time.perf_counter() - t0
+    paths = sum(w * h * spp for _, w, h, spp, _, _ in jobs)
+    log(f"[bench] {len(jobs)} scenes, {tris} triangles, {paths / 1e6:.1f} M paths, built in {build_s:.1f} s")
+    tuned = []
+    if args.autotune:
+        for sc, w, h, spp, depth, _ in jobs:
+            tuned.append(sc.autotune(w, h, spp, seed=1, max_depth=depth))
+        log(f"[bench] autotuned: {tuned}")
+
+    def sequential():
+        lat = []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for sc, w, h, spp, depth, film in jobs:
+            a = time.perf_counter()
+            sc.render_device(film.data_ptr(), w, h, spp, seed=1, max_depth=depth)
+            lat.append((time.perf_counter() - a) * 1e3)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, lat
+
+    def batched():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, ms = api.render_batch([(sc, w, h, spp, (film.data_ptr(), False), dict(seed=1, max_depth=depth))
+                                  for sc, w, h, spp, depth, film in jobs])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, ms
+
+    sequential()                       # warm-up (first-touch allocations)
+    seq_s, seq_lat = sequential()
+    log(f"[bench] sequential: {seq_s:.3f} s")
+    batched()                          # warm-up: captures the graphs
+    bat_s, bat_lat = batched()
+    log(f"[bench] batched: {bat_s:.3f} s")
+    pct = lambda v, q: float(np.percentile(np.asarray(v), q))  # noqa: E731
+    out = {
+        "metric": "aggregate Mpaths/s over a 64-scene batch",
+        "value": round(paths / bat_s / 1e6, 2),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(bat_s * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch{len(jobs)}_seed3" + (f"_side{args.batch_side}" if args.batch_side else "") +
+                               (f"_spp{args.batch_spp}" if args.batch_spp else ""), "scenes": len(jobs), "triangles": tris,
+                   "paths": paths, "autotune": bool(args.autotune), "parallelism": "1 GPU"},
+        "latency_ms": {"p50": round(pct(bat_lat, 50), 3), "p99": round(pct(bat_lat, 99), 3),
+                       "note": "GPU time of each item on its scene's stream (HIP events), items running concurrently"},
+        "sequential": {"mpaths_per_s": round(paths / seq_s / 1e6, 2), "seconds": round(seq_s, 3),
+                       "latency_ms": {"p50": round(pct(seq_lat, 50), 3), "p99": round(pct(seq_lat, 99), 3)}},
+        "build_s": round(build_s, 1),
+    }
+    if tuned:
+        out["tuned_blocks_per_cu"] = tuned
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -158,6 +248,8 @@ def main():
         sys.exit(2)
     if args.workload == "guiding":
         return guiding_main(args)
+    if args.workload == "batch":
+        return batch_main(args)
     import numpy as np
     import torch
     from xitils_amd import api, scenes

@@ -981,11 +981,17 @@ static xt_status check_render_args(xt_scene* s, const xt_camera* cam, const xt_r
     return XT_OK;
 }
 
+// Paths in flight per batch of a graph render: at most 4 Mi (0.9 GB of path
+// state), so many scenes of a batch can hold their buffers at once.
+static uint32_t graph_capacity(uint64_t npix, uint32_t spp) {
+    return (uint32_t)std::min<uint64_t>(std::min<uint32_t>(path_capacity(), 1u << 22), npix * spp);
+}
+
 // The bounce loop of one batch, captured from s->stream into a graph.
 static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* d_film, int f64,
                                RenderGraph* out) {
     const uint64_t npix = (uint64_t)p->width * p->height;
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix * p->spp);
+    const uint32_t cap = graph_capacity(npix, p->spp);
     xt_status st = ensure_buffers(s, cap);
     if (st != XT_OK) return st;
     RenderBuffers& rb = s->rb;
@@ -1088,7 +1094,7 @@ static xt_status find_graph(xt_scene* s, const xt_camera* cam, const xt_render_p
     key.trace_grid = s->trace_grid;
     key.shade_grid = s->shade_grid;
     const uint64_t npix = (uint64_t)p->width * p->height;
-    key.capacity = (uint32_t)std::min<uint64_t>(path_capacity(), npix * p->spp);
+    key.capacity = graph_capacity(npix, p->spp);
     for (RenderGraph& g : s->graphs)
         if (std::memcmp(&g.key, &key, sizeof key) == 0 && s->rb.capacity >= key.capacity) {
             *out = &g;
@@ -1190,7 +1196,7 @@ xt_status xt_scene_autotune(xt_scene* s, const xt_camera* cam, const xt_render_p
     if (st != XT_OK) return st;
     XT_HIP(hipSetDevice(s->dev->ordinal));
     const uint64_t npix = (uint64_t)q.width * q.height;
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix * std::max(1u, q.spp));
+    const uint32_t cap = graph_capacity(npix, std::max(1u, q.spp));
     q.spp = std::max(1u, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cap / npix, 1), std::max(1u, p->spp)));
     void* d_film = nullptr;
     XT_HIP(hipMalloc(&d_film, npix * 3 * sizeof(float)));

@@ -481,6 +481,53 @@ def sky_scene(kind=ENV_MAP_LATLONG, width=64, height=48, map_size=None, albedo=0
     return s
 
 
+# --------------------------------------------------------------------------- batch of small scenes
+
+def batch_scene(index, master_seed=3):
+    """Scene `index` of the 64-scene batch workload: a unit room (Lambert
+    walls, one ceiling area light) holding tessellated spheres and boxes.
+    Drawn from the master seed: triangle count log-uniform on [1e4, 2e6],
+    image side uniform on [128, 1024] (square), spp log-uniform on [4, 256],
+    depth uniform on [4, 32].  Returns (desc, width, height, spp, depth)."""
+    rng = np.random.default_rng([master_seed, index])
+    tris = int(round(math.exp(rng.uniform(math.log(1e4), math.log(2e6)))))
+    side = int(rng.integers(128, 1025))
+    spp = int(round(math.exp(rng.uniform(math.log(4), math.log(256)))))
+    depth = int(rng.integers(4, 33))
+    s = SceneDesc(name=f"batch{index:02d}")
+    wall = s.add_material(diffuse(0.7))
+    red = s.add_material(diffuse(0.6, 0.1, 0.08))
+    light = s.add_material(emission(12.0, 11.0, 9.0))
+    mats = [s.add_material(diffuse(*rng.uniform(0.2, 0.8, 3))) for _ in range(3)]
+    mats += [s.add_material(metal(float(rng.uniform(0.2, 0.6)), tuple(rng.uniform(0.5, 0.95, 3)))) for _ in range(2)]
+    # room: floor, ceiling, back, left (red), right; open front
+    for k, (a, b, c, d) in enumerate([((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)),
+                                      ((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),
+                                      ((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)),
+                                      ((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)),
+                                      ((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0))]):
+        v, i = quad(a, b, c, d)
+        s.objects.append(ObjectDesc(v, i, material=red if k == 3 else wall))
+    lv, li = quad((0.4, 0.999, 0.4), (0.6, 0.999, 0.4), (0.6, 0.999, 0.6), (0.4, 0.999, 0.6))
+    s.objects.append(ObjectDesc(lv, li, material=light))
+    nobj = int(rng.integers(3, 13))
+    per = max(12, (tris - 12) // nobj)
+    for k in range(nobj):
+        m = mats[int(rng.integers(0, len(mats)))]
+        c = rng.uniform(0.15, 0.85, 3)
+        c[1] = rng.uniform(0.1, 0.6)
+        if k % 3 == 2:
+            bv, bi = box_mesh(tuple(rng.uniform(0.08, 0.25, 3)), tuple(c), float(rng.uniform(0, 90)))
+            s.objects.append(ObjectDesc(bv, bi, material=m))
+        else:
+            nv = max(4, int(math.sqrt(per / 4)))
+            p, i, n = uv_sphere(tuple(c), float(rng.uniform(0.05, 0.15)), nu=2 * nv, nv=nv)
+            s.objects.append(ObjectDesc(p, i, material=m, normals=n))
+    cam = look_at((0.5, 0.5, -1.1), (0.5, 0.45, 0.5))
+    s.camera = Camera(cam, math.radians(45.0), 1.0)
+    return s, side, side, spp, depth
+
+
 # --------------------------------------------------------------------------- occluded room
 
 def _box_faces(lo, hi, inward=False):
