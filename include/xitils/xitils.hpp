@@ -121,9 +121,20 @@ inline Transform transformTRS(const Vector3f& t, const Vector3f& r, const Vector
     return translate(t.x, t.y, t.z) * rotateYXZ(r.x, r.y, r.z) * scale(s.x, s.y, s.z);
 }
 
+// ------------------------------------------------------------------ textures (Texture.h)
+// An RGB float texture read with Texture::rgb's bilinear filter and clamped
+// addressing (Texture.h:83-103).  The reference decodes files with stb_image
+// (Texture.h:17-23); this API takes the decoded texels, row 0 first.
+struct Texture {
+    std::vector<float> rgb;
+    int width = 0, height = 0;
+    Texture(const float* texels, int w, int h) : rgb(texels, texels + 3 * (size_t)w * h), width(w), height(h) {}
+};
+
 // ------------------------------------------------------------------ materials (Material.h)
 struct Material {
     bool emissive = false;
+    std::shared_ptr<Texture> normalmap;   // Material::normalmap, applied as Object.h:59-68
     virtual ~Material() = default;
     virtual xt_material record() const = 0;
 };
@@ -171,27 +182,49 @@ struct GGXOverLambert : Material {
     }
 };
 
+// MicrofacetNormalMapping (Sandbox/MicrofacetBasedNormalMapping/Main.cpp:
+// 86-340, SameMaterialExplicit) over a Diffuse, Metal or GGXOverLambert base.
+struct MicrofacetNormalMapping : Material {
+    std::shared_ptr<Material> material_wp;
+    explicit MicrofacetNormalMapping(std::shared_ptr<Material> base) : material_wp(std::move(base)) {}
+    xt_material record() const override {
+        xt_material b = material_wp->record();
+        xt_material m{};
+        m.type = XT_MAT_MICROFACET_NMAP;
+        m.p[0] = (float)b.type;
+        for (int k = 0; k < 6; ++k) m.p[k + 1] = b.p[k];
+        return m;
+    }
+};
+
 // ------------------------------------------------------------------ geometry
 struct Shape {
     virtual ~Shape() = default;
 };
 
-// TriangleMesh::setGeometry raw-array form (TriangleMesh.h:151-161);
-// texture coordinates, tangents and bitangents are accepted and unused.
+// TriangleMesh::setGeometry raw-array form (TriangleMesh.h:151-161).
 struct TriangleMesh : Shape {
-    std::vector<float> positions, normals;
+    std::vector<float> positions, normals, texCoords, tangents, bitangents;
     std::vector<int32_t> indices;
-    void setGeometry(int vertexNum, const Vector3f* pos, const Vector2f* /*texCoords*/, const Vector3f* nrm,
-                     const Vector3f* /*tangents*/, const Vector3f* /*bitangents*/, int indexNum, const int* indexData) {
-        positions.resize(3 * (size_t)vertexNum);
-        for (int i = 0; i < vertexNum; ++i) {
-            positions[3 * i] = pos[i].x; positions[3 * i + 1] = pos[i].y; positions[3 * i + 2] = pos[i].z;
-        }
-        normals.clear();
-        if (nrm) {
-            normals.resize(3 * (size_t)vertexNum);
+    void setGeometry(int vertexNum, const Vector3f* pos, const Vector2f* uv, const Vector3f* nrm,
+                     const Vector3f* tan, const Vector3f* bitan, int indexNum, const int* indexData) {
+        auto put3 = [vertexNum](std::vector<float>& dst, const Vector3f* src) {
+            dst.clear();
+            if (!src) return;
+            dst.resize(3 * (size_t)vertexNum);
             for (int i = 0; i < vertexNum; ++i) {
-                normals[3 * i] = nrm[i].x; normals[3 * i + 1] = nrm[i].y; normals[3 * i + 2] = nrm[i].z;
+                dst[3 * i] = src[i].x; dst[3 * i + 1] = src[i].y; dst[3 * i + 2] = src[i].z;
+            }
+        };
+        put3(positions, pos);
+        put3(normals, nrm);
+        put3(tangents, tan);
+        put3(bitangents, bitan);
+        texCoords.clear();
+        if (uv) {
+            texCoords.resize(2 * (size_t)vertexNum);
+            for (int i = 0; i < vertexNum; ++i) {
+                texCoords[2 * i] = uv[i].x; texCoords[2 * i + 1] = uv[i].y;
             }
         }
         indices.assign(indexData, indexData + indexNum);
@@ -344,6 +377,9 @@ class Scene {                                               // Scene.h:11-61
     void buildAccelerationStructure() {
         std::vector<xt_object_desc> od(objects_.size());
         std::vector<xt_material> mats;
+        std::vector<uint32_t> nmaps;              // per material: 1 + texture index, 0 = none
+        std::vector<xt_texture> texs;
+        std::vector<const Texture*> texOwners;
         std::vector<std::vector<float>> xf(objects_.size()), xfi(objects_.size());
         for (size_t k = 0; k < objects_.size(); ++k) {
             const Object& o = *objects_[k];
@@ -354,6 +390,11 @@ class Scene {                                               // Scene.h:11-61
             d.indices = o.shape->indices.data();
             d.num_vertices = (uint32_t)(o.shape->positions.size() / 3);
             d.num_triangles = (uint32_t)o.shape->triangleNum();
+            d.texcoords = o.shape->texCoords.empty() ? nullptr : o.shape->texCoords.data();
+            if (!o.shape->tangents.empty() && !o.shape->bitangents.empty()) {
+                d.tangents = o.shape->tangents.data();
+                d.bitangents = o.shape->bitangents.data();
+            }
             if (!o.objectToWorld.isIdentity()) {
                 xf[k].assign(&o.objectToWorld.mat[0][0], &o.objectToWorld.mat[0][0] + 16);
                 xfi[k].assign(&o.objectToWorld.matInv[0][0], &o.objectToWorld.matInv[0][0] + 16);
@@ -362,13 +403,32 @@ class Scene {                                               // Scene.h:11-61
             }
             d.material = (uint32_t)mats.size();
             mats.push_back(o.material->record());
+            uint32_t nm = 0;
+            if (const Texture* t = o.material->normalmap.get()) {
+                size_t ti = 0;
+                while (ti < texOwners.size() && texOwners[ti] != t) ++ti;
+                if (ti == texOwners.size()) {
+                    texOwners.push_back(t);
+                    texs.push_back(xt_texture{(uint32_t)t->width, (uint32_t)t->height, t->rgb.data()});
+                }
+                nm = (uint32_t)ti + 1;
+            }
+            nmaps.push_back(nm);
         }
-        if (mats.empty()) mats.push_back(make_record(XT_MAT_DIFFUSE, {0, 0, 0}));
+        if (mats.empty()) {
+            mats.push_back(make_record(XT_MAT_DIFFUSE, {0, 0, 0}));
+            nmaps.push_back(0);
+        }
         xt_scene_desc sd{};
         sd.objects = od.data();
         sd.num_objects = (uint32_t)od.size();
         sd.materials = mats.data();
         sd.num_materials = (uint32_t)mats.size();
+        if (!texs.empty()) {
+            sd.textures = texs.data();
+            sd.num_textures = (uint32_t)texs.size();
+            sd.material_normal_maps = nmaps.data();
+        }
         if (skySphere) sd.env = skySphere->env();
         xt_scene* s = nullptr;
         check(xt_scene_create(Device::get().handle(), &sd, &s));

@@ -47,7 +47,12 @@ typedef struct xt_scene xt_scene;
  *   XT_MAT_EMISSION      power r, g, b                          Emission           Material.h:386-411
  *   XT_MAT_GGX_LAMBERT   albedo r, g, b, alpha, f0, p_spec      GGX specular over Lambert, one-sample
  *                                                               lobe mixture as GlossyWithHighLight
- *                                                               (Material.h:134-175); no reference twin */
+ *                                                               (Material.h:134-175); no reference twin
+ *   XT_MAT_MICROFACET_NMAP  base type, base p[0..5]             MicrofacetNormalMapping (Schuessler 2017,
+ *                                                               Sandbox/MicrofacetBasedNormalMapping/
+ *                                                               Main.cpp:86-340, SameMaterialExplicit)
+ *                                                               over a DIFFUSE, METAL or GGX_LAMBERT base
+ *                                                               whose parameters follow the type */
 enum {
     XT_MAT_DIFFUSE = 0,
     XT_MAT_SPEC_REFL = 1,
@@ -56,7 +61,8 @@ enum {
     XT_MAT_METAL = 4,
     XT_MAT_EMISSION = 5,
     XT_MAT_GGX_LAMBERT = 6,
-    XT_MAT_COUNT_ = 7
+    XT_MAT_MICROFACET_NMAP = 7,
+    XT_MAT_COUNT_ = 8
 };
 
 typedef struct xt_material {   /* 32 bytes */
@@ -78,7 +84,20 @@ typedef struct xt_object_desc {
     const float* world_to_object;  /* 16 floats row-major (Transform::matInv), or NULL */
     uint32_t material;             /* index into xt_scene_desc.materials */
     const uint32_t* tri_materials; /* num_triangles material indices, or NULL */
+    /* TriangleMesh texCoords / tangents / bitangents (TriangleMesh.h:151-161),
+     * read where a material has a normal map: */
+    const float* texcoords;        /* num_vertices * 2, or NULL (uv = 0) */
+    const float* tangents;         /* num_vertices * 3, or NULL (no tangent frame) */
+    const float* bitangents;       /* num_vertices * 3; required with tangents */
 } xt_object_desc;
+
+/* An RGB float texture (Texture.h): rgb(uv) is the bilinear lookup with
+ * clamped addressing (Texture.h:83-103, 161-171); texel (x, y) at
+ * rgb[3 * (y * width + x)], row 0 first.  Copied at scene creation. */
+typedef struct xt_texture {
+    uint32_t width, height;
+    const float* rgb;
+} xt_texture;
 
 enum {
     XT_ENV_MAP_NONE = 0,
@@ -110,6 +129,13 @@ typedef struct xt_scene_desc {
     const xt_material* materials;
     uint32_t num_materials;
     xt_env env;
+    const xt_texture* textures;
+    uint32_t num_textures;
+    /* Material::normalmap per material: 1 + index into textures, 0 = none;
+     * NULL = no normal maps.  Applied at each hit as Object.h:59-68:
+     * n = rgb(texCoord) * 2 - 1, shading.n = normalize(n.b * shading.n +
+     * n.r * tangent + n.g * bitangent). */
+    const uint32_t* material_normal_maps;
 } xt_scene_desc;
 
 /* PinholeCamera (Camera.h:17-55): cam_to_world is Transform::mat (row-major),

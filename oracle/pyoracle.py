@@ -57,6 +57,15 @@ def lib():
         L.orc_uniform.argtypes = [C.c_uint32] * 5 + [C.c_int]
         L.orc_uniform.restype = C.c_float
         L.orc_bsdf_sample.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
+        L.orc_bsdf_sample_walk.argtypes = [C.c_uint32, f32p, f32p, u32p, f32p]
+        L.orc_bsdf_eval_walk.argtypes = [C.c_uint32, f32p, f32p, f32p, u32p, f32p]
+        L.orc_nmap_terms.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
+        L.orc_bsdf_moments.argtypes = [C.c_uint32, f32p, f32p, C.c_uint32, C.c_int, C.c_int, C.c_int, f64p]
+        L.orc_shading_normals.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+        L.orc_object_set_attributes.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_scene_add_texture.argtypes = [C.c_void_p, C.c_int, C.c_int, f32p]
+        L.orc_scene_add_texture.restype = C.c_int
+        L.orc_material_set_normal_map.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_bsdf_eval.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
         L.orc_ggx_D.argtypes = [C.c_float, f32p, f32p]
         L.orc_ggx_D.restype = C.c_float
@@ -98,8 +107,21 @@ class OracleScene:
             xfi = None if o.xf_inv is None else np.ascontiguousarray(o.xf_inv, np.float32)
             tm = None if o.tri_materials is None else np.ascontiguousarray(o.tri_materials, np.uint32)
             self._keep += [pos, idx, nrm, xf, xfi, tm]
-            L.orc_scene_add_object(self.h, pos, _ptr(nrm), pos.shape[0], idx, idx.shape[0], _ptr(xf), _ptr(xfi),
-                                   o.material, _ptr(tm))
+            oi = L.orc_scene_add_object(self.h, pos, _ptr(nrm), pos.shape[0], idx, idx.shape[0], _ptr(xf),
+                                        _ptr(xfi), o.material, _ptr(tm))
+            attrs = [None if getattr(o, k, None) is None else np.ascontiguousarray(getattr(o, k), np.float32)
+                     for k in ("texcoords", "tangents", "bitangents")]
+            if any(a is not None for a in attrs):
+                self._keep += attrs
+                L.orc_object_set_attributes(self.h, oi, *[_ptr(a) for a in attrs])
+        for t in getattr(desc, "textures", None) or []:
+            t = np.ascontiguousarray(t, np.float32)
+            L.orc_scene_add_texture(self.h, t.shape[1], t.shape[0], t.reshape(-1))
+        from xitils_amd.scenes import material_normal_maps
+        nmaps = material_normal_maps(desc.materials)
+        if nmaps is not None:
+            for k in np.nonzero(nmaps)[0]:
+                L.orc_material_set_normal_map(self.h, int(k), int(nmaps[k]) - 1)
         e = desc.env
         omc, sin2 = e.sun_terms()
         L.orc_scene_set_env(self.h, int(e.has_sky), np.array(e.sky, np.float32), np.array(e.sun_dir, np.float32),
@@ -124,6 +146,12 @@ class OracleScene:
         hits = np.zeros((rays.shape[0], 4), np.float32)
         lib().orc_trace_closest(self.h, rays, rays.shape[0], hits, mode, nthreads)
         return hits
+
+    def shading_normals(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 3), np.float32)
+        lib().orc_shading_normals(self.h, rays, rays.shape[0], out)
+        return out
 
     def trace_any(self, rays, mode=0, nthreads=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -194,6 +222,41 @@ def bsdf_sample(mtype, p7, isect9, u3):
     out = np.zeros(7, np.float32)
     lib().orc_bsdf_sample(mtype, np.asarray(p7, np.float32), np.asarray(isect9, np.float32),
                           np.asarray(u3, np.float32), out)
+    return out
+
+
+def bsdf_sample_walk(mtype, p7, isect12, rng6):
+    """evalAndSample with an explicit shading normal (isect12 = p, n, wo, sn)
+    and walk stream rng6 = (seed, stream, pixel, sample, slot, base dim)."""
+    out = np.zeros(7, np.float32)
+    lib().orc_bsdf_sample_walk(mtype, np.asarray(p7, np.float32), np.asarray(isect12, np.float32),
+                               np.asarray(rng6, np.uint32), out)
+    return out
+
+
+def bsdf_eval_walk(mtype, p7, isect12, wi3, rng6):
+    """(bsdfCos[3], getPDF) with an explicit shading normal and walk stream."""
+    out = np.zeros(4, np.float32)
+    lib().orc_bsdf_eval_walk(mtype, np.asarray(p7, np.float32), np.asarray(isect12, np.float32),
+                             np.asarray(wi3, np.float32), np.asarray(rng6, np.uint32), out)
+    return out
+
+
+def bsdf_moments(mtype, p7, isect12, n, mode, seed=1, nthreads=0):
+    """Monte Carlo moments: mode 0 samples (evalAndSample weights), mode 1
+    evaluates bsdfCos at uniform sphere directions.  Returns (mean[3],
+    first moment[3], variance of the channel mean, delta fraction)."""
+    out = np.zeros(8, np.float64)
+    lib().orc_bsdf_moments(mtype, np.asarray(p7, np.float32), np.asarray(isect12, np.float32), seed, n, mode,
+                           nthreads, out)
+    return out[0:3], out[3:6], out[6], out[7]
+
+
+def nmap_terms(w, wm, wp, wt, wg):
+    """(G1(w, wm), lambda_p(w)) of the two-facet model."""
+    out = np.zeros(2, np.float32)
+    f = lambda v: np.asarray(v, np.float32)
+    lib().orc_nmap_terms(f(w), f(wm), f(wp), f(wt), f(wg), out)
     return out
 
 

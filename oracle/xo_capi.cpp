@@ -48,6 +48,39 @@ int orc_scene_add_object(void* s, const float* positions, const float* normals, 
     return (int)sc->objects.size() - 1;
 }
 
+// Optional per-vertex texCoords (2 floats) and tangents/bitangents (3 floats).
+void orc_object_set_attributes(void* s, int obj, const float* texcoords, const float* tangents,
+                               const float* bitangents) {
+    Mesh& m = ((Scene*)s)->objects[obj].mesh;
+    const size_t nv = m.pos.size();
+    if (texcoords) m.uv.assign(texcoords, texcoords + 2 * nv);
+    if (tangents && bitangents) {
+        m.tan.resize(nv);
+        m.bitan.resize(nv);
+        for (size_t i = 0; i < nv; ++i) {
+            m.tan[i] = V3(tangents[3 * i], tangents[3 * i + 1], tangents[3 * i + 2]);
+            m.bitan[i] = V3(bitangents[3 * i], bitangents[3 * i + 1], bitangents[3 * i + 2]);
+        }
+    }
+}
+
+int orc_scene_add_texture(void* s, int width, int height, const float* rgb) {
+    Scene* sc = (Scene*)s;
+    Texture2D t;
+    t.w = width;
+    t.h = height;
+    t.rgb.assign(rgb, rgb + 3 * (size_t)width * height);
+    sc->textures.push_back(std::move(t));
+    return (int)sc->textures.size() - 1;
+}
+
+// Material::normalmap of material `mat` (texture index, -1 = none)
+void orc_material_set_normal_map(void* s, int mat, int tex) {
+    Scene* sc = (Scene*)s;
+    if ((int)sc->matNormalMap.size() <= mat) sc->matNormalMap.resize(mat + 1, -1);
+    sc->matNormalMap[mat] = tex;
+}
+
 // sky: has_sky, rgb[3]; sun: dir[3] (unit), 1 - cos(half), sin^2(half), rgb[3]
 void orc_scene_set_env(void* s, int has_sky, const float* sky_rgb, const float* sun_dir, float sun_omc,
                        float sun_sin2, const float* sun_rgb) {
@@ -227,6 +260,108 @@ void orc_bsdf_eval(uint32_t type, const float* p7, const float* isect9, const fl
     V3 f = bsdfCos(m, is, wi);
     out4[0] = f.x; out4[1] = f.y; out4[2] = f.z;
     out4[3] = bsdfPdf(m, is, wi);
+}
+
+// The same with an explicit shading normal and the walk stream of the
+// stochastic material: isect12 = p, n, wo, shading n; rng6 = seed, stream,
+// pixel, sample, vertex slot, base dim.
+static Isect isectFrom12(const float* i12, const Material* m) {
+    Isect is = isectFrom(i12, m);
+    is.sn = V3(i12[9], i12[10], i12[11]);
+    return is;
+}
+static PathRng rngFrom(const uint32_t* r6) {
+    PathRng r;
+    r.key[0] = r6[0]; r.key[1] = r6[1]; r.pixel = r6[2]; r.sample = r6[3];
+    return r;
+}
+void orc_bsdf_sample_walk(uint32_t type, const float* p7, const float* isect12, const uint32_t* rng6, float* out7) {
+    Material m; m.type = type; for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    Isect is = isectFrom12(isect12, &m);
+    PathRng r = rngFrom(rng6);
+    Walk w{&r, rng6[4], (int)rng6[5]};
+    V3 wi; float pdf = 0;
+    V3 v = evalAndSample(m, is, r.uni(rng6[4], DIM_BSDF_SEL), r.uni(rng6[4], DIM_BSDF_U1), r.uni(rng6[4], DIM_BSDF_U2),
+                         &wi, &pdf, &w);
+    out7[0] = wi.x; out7[1] = wi.y; out7[2] = wi.z; out7[3] = pdf; out7[4] = v.x; out7[5] = v.y; out7[6] = v.z;
+}
+void orc_bsdf_eval_walk(uint32_t type, const float* p7, const float* isect12, const float* wi3, const uint32_t* rng6,
+                        float* out4) {
+    Material m; m.type = type; for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    Isect is = isectFrom12(isect12, &m);
+    PathRng r = rngFrom(rng6);
+    Walk w{&r, rng6[4], (int)rng6[5]};
+    V3 wi(wi3[0], wi3[1], wi3[2]);
+    V3 f = bsdfCos(m, is, wi, &w);
+    out4[0] = f.x; out4[1] = f.y; out4[2] = f.z;
+    out4[3] = bsdfPdf(m, is, wi);
+}
+// G1 and lambda_p of the two-facet model (Main.cpp:323-337) for a frame
+// (wg, wp, wt) and direction w: out2 = G1(w, wm), lambda_p(w).
+void orc_nmap_terms(const float* w3, const float* wm3, const float* wp3, const float* wt3, const float* wg3, float* out2) {
+    V3 w(w3[0], w3[1], w3[2]), wm(wm3[0], wm3[1], wm3[2]), wp(wp3[0], wp3[1], wp3[2]);
+    V3 wt(wt3[0], wt3[1], wt3[2]), wg(wg3[0], wg3[1], wg3[2]);
+    out2[0] = nmapG1(w, wm, wp, wt, wg);
+    out2[1] = nmapLambdaP(w, wp, wt, wg);
+}
+
+// Monte Carlo moments of a (stochastic) BSDF at one isect, n samples on
+// `nthreads` threads.  mode 0: evalAndSample, x = returned weight; mode 1:
+// bsdfCos at directions uniform on the sphere, x = f / (1 / 4pi).  out[0..2]
+// = mean of x, out[3..5] = mean of mean(x) * wi, out[6] = variance of
+// mean(x) (per sample), out[7] = fraction of pdf < 0 (delta) samples.
+void orc_bsdf_moments(uint32_t type, const float* p7, const float* isect12, uint32_t seed, int n, int mode, int nthreads,
+                      double* out8) {
+    Material m; m.type = type; for (int i = 0; i < 7; ++i) m.p[i] = p7[i];
+    Isect is = isectFrom12(isect12, &m);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+    {
+        double loc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(static)
+        for (int k = 0; k < n; ++k) {
+            PathRng r;
+            r.key[0] = seed; r.key[1] = 77u; r.pixel = (uint32_t)k; r.sample = 0;
+            V3 x, wi;
+            if (mode == 0) {
+                Walk w{&r, 1, DIM_WALK_SAMPLE};
+                float pdf = 0;
+                x = evalAndSample(m, is, r.uni(1, DIM_BSDF_SEL), r.uni(1, DIM_BSDF_U1), r.uni(1, DIM_BSDF_U2), &wi, &pdf, &w);
+                if (pdf < 0) loc[7] += 1;
+            } else {
+                double z = 1.0 - 2.0 * r.uni(1, 0), ph = 2.0 * M_PI * r.uni(1, 1), rr = std::sqrt(std::max(0.0, 1 - z * z));
+                wi = V3((float)(rr * std::cos(ph)), (float)(rr * std::sin(ph)), (float)z);
+                Walk w{&r, 1, DIM_WALK_EVAL};
+                x = bsdfCos(m, is, wi, &w) * (float)(4.0 * M_PI);
+            }
+            double xm = ((double)x.x + x.y + x.z) / 3.0;
+            loc[0] += x.x; loc[1] += x.y; loc[2] += x.z;
+            loc[3] += xm * wi.x; loc[4] += xm * wi.y; loc[5] += xm * wi.z;
+            loc[6] += xm * xm;
+        }
+#pragma omp critical
+        for (int j = 0; j < 8; ++j) acc[j] += loc[j];
+    }
+    for (int j = 0; j < 8; ++j) out8[j] = acc[j] / n;
+    double mean = (out8[0] + out8[1] + out8[2]) / 3.0;
+    out8[6] = out8[6] - mean * mean;
+}
+
+// Shading normal (world) at the closest hit of each ray: out 3 floats per
+// ray, NaN on a miss.
+void orc_shading_normals(void* s, const float* rays, int n, float* out) {
+    const Scene* sc = (const Scene*)s;
+    for (int i = 0; i < n; ++i) {
+        Ray r;
+        r.o = V3(rays[8 * i], rays[8 * i + 1], rays[8 * i + 2]);
+        r.tMax = rays[8 * i + 3];
+        r.d = V3(rays[8 * i + 4], rays[8 * i + 5], rays[8 * i + 6]);
+        Isect is;
+        V3 sn(NAN, NAN, NAN);
+        if (sc->intersect(r, &is)) sn = is.sn;
+        out[3 * i] = sn.x; out[3 * i + 1] = sn.y; out[3 * i + 2] = sn.z;
+    }
 }
 
 float orc_ggx_D(float alpha, const float* h3, const float* n3) {

@@ -169,7 +169,10 @@ enum RngDim : int {
     DIM_LIGHT_KIND = 4, DIM_LIGHT_OBJ = 5, DIM_LIGHT_TRI = 6, DIM_ENV_SEL = 7,
     DIM_BARY0 = 8, DIM_BARY1 = 9, DIM_BARY2 = 10,
     DIM_ENV_U1 = 5, DIM_ENV_U2 = 6,   // an NEE sample is either an area point or an env direction
-    DIM_FILM_X = 0, DIM_FILM_Y = 1   // vertex slot 0 (camera)
+    DIM_FILM_X = 0, DIM_FILM_Y = 1,  // vertex slot 0 (camera)
+    // uniform streams of the stochastic (random-walk) BSDF: sampling, the
+    // light-sample evaluation, and a second evaluation of the same vertex
+    DIM_WALK_SAMPLE = 64, DIM_WALK_EVAL = 1024, DIM_WALK_EVAL2 = 2048
 };
 
 struct PathRng {
@@ -188,7 +191,8 @@ struct PathRng {
 // ---------------------------------------------------------------- materials (L3)
 enum MatType : uint32_t {
     MAT_DIFFUSE = 0, MAT_SPEC_REFL = 1, MAT_SPEC_REFR = 2, MAT_SPEC_FRESNEL = 3,
-    MAT_METAL = 4, MAT_EMISSION = 5, MAT_GGX_LAMBERT = 6
+    MAT_METAL = 4, MAT_EMISSION = 5, MAT_GGX_LAMBERT = 6,
+    MAT_MICROFACET_NMAP = 7   // MicrofacetNormalMapping over a Metal base (p[0] alpha, p[1..3] f0)
 };
 struct Material {
     uint32_t type = MAT_DIFFUSE;
@@ -206,6 +210,28 @@ struct Isect {
     const Material* mat = nullptr;
     float t = Infinity;
     float b0 = 0, b1 = 0;  // barycentrics of p0, p1 from the triangle test
+};
+
+// Texture::rgb with the default bilinear filter and clamped addressing
+// (Texture.h:83-103, 161-171); RGB floats, row 0 first.
+struct Texture2D {
+    int w = 0, h = 0;
+    std::vector<float> rgb;
+    V3 texel(int x, int y) const {
+        x = std::min(std::max(x, 0), w - 1);
+        y = std::min(std::max(y, 0), h - 1);
+        const float* t = &rgb[3 * ((size_t)y * w + x)];
+        return V3(t[0], t[1], t[2]);
+    }
+    V3 bilinear(float u, float v) const {
+        int x0 = (int)floorf(u * w - 0.5f);
+        int y0 = (int)floorf(v * h - 0.5f);
+        float wx1 = (u * w - 0.5f) - x0;
+        float wy1 = (v * h - 0.5f) - y0;
+        float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
+        return wx0 * wy0 * texel(x0, y0) + wx0 * wy1 * texel(x0, y0 + 1) + wx1 * wy0 * texel(x0 + 1, y0) +
+               wx1 * wy1 * texel(x0 + 1, y0 + 1);
+    }
 };
 
 // GGX pieces, Material.h:288-378 (Metal).  Arithmetic types follow the
@@ -259,8 +285,25 @@ inline V3 schlick(const V3& f0, float c) {                                // Mat
 }
 inline bool finite3(const V3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
 
-// Material::bsdfCos (Material.h:17) per subclass.
-inline V3 bsdfCos(const Material& m, const Isect& is, const V3& wi) {
+// Uniform stream of one call of a stochastic BSDF (the random walk of
+// MicrofacetNormalMapping, which draws from the Sampler inside bsdfCos and
+// evalAndSample): dim `base` picks the first facet, dims base + 4 + 4k ..
+// base + 7 + 4k are walk step k (lobe select, u1, u2, escape test), so each
+// step is one Philox block.
+struct Walk {
+    const PathRng* rng = nullptr;
+    uint32_t slot = 0;
+    int base = 0;
+    float first() const { return rng->uni(slot, base); }
+    float step(int k, int j) const { return rng->uni(slot, base + 4 + 4 * k + j); }
+};
+inline V3 nmapBsdfCos(const Material& m, const Isect& is, const V3& wi, const Walk* walk);
+inline float nmapPdf(const Material& m, const Isect& is, const V3& wi);
+inline V3 nmapEvalAndSample(const Material& m, const Isect& is, const Walk* walk, V3* wi, float* pdf);
+
+// Material::bsdfCos (Material.h:17) per subclass.  `walk` is read only by
+// the stochastic material.
+inline V3 bsdfCos(const Material& m, const Isect& is, const V3& wi, const Walk* walk = nullptr) {
     switch (m.type) {
     case MAT_DIFFUSE:                                                       // Material.h:59-65
         return m.rgb() / (float)M_PI * clampPositive(dot(is.sn, wi));
@@ -288,6 +331,8 @@ inline V3 bsdfCos(const Material& m, const Isect& is, const V3& wi) {
         V3 diff = m.rgb() * ((1.0f - m.p[4]) / (float)M_PI) * cosI;
         return spec + diff;
     }
+    case MAT_MICROFACET_NMAP:
+        return nmapBsdfCos(m, is, wi, walk);
     default:                                                                // delta / emission: Material.h:396-398
         return V3(0.0f);
     }
@@ -311,6 +356,8 @@ inline float bsdfPdf(const Material& m, const Isect& is, const V3& wi) {
         float pd = (float)((double)cosI / M_PI);
         return ps * g.pdf(is.wo, is.sn, wi) + (1.0f - ps) * pd;
     }
+    case MAT_MICROFACET_NMAP:
+        return nmapPdf(m, is, wi);
     default:
         return 0.0f;
     }
@@ -341,8 +388,10 @@ inline V3 refractDir(const V3& wo, const V3& n, float rindex) {           // Mat
 }
 
 // Material::evalAndSample (Material.h:25): returns f*cos/pdf, sets wi and pdf
-// (pdf = -1 for delta lobes).  u = (sel, u1, u2) from dims 1..3.
-inline V3 evalAndSample(const Material& m, const Isect& is, float usel, float u1, float u2, V3* wi, float* pdf) {
+// (pdf = -1 for delta lobes).  u = (sel, u1, u2) from dims 1..3; the
+// stochastic material draws from `walk` instead.
+inline V3 evalAndSample(const Material& m, const Isect& is, float usel, float u1, float u2, V3* wi, float* pdf,
+                        const Walk* walk = nullptr) {
     switch (m.type) {
     case MAT_DIFFUSE: {                                                     // Material.h:67-76
         *wi = sampleCosHemisphere(is.sn, u1, u2);
@@ -395,10 +444,174 @@ inline V3 evalAndSample(const Material& m, const Isect& is, float usel, float u1
         if (!finite3(r)) return V3(0.0f);
         return r;
     }
+    case MAT_MICROFACET_NMAP:
+        return nmapEvalAndSample(m, is, walk, wi, pdf);
     default:                                                                // Emission: Material.h:400-402
         *pdf = 0.0f;
         return V3(0.0f);
     }
+}
+
+// ---------------------------------------------------------------- microfacet-based normal mapping
+// Schuessler et al. 2017, the MicrofacetNormalMapping material of
+// Sandbox/MicrofacetBasedNormalMapping/Main.cpp:86-340 in its
+// SameMaterialExplicit mode (Main.cpp:22): the normal-mapped facet wp and the
+// tangent facet wt both carry the base material.  Parameters: p[0] = base
+// material type (Diffuse, Metal or GGX+Lambert), p[1..6] = the base's p[0..5].
+inline Material nmapBase(const Material& m) {
+    Material b;
+    b.type = (uint32_t)m.p[0];
+    for (int k = 0; k < 6; ++k) b.p[k] = m.p[k + 1];
+    return b;
+}
+// Facet frame (Main.cpp:96-117, 178-194, 244-260): 0 = wg == wp (the base
+// material alone), 1 = wt has a NaN, 2 = the walk.
+inline int nmapFrame(const Isect& is, V3* wg, V3* wp, V3* wt) {
+    *wg = is.n;
+    *wp = is.sn;
+    if (dot(is.wo, *wg) < 0.0f) *wg *= -1.0f;
+    if (dot(*wg, *wp) < 0.0f) *wp *= -1.0f;
+    // Vector3f::operator== compares the lanes (Vector.h:344-349)
+    if (wg->x == wp->x && wg->y == wp->y && wg->z == wp->z) return 0;
+    *wt = cross(cross(*wg, *wp).normalize(), *wg).normalize();
+    if (std::isnan(wt->x) || std::isnan(wt->y) || std::isnan(wt->z)) return 1;
+    if (dot(*wp, *wt) > 0) *wt *= -1.0f;
+    return 2;
+}
+// Main.cpp:323-337; min is Utils.h:37 (x < y ? x : y, so a NaN y passes).
+inline float nmapAp(const V3& w, const V3& wp, const V3& wg) { return clampPositive(dot(w, wp)) / fabsf(dot(wp, wg)); }
+inline float nmapAt(const V3& w, const V3& wp, const V3& wt, const V3& wg) {
+    float d = dot(wp, wg);
+    return clampPositive(dot(w, wt)) * safeSqrt(1 - powf(d, 2)) / fabsf(d);
+}
+inline float nmapLambdaP(const V3& w, const V3& wp, const V3& wt, const V3& wg) {
+    return nmapAp(w, wp, wg) / (nmapAp(w, wp, wg) + nmapAt(w, wp, wt, wg));
+}
+inline float nmapG1(const V3& w, const V3& wm, const V3& wp, const V3& wt, const V3& wg) {
+    if (!(dot(w, wm) > 0.0f)) return 0.0f;
+    float g = clampPositive(dot(w, wg)) / (nmapAp(w, wp, wg) + nmapAt(w, wp, wt, wg));
+    return 1.0f < g ? 1.0f : g;
+}
+inline bool isZero3(const V3& v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; }
+// the isect the base material sees on facet wm (Main.cpp:125-130)
+inline Isect nmapFacet(const Isect& is, const V3& wm, const V3& wr) {
+    Isect f = is;
+    f.n = wm;
+    f.sn = wm;
+    f.wo = -wr;
+    return f;
+}
+// The base on a facet is one-sided in both directions, as GlossyClamped, the
+// base the reference pairs with this material (Main.cpp:31-84): zero unless
+// wo and wi are both above the facet.  (Unclamped, GGX's bsdfCos divides by
+// a non-positive wo.n once the walk reaches a facet from behind.)
+inline bool nmapSides(const Isect& f, const V3& wi) { return dot(f.wo, f.sn) > 0.0f && dot(wi, f.sn) > 0.0f; }
+inline V3 nmapFacetCos(const Material& base, const Isect& f, const V3& wi) {
+    return nmapSides(f, wi) ? bsdfCos(base, f, wi) : V3(0.0f);
+}
+inline float nmapFacetPdf(const Material& base, const Isect& f, const V3& wi) {
+    return nmapSides(f, wi) ? bsdfPdf(base, f, wi) : 0.0f;
+}
+inline V3 nmapFacetSample(const Material& base, const Isect& f, const Walk* walk, int k, V3* wr, float* pdf) {
+    V3 v = evalAndSample(base, f, walk->step(k, 0), walk->step(k, 1), walk->step(k, 2), wr, pdf);
+    if (!nmapSides(f, *wr)) {
+        *pdf = 0.0f;
+        return V3(0.0f);
+    }
+    return v;
+}
+// bsdfCos as a random walk on the two facets (Main.cpp:94-170).
+inline V3 nmapBsdfCos(const Material& m, const Isect& is, const V3& wi, const Walk* walk) {
+    const Material base = nmapBase(m);
+    V3 wg, wp, wt;
+    int mode = nmapFrame(is, &wg, &wp, &wt);
+    if (mode == 0) return bsdfCos(base, is, wi);
+    if (mode == 1) return V3(0.0f);
+    V3 result(0.0f), weight(1.0f);
+    bool d = walk->first() < nmapLambdaP(is.wo, wp, wt, wg);
+    V3 wr = -is.wo;
+    int i = 0;
+    float pdf;
+    while (true) {
+        const V3& wm = d ? wp : wt;
+        Isect f = nmapFacet(is, wm, wr);
+        result += weight * nmapG1(wi, wm, wp, wt, wg) * nmapFacetCos(base, f, wi);
+        weight *= nmapFacetSample(base, f, walk, i, &wr, &pdf);
+        float g = nmapG1(wr, wm, wp, wt, wg);
+        if (walk->step(i, 3) < g) break;
+        d = !d;
+        if (isZero3(weight)) break;
+        ++i;
+        if (i > 100) break;
+    }
+    return result;
+}
+// evalAndSample as the same walk (Main.cpp:172-241); pdf = -1 only when every
+// step sampled a delta lobe.  A NaN tangent facet leaves the reference's pdf
+// unset; here it is 0.
+inline V3 nmapEvalAndSample(const Material& m, const Isect& is, const Walk* walk, V3* wi, float* pdf) {
+    const Material base = nmapBase(m);
+    V3 wg, wp, wt;
+    int mode = nmapFrame(is, &wg, &wp, &wt);
+    if (mode == 0) return evalAndSample(base, is, walk->step(0, 0), walk->step(0, 1), walk->step(0, 2), wi, pdf);
+    if (mode == 1) {
+        *pdf = 0.0f;
+        return V3(0.0f);
+    }
+    V3 weight(1.0f);
+    bool d = walk->first() < nmapLambdaP(is.wo, wp, wt, wg);
+    V3 wr = -is.wo;
+    int i = 0;
+    float tmppdf;
+    bool specular = true;
+    while (true) {
+        const V3& wm = d ? wp : wt;
+        Isect f = nmapFacet(is, wm, wr);
+        weight *= nmapFacetSample(base, f, walk, i, &wr, &tmppdf);
+        if (tmppdf >= 0.0f) specular = false;
+        float g = nmapG1(wr, wm, wp, wt, wg);
+        if (walk->step(i, 3) < g) break;
+        d = !d;
+        if (isZero3(weight)) break;
+        ++i;
+        if (i > 100) {
+            weight *= V3(0.0f);
+            break;
+        }
+    }
+    *wi = wr;
+    *pdf = specular ? -1.0f : nmapPdf(m, is, *wi);
+    return weight;
+}
+// getPDF (Main.cpp:243-318): the approximation that treats the facets'
+// inter-reflection as diffuse.  The tangent-facet terms are weighted by
+// probability_wp as the reference writes them (Main.cpp:295-296), not by
+// 1 - probability_wp.
+inline float nmapPdf(const Material& m, const Isect& is, const V3& wi) {
+    const Material base = nmapBase(m);
+    V3 wg, wp, wt;
+    int mode = nmapFrame(is, &wg, &wp, &wt);
+    if (mode == 0) return bsdfPdf(base, is, wi);
+    if (mode == 1) return 1.0f;
+    float probability_wp = nmapLambdaP(is.wo, wp, wt, wg);
+    float pdf = 0.0f, diff_pdf = 0.0f;
+    float g_wi_wp = nmapG1(wi, wp, wp, wt, wg);
+    if (probability_wp > 0.0f) {
+        Isect f = is;
+        f.n = wp;
+        f.sn = wp;
+        pdf += probability_wp * g_wi_wp * nmapFacetPdf(base, f, wi);
+        diff_pdf += probability_wp * (1.0f - g_wi_wp);
+    }
+    if (probability_wp < 1.0f) {
+        Isect f = is;
+        f.n = wt;
+        f.sn = wt;
+        float g_wi_wt = nmapG1(wi, wt, wp, wt, wg);
+        pdf += probability_wp * g_wi_wt * nmapFacetPdf(base, f, wi);
+        diff_pdf += probability_wp * g_wi_wt;
+    }
+    return (float)((double)pdf + (double)(diff_pdf * clampPositive(dot(wi, wg))) / M_PI);
 }
 
 // Emission::getEmission (Material.h:408-410), one-sided on n.
@@ -409,6 +622,8 @@ inline V3 emission(const Material& m, const V3& wo, const V3& n) {
 // ---------------------------------------------------------------- geometry (L2)
 struct Mesh {                                                              // TriangleMesh.h:237-251
     std::vector<V3> pos, nrm;
+    std::vector<float> uv;          // 2 per vertex (texCoords), optional
+    std::vector<V3> tan, bitan;     // per vertex, optional
     std::vector<int> idx;
     int ntri() const { return (int)idx.size() / 3; }
     const V3& P(int t, int k) const { return pos[idx[t * 3 + k]]; }
@@ -736,6 +951,8 @@ struct SampledLight { V3 p, n, sn; int obj = -1, tri = -1; float pdf = 0; };
 struct Scene {
     std::vector<Object> objects;
     std::vector<Material> materials;
+    std::vector<int> matNormalMap;        // per material: index into textures, -1 = none (Material::normalmap)
+    std::vector<Texture2D> textures;
     std::vector<int> lights;     // emissive objects in insertion order (Scene.h:17-22)
     Env env;
     RefBVH tlas;
@@ -797,6 +1014,25 @@ struct Scene {
         if (!m.nrm.empty()) n = lerp3(m.N(t, 0), m.N(t, 1), m.N(t, 2), b0, b1).normalize();
         else n = cross(m.P(t, 1) - m.P(t, 0), m.P(t, 2) - m.P(t, 0)).normalize();
         V3 sn = faceForward(n, wo);
+        const uint32_t mi = o.matOf(t);
+        const int nm = mi < matNormalMap.size() ? matNormalMap[mi] : -1;
+        if (nm >= 0) {
+            // TriangleIndexed.h:139-168 (texCoord, tangent, bitangent), then the
+            // normal map in object space, Object.h:59-68
+            V3 uv(0.0f), tg(0.0f), bt(0.0f);
+            if (!m.uv.empty()) {
+                auto UV = [&](int k) { int v = m.idx[t * 3 + k]; return V3(m.uv[2 * v], m.uv[2 * v + 1], 0.0f); };
+                uv = lerp3(UV(0), UV(1), UV(2), b0, b1);
+            }
+            if (!m.tan.empty()) {
+                auto T = [&](int k) { return m.tan[m.idx[t * 3 + k]]; };
+                auto B = [&](int k) { return m.bitan[m.idx[t * 3 + k]]; };
+                tg = lerp3(T(0), T(1), T(2), b0, b1).normalize();
+                bt = lerp3(B(0), B(1), B(2), b0, b1).normalize();
+            }
+            V3 c = textures[nm].bilinear(uv.x, uv.y) * 2.0f - V3(1.0f);
+            sn = (c.z * sn + c.x * tg + c.y * bt).normalize();
+        }
         is->p = o.xf(p);
         is->n = o.xf.asNormal(n);
         is->wo = o.xf.asNormal(wo);
@@ -1040,8 +1276,9 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
         bool nextSampled = true;
         float pdfB;
         V3 wi;
+        const Walk walkS{&rng, slot, DIM_WALK_SAMPLE}, walkE{&rng, slot, DIM_WALK_EVAL};
         V3 meval = evalAndSample(*isect.mat, isect, rng.uni(slot, DIM_BSDF_SEL), rng.uni(slot, DIM_BSDF_U1),
-                                 rng.uni(slot, DIM_BSDF_U2), &wi, &pdfB);
+                                 rng.uni(slot, DIM_BSDF_U2), &wi, &pdfB, &walkS);
         if (!meval.isZero()) {
             cur.d = wi;
             cur.o = isect.p + rp.eps * cur.d;
@@ -1083,7 +1320,7 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
             if (pdfB >= 0.0f) {
                 V3 d = envn.sample(isect.sn, rng.uni(slot, DIM_ENV_SEL), rng.uni(slot, DIM_ENV_U1), rng.uni(slot, DIM_ENV_U2));
                 float pe = envn.pSelect * envn.pdf(d, isect.sn);
-                V3 fc = bsdfCos(*isect.mat, isect, d);
+                V3 fc = bsdfCos(*isect.mat, isect, d, &walkE);
                 float pb = bsdfPdf(*isect.mat, isect, d);
                 float mis = (pe * pe) / (pb * pb + pe * pe);
                 if (pe > 0.0f && !fc.isZero() && mis > 0.0f) {   // a NaN pdf drops the sample
@@ -1118,7 +1355,7 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
                     if (mis > 0.0f) {
                         float G = fabsf(dot(-sr.d, ls.sn)) / distSq;
                         const Material& lm = sc.materials[sc.objects[ls.obj].matOf(ls.tri)];
-                        radiance += weight * mis * bsdfCos(*isect.mat, isect, sr.d) * emission(lm, -sr.d, ls.n) * G / pdfL;
+                        radiance += weight * mis * bsdfCos(*isect.mat, isect, sr.d, &walkE) * emission(lm, -sr.d, ls.n) * G / pdfL;
                     }
                 }
             }
@@ -1153,8 +1390,9 @@ inline V3 evalNaive(const Scene& sc, const PathRng& rng, const Ray& camRay, cons
             if (is.mat->emissive()) radiance += weight * emission(*is.mat, -cur.d, is.n);
             float pdf;
             V3 wi;
+            const Walk walkS{&rng, slot, DIM_WALK_SAMPLE};
             weight *= evalAndSample(*is.mat, is, rng.uni(slot, DIM_BSDF_SEL), rng.uni(slot, DIM_BSDF_U1),
-                                    rng.uni(slot, DIM_BSDF_U2), &wi, &pdf);
+                                    rng.uni(slot, DIM_BSDF_U2), &wi, &pdf, &walkS);
             if (weight.isZero()) break;
             cur.d = wi;
             cur.o = is.p + rp.eps * cur.d;

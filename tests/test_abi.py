@@ -40,6 +40,7 @@ STRUCTS = {
     "xt_object_desc": api.xt_object_desc,
     "xt_env": api.xt_env,
     "xt_scene_desc": api.xt_scene_desc,
+    "xt_texture": api.xt_texture,
     "xt_camera": api.xt_camera,
     "xt_render_params": api.xt_render_params,
     "xt_render_stats": api.xt_render_stats,

@@ -48,7 +48,12 @@ class xt_object_desc(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
                 ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
                 ("object_to_world", C.c_void_p), ("world_to_object", C.c_void_p),
-                ("material", C.c_uint32), ("tri_materials", C.c_void_p)]
+                ("material", C.c_uint32), ("tri_materials", C.c_void_p),
+                ("texcoords", C.c_void_p), ("tangents", C.c_void_p), ("bitangents", C.c_void_p)]
+
+
+class xt_texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.c_void_p)]
 
 
 class xt_env(C.Structure):
@@ -60,7 +65,8 @@ class xt_env(C.Structure):
 
 class xt_scene_desc(C.Structure):
     _fields_ = [("objects", C.c_void_p), ("num_objects", C.c_uint32), ("materials", C.c_void_p),
-                ("num_materials", C.c_uint32), ("env", xt_env)]
+                ("num_materials", C.c_uint32), ("env", xt_env), ("textures", C.c_void_p),
+                ("num_textures", C.c_uint32), ("material_normal_maps", C.c_void_p)]
 
 
 class xt_camera(C.Structure):
@@ -297,6 +303,12 @@ class Scene:
                 tm = np.ascontiguousarray(o.tri_materials, np.uint32)
                 keep.append(tm)
                 objs[k].tri_materials = tm.ctypes.data
+            for name in ("texcoords", "tangents", "bitangents"):
+                a = getattr(o, name, None)
+                if a is not None:
+                    a = np.ascontiguousarray(a, np.float32)
+                    keep.append(a)
+                    setattr(objs[k], name, a.ctypes.data)
         mats = _sc.pack_materials(desc.materials)
         keep.append(mats)
         sd = xt_scene_desc()
@@ -305,6 +317,20 @@ class Scene:
         sd.materials = mats.ctypes.data
         sd.num_materials = mats.shape[0]
         sd.env = env_struct(desc.env, keep)
+        texs = getattr(desc, "textures", None) or []
+        if texs:
+            tarr = (xt_texture * len(texs))()
+            for k, t in enumerate(texs):
+                t = np.ascontiguousarray(t, np.float32)
+                keep.append(t)
+                tarr[k].width, tarr[k].height, tarr[k].rgb = t.shape[1], t.shape[0], t.ctypes.data
+            keep.append(tarr)
+            sd.textures = C.cast(tarr, C.c_void_p)
+            sd.num_textures = len(texs)
+        nmaps = _sc.material_normal_maps(desc.materials)
+        if nmaps is not None:
+            keep.append(nmaps)
+            sd.material_normal_maps = nmaps.ctypes.data
         h = C.c_void_p()
         check(lib().xt_scene_create(device.h, C.byref(sd), C.byref(h)))
         self.h = h

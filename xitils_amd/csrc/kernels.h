@@ -115,7 +115,7 @@ __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, i
                         int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
-template <bool GUIDE>
+template <bool GUIDE, bool NMAP>
 __global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_t* count_in, const float4* hits,
                         PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
                         float4* radiance, float4* pending, GuideRender gr);

@@ -15,6 +15,7 @@
 // does not depend on queue order.
 #include "kernels.h"
 #include "xt_traverse.h"
+#include "xt_nmap.h"
 
 namespace xt {
 
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, fl
     }
 }
 
-template <bool GUIDE>
+template <bool GUIDE, bool NMAP>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
                                                const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
                                                PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
             xt_material mat;
             bool emissive = false;
             if (hit) {
-                s = make_surf(sc, slot, d, h.z, h.w);
+                s = make_surf<NMAP>(sc, slot, d, h.z, h.w);
                 mat = sc.mats[s.mat];
                 emissive = mat.type == XT_MAT_EMISSION;
             }
@@ -298,6 +299,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
                     // pdf wherever the BSDF pdf is positive, and zero elsewhere.
                     guided = GUIDE && gr.lobes4 != nullptr && !mat_is_delta(mat.type);
+                    const WalkKey wk{key, rng.pixel, rng.sample, v};
                     auto l4 = [&](int j) { return gr.lobes4[(size_t)j * gr.stride + i]; };
                     auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
                     // (a NaN BSDF pdf -- GGX D at a grazing half vector -- stays NaN, so
@@ -308,22 +310,30 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     };
                     if (guided) {
                         const uint4 blk3 = rng.block(v, 3);
+                        // The random walk's sampling density is not its getPDF (an
+                        // approximation, Main.cpp:245), so a walk-sampled direction is
+                        // weighted with its throughput, the unbiased estimate of f / p:
+                        // me = throughput * p^ / (c q + (1 - c) p^), the one-sample MIS
+                        // whose weights (computed with p^) still sum to one.
+                        const bool walk = NMAP && mat.type == XT_MAT_MICROFACET_NMAP;
+                        float pb;
+                        f3 fc;
                         if (Rng::to_unit(blk3.x) < gr.prob) {
                             wi = guide_sample_lobes(l4, l2, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z),
                                                     Rng::to_unit(blk3.w));
+                            fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
+                        } else if (walk) {
+                            fc = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pb) * pb;
                         } else {
-                            bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
-                                        &pdfB);
+                            mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
+                            fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
                         }
-                        float pb;
-                        const f3 fc = bsdf_eval(mat, s, wi, &pb);
                         pdfB = strategy_pdf(wi, pb);
                         if (!(pdfB > 0.0f)) pdfB = 0.0f;
                         me = pdfB > 0.0f ? div(fc, pdfB) : splat(0.0f);
                         if (!finite3(me)) me = splat(0.0f);
                     } else {
-                        me = bsdf_sample(mat, s, fs, Rng::to_unit(blk0.y), Rng::to_unit(blk0.z), Rng::to_unit(blk0.w), &wi,
-                                         &pdfB);
+                        me = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
                     }
                     if (!is_zero(me)) {                            // PathTracer.h:165-167
                         push_ext = true;
@@ -346,7 +356,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                                                    Rng::to_unit(blk1.z));
                             float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
                             float pb;
-                            f3 fc = bsdf_eval(mat, s, ld, &pb);
+                            f3 fc = mat_eval<NMAP>(mat, s, ld, &pb, wk, D_WALK_EVAL);
                             if (guided) pb = strategy_pdf(ld, pb);
                             const float mis = (pe * pe) / (pb * pb + pe * pe);
                             if (pe > 0.0f && !is_zero(fc) && mis > 0.0f) {   // NaN mis: dropped
@@ -397,7 +407,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                             float distSq = dist * dist;
                             float mis;
                             float pb;
-                            f3 fc = bsdf_eval(mat, s, sd, &pb);
+                            f3 fc = mat_eval<NMAP>(mat, s, sd, &pb, wk, D_WALK_EVAL);
                             if (guided) pb = strategy_pdf(sd, pb);
                             if (pdfB >= 0.0f) {
                                 float cosL = fabsf(dot(-sd, lsn));
@@ -423,8 +433,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                 uint4 blk0 = rng.block(v, 0);
                 f3 wi = splat(0.0f);
                 float pdfB = 0.0f;
-                weight = weight * bsdf_sample(mat, s, make_frame(s.sn), Rng::to_unit(blk0.y), Rng::to_unit(blk0.z),
-                                              Rng::to_unit(blk0.w), &wi, &pdfB);
+                const WalkKey wk{key, rng.pixel, rng.sample, v};
+                weight = weight * mat_sample<NMAP>(mat, s, make_frame(s.sn), blk0, wk, &wi, &pdfB);
                 if (is_zero(weight)) live = false;
                 if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
                 if (live && (int)(v + 1) > wp.rr_start) {
@@ -476,10 +486,14 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
     }
 }
 
-template __global__ void k_shade<false>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue,
-                                        uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
-template __global__ void k_shade<true>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue,
-                                       uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
+#define XT_INST_SHADE(G, N)                                                                                   \
+    template __global__ void k_shade<G, N>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue, \
+                                           uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
+XT_INST_SHADE(false, false)
+XT_INST_SHADE(true, false)
+XT_INST_SHADE(false, true)
+XT_INST_SHADE(true, true)
+#undef XT_INST_SHADE
 
 // Any-hit for queued shadow rays; unoccluded contributions go straight to the
 // path radiance, or to pending[] when the path continues (see k_shade).

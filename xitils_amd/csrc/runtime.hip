@@ -109,6 +109,10 @@ struct xt_scene {
     float4* d_nodes = nullptr;
     float4* d_tris = nullptr;
     float4* d_normals = nullptr;
+    float4* d_nmap_attr = nullptr;    // normal-map attributes per slot
+    bool nmap = false;                // normal maps or microfacet normal mapping: the NMAP shade kernels
+    DevTexture* d_textures = nullptr;
+    std::vector<float4*> d_texels;    // one allocation per texture
     xt_material* d_mats = nullptr;
     float* d_light_pdf = nullptr;
     uint2* d_light_objs = nullptr;
@@ -139,6 +143,13 @@ struct xt_scene {
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
     GuideState guide;
 };
+
+using ShadeKernel = void (*)(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue, uint32_t*,
+                            ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
+static ShadeKernel shade_kernel(bool guide, bool nmap) {
+    if (nmap) return guide ? k_shade<true, true> : k_shade<false, true>;
+    return guide ? k_shade<true, false> : k_shade<false, false>;
+}
 
 extern "C" {
 
@@ -175,6 +186,8 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipSetDevice(s->dev->ordinal);
     if (s->stream) hipStreamSynchronize(s->stream);
     hipFree(s->d_nodes); hipFree(s->d_tris); hipFree(s->d_normals); hipFree(s->d_mats);
+    hipFree(s->d_nmap_attr); hipFree(s->d_textures);
+    for (float4* t : s->d_texels) hipFree(t);
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
     hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc);
     hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
@@ -255,10 +268,33 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
         if ((uint64_t)env_in.map_width * env_in.map_height > (1ull << 28))
             return fail(XT_ERR_TOO_LARGE, "env map larger than 2^28 texels");
     }
-    for (uint32_t m = 0; m < desc->num_materials; ++m)
-        if (desc->materials[m].type >= XT_MAT_COUNT_)
+    for (uint32_t m = 0; m < desc->num_materials; ++m) {
+        const xt_material& mm = desc->materials[m];
+        if (mm.type >= XT_MAT_COUNT_)
             return fail(XT_ERR_UNSUPPORTED_MATERIAL, "material " + std::to_string(m) + " has unknown type " +
-                                                         std::to_string(desc->materials[m].type));
+                                                         std::to_string(mm.type));
+        if (mm.type == XT_MAT_MICROFACET_NMAP) {
+            const float bt = mm.p[0];
+            if (!(bt == (float)XT_MAT_DIFFUSE || bt == (float)XT_MAT_METAL || bt == (float)XT_MAT_GGX_LAMBERT))
+                return fail(XT_ERR_UNSUPPORTED_MATERIAL, "material " + std::to_string(m) +
+                                                             ": microfacet normal mapping needs a Diffuse, Metal or "
+                                                             "GGX+Lambert base");
+        }
+    }
+    if (desc->num_textures > 0 && !desc->textures) return fail(XT_ERR_INVALID_ARGUMENT, "textures is NULL");
+    for (uint32_t k = 0; k < desc->num_textures; ++k) {
+        const xt_texture& t = desc->textures[k];
+        if (!t.rgb || t.width == 0 || t.height == 0)
+            return fail(XT_ERR_INVALID_ARGUMENT, "texture " + std::to_string(k) + " has no texels");
+        if ((uint64_t)t.width * t.height > (1ull << 28))
+            return fail(XT_ERR_TOO_LARGE, "texture " + std::to_string(k) + " larger than 2^28 texels");
+    }
+    const uint32_t* mat_nmap = desc->material_normal_maps;
+    if (mat_nmap)
+        for (uint32_t m = 0; m < desc->num_materials; ++m)
+            if (mat_nmap[m] > desc->num_textures)
+                return fail(XT_ERR_INVALID_ARGUMENT, "material " + std::to_string(m) + " normal map out of range");
+    auto nmap_of = [&](uint32_t m) -> uint32_t { return mat_nmap ? mat_nmap[m] : 0u; };
     auto t_start = std::chrono::steady_clock::now();
     // ---- flatten objects to world space (Object.h:106-144 applied once per vertex)
     uint64_t total = 0;
@@ -289,6 +325,8 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
             return fail(XT_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + " material out of range");
         if (od.object_to_world && !od.world_to_object)
             return fail(XT_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + " has a transform without its inverse");
+        if ((od.tangents != nullptr) != (od.bitangents != nullptr))
+            return fail(XT_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + " has tangents without bitangents");
         const float* M = od.object_to_world;
         const bool emissive_obj = desc->materials[od.material].type == XT_MAT_EMISSION;
         float area = 0.0f, scaled = 0.0f;
@@ -362,6 +400,10 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
         }
         uint32_t flags = (prim_has_n[p] ? TRI_FLAG_NORMALS : 0u) |
                          (desc->materials[prim_mat[p]].type == XT_MAT_EMISSION ? TRI_FLAG_EMISSIVE : 0u);
+        if (const uint32_t nm = nmap_of(prim_mat[p])) {
+            flags |= nm << TRI_NMAP_SHIFT;
+            if (desc->objects[prim_obj[p]].tangents) flags |= TRI_FLAG_TANGENTS;
+        }
         std::memcpy(&r[3], &p, 4);
         std::memcpy(&r[7], &prim_mat[p], 4);
         std::memcpy(&r[11], &flags, 4);
@@ -370,15 +412,52 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
                 for (int c = 0; c < 3; ++c) nrms[12 * (size_t)s + 4 * k + c] = prim_nrm[9 * (size_t)p + 3 * k + c];
         slot_pdf[s] = prim_light_pdf[p];
     }
-    std::vector<uint2> lobjs;
-    std::vector<uint32_t> lslots;
+    std::vector<uint32_t> obj_base(desc->num_objects);
     {
         uint32_t b2 = 0;
-        std::vector<uint32_t> obj_base(desc->num_objects);
         for (uint32_t o = 0; o < desc->num_objects; ++o) {
             obj_base[o] = b2;
             b2 += desc->objects[o].num_triangles;
         }
+    }
+    // ---- normal-map attributes per slot (tangent frame in world space, as the normals)
+    bool any_nmap = false;
+    for (uint32_t p = 0; p < nprim && !any_nmap; ++p) any_nmap = nmap_of(prim_mat[p]) != 0;
+    std::vector<float4> nmap_attr;
+    if (any_nmap) {
+        nmap_attr.assign((size_t)nprim * NMAP_F4, make_float4(0, 0, 0, 0));
+        for (uint32_t sl = 0; sl < nprim; ++sl) {
+            const uint32_t p = bvh.slot_prim[sl];
+            if (!nmap_of(prim_mat[p])) continue;
+            const uint32_t o = prim_obj[p];
+            const xt_object_desc& od = desc->objects[o];
+            const uint32_t t = p - obj_base[o];
+            const float* M = od.object_to_world;
+            float4* a = &nmap_attr[(size_t)sl * NMAP_F4];
+            float uvs[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < 3; ++k) {
+                const int32_t vi = od.indices[3 * t + k];
+                if (od.texcoords) {
+                    uvs[2 * k] = od.texcoords[2 * vi];
+                    uvs[2 * k + 1] = od.texcoords[2 * vi + 1];
+                }
+                if (od.tangents) {
+                    f3 tg = mk(od.tangents[3 * vi], od.tangents[3 * vi + 1], od.tangents[3 * vi + 2]);
+                    f3 bn = mk(od.bitangents[3 * vi], od.bitangents[3 * vi + 1], od.bitangents[3 * vi + 2]);
+                    if (M) {
+                        tg = normalize(xf_vector(M, tg));
+                        bn = normalize(xf_vector(M, bn));
+                    }
+                    a[k] = make_float4(tg.x, tg.y, tg.z, 0.0f);
+                    a[3 + k] = make_float4(bn.x, bn.y, bn.z, 0.0f);
+                }
+            }
+            for (int k = 0; k < 6; ++k) a[k].w = uvs[k];
+        }
+    }
+    std::vector<uint2> lobjs;
+    std::vector<uint32_t> lslots;
+    {
         for (uint32_t o : light_objects) {
             const xt_object_desc& od = desc->objects[o];
             if (od.num_triangles == 0) continue;
@@ -410,6 +489,26 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
         XT_HIP_S(dalloc(&s->d_normals, nrms.size() / 4));
         XT_HIP_S(hipMemcpy(s->d_normals, nrms.data(), nrms.size() * 4, hipMemcpyHostToDevice));
     }
+    if (any_nmap) {
+        XT_HIP_S(dalloc(&s->d_nmap_attr, nmap_attr.size()));
+        XT_HIP_S(hipMemcpy(s->d_nmap_attr, nmap_attr.data(), nmap_attr.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    if (desc->num_textures > 0) {
+        std::vector<DevTexture> dt(desc->num_textures);
+        for (uint32_t k = 0; k < desc->num_textures; ++k) {
+            const xt_texture& t = desc->textures[k];
+            std::vector<float4> tex((size_t)t.width * t.height);
+            for (size_t q = 0; q < tex.size(); ++q)
+                tex[q] = make_float4(t.rgb[3 * q], t.rgb[3 * q + 1], t.rgb[3 * q + 2], 0.0f);
+            float4* d = nullptr;
+            XT_HIP_S(dalloc(&d, tex.size()));
+            s->d_texels.push_back(d);
+            XT_HIP_S(hipMemcpy(d, tex.data(), tex.size() * sizeof(float4), hipMemcpyHostToDevice));
+            dt[k] = DevTexture{(int)t.width, (int)t.height, d};
+        }
+        XT_HIP_S(dalloc(&s->d_textures, dt.size()));
+        XT_HIP_S(hipMemcpy(s->d_textures, dt.data(), dt.size() * sizeof(DevTexture), hipMemcpyHostToDevice));
+    }
     XT_HIP_S(dalloc(&s->d_mats, desc->num_materials));
     XT_HIP_S(hipMemcpy(s->d_mats, desc->materials, desc->num_materials * sizeof(xt_material), hipMemcpyHostToDevice));
     XT_HIP_S(dalloc(&s->d_light_pdf, slot_pdf.size()));
@@ -432,6 +531,10 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     ds.nodes = s->d_nodes;
     ds.tris = s->d_tris;
     ds.normals = s->d_normals;
+    ds.nmap_attr = s->d_nmap_attr;
+    s->nmap = any_nmap;
+    for (uint32_t m = 0; m < desc->num_materials; ++m) s->nmap |= desc->materials[m].type == XT_MAT_MICROFACET_NMAP;
+    ds.textures = s->d_textures;
     ds.mats = s->d_mats;
     ds.light_pdf = s->d_light_pdf;
     ds.light_objs = s->d_light_objs;
@@ -863,14 +966,9 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     }
                 }
                 a = tm.mark();
-                if (guide)
-                    hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
-                                       rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
-                                       rb.counts + 2, rb.radiance, rb.pending, gr);
-                else
-                    hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
-                                       rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
-                                       rb.counts + 2, rb.radiance, rb.pending, gr);
+                hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0,
+                                   s->stream, wp, s->ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
+                                   rb.counts + (1 - cur), rb.sq, rb.counts + 2, rb.radiance, rb.pending, gr);
                 XT_LAUNCHED(result);
                 b = tm.mark();
                 tm.add(1, a, b);
@@ -1050,14 +1148,10 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
             gr.lobes2 = l2;
             gr.stride = rb.capacity;
             gr.prob = g.cfg.guide_prob;
-            hipLaunchKernelGGL(k_shade<true>, dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
-                               rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
-                               rb.radiance, rb.pending, gr);
-        } else {
-            hipLaunchKernelGGL(k_shade<false>, dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds, rb.q[cur],
-                               rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq, rb.counts + 2,
-                               rb.radiance, rb.pending, gr);
         }
+        hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
+                           rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
+                           rb.counts + 2, rb.radiance, rb.pending, gr);
         hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.sq, rb.counts + 2,
                            rb.radiance, rb.pending, s->d_ovf, nullptr);
         cur = 1 - cur;

@@ -18,10 +18,17 @@ namespace xt {
 // Triangles, in leaf order ("slots"), 3 float4 = 48 bytes each:
 //   [0] p0.xyz, global prim id   [1] p1.xyz, material index   [2] p2.xyz, flags
 // Optional vertex normals per slot, 3 float4 (n0, n1, n2).
+// Optional normal-map attributes per slot (slots whose material has a normal
+// map), 6 float4: (tangent k.xyz, uv) for k = 0..2 then (bitangent k.xyz, uv):
+//   [0] t0, u0  [1] t1, v0  [2] t2, u1  [3] b0, v1  [4] b1, u2  [5] b2, v2
+// Flags bits 8..31 hold 1 + the normal map's texture index (0 = none).
 constexpr int NODE_F4 = 8;
 constexpr int TRI_F4 = 3;
+constexpr int NMAP_F4 = 6;
 constexpr uint32_t TRI_FLAG_NORMALS = 1u;
 constexpr uint32_t TRI_FLAG_EMISSIVE = 2u;
+constexpr uint32_t TRI_FLAG_TANGENTS = 4u;   // the object supplied tangents and bitangents
+constexpr uint32_t TRI_NMAP_SHIFT = 8u;
 constexpr uint32_t INVALID = 0xFFFFFFFFu;
 
 XT_HD int leaf_code(uint32_t first, uint32_t count) { return ~(int)((first << 3) | (count - 1)); }
@@ -43,10 +50,18 @@ struct DevEnv {
     const float* map_cond;   // lat-long: per-row column CDFs (map_h x (map_w + 1))
 };
 
+// an RGB texture (rgb, pad per texel), row 0 first
+struct DevTexture {
+    int w, h;
+    const float4* texels;
+};
+
 struct DevScene {
     const float4* nodes;
     const float4* tris;
     const float4* normals;      // nullptr when no object has vertex normals
+    const float4* nmap_attr;    // NMAP_F4 per slot; nullptr without normal maps
+    const DevTexture* textures; // normal maps
     const xt_material* mats;
     const float* light_pdf;     // per slot; area pdf of emissive slots (Scene.h:49-55)
     const uint2* light_objs;    // {first, count} into light_slots, one per emissive object
@@ -293,15 +308,16 @@ XT_HD bool env_in_sun(const DevEnv& e, f3 du) {
     f3 c = cross(du, e.sun_dir);
     return dot(du, e.sun_dir) > 0.0f && dot(c, c) <= e.sun_sin2;
 }
-// An image sky as the lookups see it (plain values, so the out-of-line
-// lookup below takes no address of a kernel argument).
-struct EnvMapRef {
+// A texture as the lookups see it (plain values, so the out-of-line lookups
+// take no address of a kernel argument); kind XT_ENV_MAP_LATLONG wraps in x,
+// any other kind clamps.
+struct TexRef {
     int kind, w, h;
     const float4* texels;
 };
 // Texture::rgb(x, y) (Texture.h:76-80) with warpClamp (Texture.h:161-171);
 // the lat-long map wraps in x.
-XT_HD f3 env_texel(const EnvMapRef& m, int x, int y) {
+XT_HD f3 tex_texel(const TexRef& m, int x, int y) {
     if (m.kind == XT_ENV_MAP_LATLONG) {
         x %= m.w;
         if (x < 0) x += m.w;
@@ -313,14 +329,14 @@ XT_HD f3 env_texel(const EnvMapRef& m, int x, int y) {
     return mk(t.x, t.y, t.z);
 }
 // Texture::rgb(uv), bilinear filter (Texture.h:83-103)
-XT_HD f3 env_bilinear(const EnvMapRef& m, float u, float v) {
+XT_HD f3 tex_bilinear(const TexRef& m, float u, float v) {
     const int x0 = (int)floorf(u * m.w - 0.5f);
     const int y0 = (int)floorf(v * m.h - 0.5f);
     const float wx1 = (u * m.w - 0.5f) - x0;
     const float wy1 = (v * m.h - 0.5f) - y0;
     const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
-    return wx0 * wy0 * env_texel(m, x0, y0) + wx0 * wy1 * env_texel(m, x0, y0 + 1) + wx1 * wy0 * env_texel(m, x0 + 1, y0) +
-           wx1 * wy1 * env_texel(m, x0 + 1, y0 + 1);
+    return wx0 * wy0 * tex_texel(m, x0, y0) + wx0 * wy1 * tex_texel(m, x0, y0 + 1) + wx1 * wy0 * tex_texel(m, x0 + 1, y0) +
+           wx1 * wy1 * tex_texel(m, x0 + 1, y0 + 1);
 }
 // lat-long coordinates of a unit direction (y up, row 0 at the zenith)
 XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
@@ -332,17 +348,17 @@ XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
 // Kept out of line: inlined, the lookup's registers lower the occupancy of
 // every kernel that evaluates the sky.
 static __host__ __device__ __noinline__ f3 env_sky_map(int kind, int w, int h, const float4* texels, f3 wi) {
-    const EnvMapRef m{kind, w, h, texels};
+    const TexRef m{kind, w, h, texels};
     if (kind == XT_ENV_MAP_ANGULAR) {
         const f3 d = normalize(-wi);
         const float sq = sqrtf(d.x * d.x + d.y * d.y);
         if (sq == 0.0f) return splat(0.0f);
         const float r = (float)((1.0 / 3.14159265358979323846) * (double)acosf(d.z) / (double)sq);
-        return env_bilinear(m, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
+        return tex_bilinear(m, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
     }
     float u, v;
     env_latlong_uv(normalize(wi), &u, &v);
-    return env_bilinear(m, u, v);
+    return tex_bilinear(m, u, v);
 }
 // the image sky, or SkySphereUniform::getRadiance (SkySphere.h:40-42)
 XT_HD f3 env_sky(const DevEnv& e, f3 wi) {
@@ -432,9 +448,30 @@ XT_HD f3 env_nee_sample(const DevEnv& e, const EnvNee& s, const Frame& fr, float
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
+// Object.h:59-68: shading.n = normalize(n.b * shading.n + n.r * tangent +
+// n.g * bitangent) with n = rgb(texCoord) * 2 - 1, from the slot's
+// interpolated texCoord and normalised tangent frame (TriangleIndexed.h:
+// 139-168); without tangents the frame is zero.  Out of line: only
+// normal-mapped hits run it.
+static __device__ __noinline__ f3 normal_mapped(const float4* attr, bool tangents, int w, int h, const float4* texels,
+                                                float b0, float b1, f3 sn) {
+    const float4 a0 = attr[0], a1 = attr[1], a2 = attr[2], a3 = attr[3], a4 = attr[4], a5 = attr[5];
+    const f3 uv = lerp3(mk(a0.w, a1.w, 0.0f), mk(a2.w, a3.w, 0.0f), mk(a4.w, a5.w, 0.0f), b0, b1);
+    f3 tg = splat(0.0f), bt = splat(0.0f);
+    if (tangents) {
+        tg = normalize(lerp3(xyz(a0), xyz(a1), xyz(a2), b0, b1));
+        bt = normalize(lerp3(xyz(a3), xyz(a4), xyz(a5), b0, b1));
+    }
+    const TexRef m{XT_ENV_MAP_NONE, w, h, texels};
+    const f3 c = tex_bilinear(m, uv.x, uv.y) * 2.0f - splat(1.0f);
+    return normalize(c.z * sn + c.x * tg + c.y * bt);
+}
+
 // SurfaceIntersection at a hit (TriangleIndexed.h:133-172, then the identity
 // pass of Object.h:106-144 where Transform::asNormal normalises n, wo and
 // shading.n once more).
+// NMAP: the scene has normal maps (the lookup is compiled in only then).
+template <bool NMAP = true>
 __device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 d, float b0, float b1) {
     const float4* tr = sc.tris + (size_t)slot * TRI_F4;
     float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
@@ -453,6 +490,12 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, uint32_t slot, f3 
         n = normalize(cross(p1 - p0, p2 - p0));
     }
     f3 sn = face_forward(n, wo);
+    const uint32_t nm = flags >> TRI_NMAP_SHIFT;
+    if (NMAP && nm != 0) {
+        const DevTexture t = sc.textures[nm - 1];
+        sn = normal_mapped(sc.nmap_attr + (size_t)slot * NMAP_F4, (flags & TRI_FLAG_TANGENTS) != 0, t.w, t.h, t.texels,
+                           b0, b1, sn);
+    }
     s.n = normalize(n);
     s.wo = normalize(wo);
     s.sn = normalize(sn);

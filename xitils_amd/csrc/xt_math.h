@@ -53,7 +53,7 @@ XT_HD f3 face_forward(f3 n, f3 v) { return dot(n, v) < 0.0f ? -n : n; }
 XT_HD f3 lerp3(f3 a, f3 b, f3 c, float t1, float t2) { return t1 * a + t2 * b + (1.0f - t1 - t2) * c; }
 XT_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 XT_HD int max_dim(f3 a) { return (a.x > a.y && a.x > a.z) ? 0 : ((a.y > a.z) ? 1 : 2); }
-XT_HD bool finite3(f3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+XT_HD bool finite3(f3 a) { return __builtin_isfinite(a.x) && __builtin_isfinite(a.y) && __builtin_isfinite(a.z); }
 
 // Orthonormal frame around v (BasisVectors, Vector.h:781-795).
 struct Frame {

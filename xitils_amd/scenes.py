@@ -17,6 +17,11 @@ Material parameter layout (7 floats, shared with include/xitils_amd.h):
   GGX_LAMBERT   albedo r, g, b, alpha, f0, p_spec       (new; lobe mixture as
                                                           GlossyWithHighLight,
                                                           Material.h:134-175)
+  MICROFACET_NMAP  base type, base p[0..5]              (MicrofacetNormalMapping,
+                                                          Sandbox/MicrofacetBasedNormal
+                                                          Mapping/Main.cpp:86-340)
+A material's normal map (Material::normalmap, applied as Object.h:59-68) is an
+index into SceneDesc.textures.
 """
 from __future__ import annotations
 
@@ -26,7 +31,8 @@ from typing import List, Optional
 
 import numpy as np
 
-MAT_DIFFUSE, MAT_SPEC_REFL, MAT_SPEC_REFR, MAT_SPEC_FRESNEL, MAT_METAL, MAT_EMISSION, MAT_GGX_LAMBERT = range(7)
+(MAT_DIFFUSE, MAT_SPEC_REFL, MAT_SPEC_REFR, MAT_SPEC_FRESNEL, MAT_METAL, MAT_EMISSION, MAT_GGX_LAMBERT,
+ MAT_MICROFACET_NMAP) = range(8)
 
 FLAG_UNIFORM_BARY = 1
 FLAG_ENV_NEE = 2
@@ -39,6 +45,7 @@ INTEGRATOR_NAIVE = 1
 class Material:
     type: int
     params: tuple = (0.0,) * 7
+    normal_map: int = -1     # index into SceneDesc.textures (Material::normalmap), -1 = none
 
     def p7(self) -> np.ndarray:
         p = np.zeros(7, np.float32)
@@ -66,6 +73,14 @@ def ggx_lambert(albedo, alpha, f0, p_spec=0.5):
     return Material(MAT_GGX_LAMBERT, (albedo[0], albedo[1], albedo[2], alpha, f0, p_spec))
 
 
+def microfacet_nmap(base: Material, normal_map: int = -1):
+    """MicrofacetNormalMapping(base) (Main.cpp:86-92); base is DIFFUSE, METAL
+    or GGX_LAMBERT."""
+    if base.type not in (MAT_DIFFUSE, MAT_METAL, MAT_GGX_LAMBERT):
+        raise ValueError("microfacet normal mapping needs a Diffuse, Metal or GGX+Lambert base")
+    return Material(MAT_MICROFACET_NMAP, (float(base.type),) + tuple(base.p7()[:6]), normal_map)
+
+
 @dataclass
 class ObjectDesc:
     positions: np.ndarray                 # (nv, 3) float32, object space
@@ -75,6 +90,9 @@ class ObjectDesc:
     xf: Optional[np.ndarray] = None       # (4, 4) float32 Transform::mat
     xf_inv: Optional[np.ndarray] = None   # (4, 4) float32 Transform::matInv
     tri_materials: Optional[np.ndarray] = None  # (nt,) uint32
+    texcoords: Optional[np.ndarray] = None      # (nv, 2) float32
+    tangents: Optional[np.ndarray] = None       # (nv, 3) float32
+    bitangents: Optional[np.ndarray] = None     # (nv, 3) float32
 
     @property
     def num_triangles(self) -> int:
@@ -130,6 +148,7 @@ class SceneDesc:
     env: Env = field(default_factory=Env)
     camera: Optional[Camera] = None
     name: str = ""
+    textures: List[np.ndarray] = field(default_factory=list)   # (H, W, 3) float32, row 0 first
 
     @property
     def num_triangles(self) -> int:
@@ -326,6 +345,14 @@ def _ggx_lambert_table(alb, alpha, f0, p_spec=0.5):
     packed[:, 5] = f0
     packed[:, 6] = p_spec
     return MaterialTable(packed)
+
+
+def material_normal_maps(mats):
+    """(n,) uint32 of 1 + texture index per material (0 = none), or None."""
+    if isinstance(mats, MaterialTable):
+        return None
+    nm = np.array([getattr(m, "normal_map", -1) + 1 for m in mats], np.uint32)
+    return nm if nm.any() else None
 
 
 def pack_materials(mats) -> np.ndarray:
@@ -604,4 +631,107 @@ def occluded_room(width=1280, height=720, n_tris=500_000, seed=4, light_radiance
         k += 1
     cam = look_at((0.6, 2.2, 0.4), (6.0, 0.8, 2.2))
     s.camera = Camera(cam, math.radians(70.0), height / width)
+    return s
+
+
+# --------------------------------------------------------------------------- displaced plane (normal mapping)
+
+def _sines(n, f_lo, f_hi, rng):
+    """n plane waves: frequency (cycles per unit uv), direction angle, phase."""
+    f = rng.uniform(f_lo, f_hi, n)
+    ang = rng.uniform(0.0, 2 * np.pi, n)
+    ph = rng.uniform(0.0, 2 * np.pi, n)
+    return f, ang, ph
+
+
+def _sine_height_grad(u, v, f, ang, ph):
+    """h = sum_k sin(2 pi f_k (cos a_k u + sin a_k v) + p_k) / f_k and its
+    gradient (float64)."""
+    h = np.zeros(np.broadcast(u, v).shape)
+    hu = np.zeros_like(h)
+    hv = np.zeros_like(h)
+    for fk, ak, pk in zip(f, ang, ph):
+        arg = 2 * np.pi * fk * (np.cos(ak) * u + np.sin(ak) * v) + pk
+        h += np.sin(arg) / fk
+        c = np.cos(arg) * 2 * np.pi
+        hu += c * np.cos(ak)
+        hv += c * np.sin(ak)
+    return h, hu, hv
+
+
+def sine_normal_map(size=1024, n_sines=32, f_lo=4.0, f_hi=64.0, seed=5, bump=0.03):
+    """Tangent-space normal map of the height field sum_k sin(...)/f_k over
+    uv in [0, 1]^2 (32 sines, frequency U[4, 64], amplitude 1/f): texel
+    (x, y) at uv = ((x + 0.5) / size, (y + 0.5) / size) stores
+    (n + 1) / 2 with n = normalize(-bump dh/du, -bump dh/dv, 1) in
+    (tangent, bitangent, normal) order, the layout Object.h:59-68 decodes."""
+    rng = np.random.default_rng(seed)
+    f, ang, ph = _sines(n_sines, f_lo, f_hi, rng)
+    t = (np.arange(size) + 0.5) / size
+    U, V = np.meshgrid(t, t)            # row y <-> v
+    _, hu, hv = _sine_height_grad(U, V, f, ang, ph)
+    n = np.stack([-bump * hu, -bump * hv, np.ones_like(hu)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return ((n + 1) / 2).astype(np.float32)
+
+
+def displaced_plane_mesh(grid=4096, extent=4.0, height=0.08, n_sines=8, f_lo=1.0, f_hi=4.0, seed=5):
+    """A grid x grid vertex plane over [-extent/2, extent/2]^2 in xz,
+    displaced along y by `height` times a low-frequency sine sum (the
+    geometric displacement), with analytic vertex normals, texCoords (u along
+    x, v along z) and the tangent frame dP/du, dP/dv (TriangleMesh.h:58-105
+    builds the same attributes for a displaced mesh).  Returns (positions,
+    indices, normals, texcoords, tangents, bitangents)."""
+    rng = np.random.default_rng(seed + 1000)
+    f, ang, ph = _sines(n_sines, f_lo, f_hi, rng)
+    t = np.linspace(0.0, 1.0, grid)
+    U, V = np.meshgrid(t, t)            # row j <-> v
+    h, hu, hv = _sine_height_grad(U, V, f, ang, ph)
+    P = np.stack([(U - 0.5) * extent, height * h, (V - 0.5) * extent], -1)
+    Pu = np.stack([np.full_like(U, extent), height * hu, np.zeros_like(U)], -1)
+    Pv = np.stack([np.zeros_like(U), height * hv, np.full_like(U, extent)], -1)
+    N = np.cross(Pv, Pu)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    T = Pu / np.linalg.norm(Pu, axis=-1, keepdims=True)
+    B = Pv / np.linalg.norm(Pv, axis=-1, keepdims=True)
+    del h, hu, hv, Pu, Pv
+    j, i = np.meshgrid(np.arange(grid - 1, dtype=np.int64), np.arange(grid - 1, dtype=np.int64), indexing="ij")
+    v00 = (j * grid + i).reshape(-1)
+    v10, v01, v11 = v00 + 1, v00 + grid, v00 + grid + 1
+    # wound so the face normal points +y, like the vertex normals
+    idx = np.empty((2 * v00.size, 3), np.int32)
+    idx[0::2] = np.stack([v00, v01, v10], -1)
+    idx[1::2] = np.stack([v10, v01, v11], -1)
+    f32 = lambda a: np.ascontiguousarray(a.reshape(-1, a.shape[-1]), np.float32)
+    return (f32(P), idx, f32(N), f32(np.stack([U, V], -1)), f32(T), f32(B))
+
+
+def displaced_plane(grid=4096, width=3840, height=2160, alpha=0.2, f0=(0.95, 0.9, 0.85), base="metal",
+                    microfacet=True, normal_map_size=1024, sky_size=(2048, 1024), variant=0, seed=5):
+    """The normal-mapping workload: a displaced plane (grid^2 vertices) whose
+    material is MicrofacetNormalMapping over GGX (Metal, alpha) with the
+    32-sine normal map, under a procedural lat-long HDR sky with a sun of
+    radiance 2e4.  microfacet=False keeps the base material with the same
+    normal map (the sandbox's naive mode, Main.cpp:411-413).  `variant`
+    picks one of 16 camera placements around the plane."""
+    s = SceneDesc(name=f"displaced_plane_{grid}")
+    pos, idx, nrm, uv, tan, bit = displaced_plane_mesh(grid, seed=seed)
+    s.textures.append(sine_normal_map(normal_map_size, seed=seed))
+    if base == "metal":
+        b = Material(MAT_METAL, (float(alpha), f0[0], f0[1], f0[2]))
+    elif base == "diffuse":
+        b = diffuse(*f0)
+    else:
+        b = ggx_lambert(f0, float(alpha), 0.04)
+    m = microfacet_nmap(b, normal_map=0) if microfacet else Material(b.type, b.params, normal_map=0)
+    mat = s.add_material(m)
+    s.objects.append(ObjectDesc(pos, idx, material=mat, normals=nrm, texcoords=uv, tangents=tan, bitangents=bit))
+    sw, sh = sky_size
+    sky = procedural_sky_latlong(sw, sh, sun_dir=(0.3, 0.5, 0.8), sun_half_angle_deg=0.5, sun_radiance=2e4, seed=seed)
+    s.env = Env(True, (0.0, 0.0, 0.0), map_kind=ENV_MAP_LATLONG, map=sky)
+    az = 2 * np.pi * (variant % 8) / 8
+    el = 0.55 if variant < 8 else 0.9
+    r = 2.6
+    eye = (r * np.sin(az) * np.cos(el), r * np.sin(el), -r * np.cos(az) * np.cos(el))
+    s.camera = Camera(look_at(eye, (0.0, 0.0, 0.0)), math.radians(45.0), height / width)
     return s
