@@ -47,9 +47,15 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch"], default="heightfield",
+    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch", "nmap"], default="heightfield",
                     help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room; "
-                         "batch: 64 small scenes rendered one by one and as one graph-replayed batch")
+                         "batch: 64 small scenes rendered one by one and as one graph-replayed batch; "
+                         "nmap: the displaced plane with microfacet-based normal mapping, 16 camera variants")
+    ap.add_argument("--plane-grid", type=int, default=4096, help="nmap: vertices per side of the displaced plane")
+    ap.add_argument("--plane-width", type=int, default=3840)
+    ap.add_argument("--plane-height", type=int, default=2160)
+    ap.add_argument("--plane-spp", type=int, default=1, help="nmap: samples per pixel per variant per step")
+    ap.add_argument("--variants", type=int, default=16)
     ap.add_argument("--scenes", type=int, default=64, help="batch: number of scenes")
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each scene's launch shapes first")
     ap.add_argument("--batch-side", type=int, default=0, help="batch: override every scene's image side")
@@ -240,6 +246,88 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+def nmap_main(args):
+    """Mpaths/s of the displaced-plane workload: a plane of plane_grid^2
+    vertices (2 (grid - 1)^2 triangles) displaced by a low-frequency sine sum,
+    shaded with MicrofacetNormalMapping over GGX (alpha 0.2) and the 32-sine
+    normal map, lit by a 2048x1024 lat-long HDR sky with a 2e4 sun sampled
+    as a light; one step renders every camera variant at plane_width x
+    plane_height, plane_spp spp, depth 12."""
+    import numpy as np
+    import torch
+    from xitils_amd import api, scenes
+    W, H, spp, depth, nv = args.plane_width, args.plane_height, args.plane_spp, 12, args.variants
+    flags = scenes.FLAG_ENV_NEE
+    t0 = time.perf_counter()
+    desc = scenes.displaced_plane(grid=args.plane_grid, width=W, height=H)
+    gen_s = time.perf_counter() - t0
+    log(f"[bench] displaced plane {args.plane_grid}^2: {desc.num_triangles} triangles, generated in {gen_s:.1f} s")
+    dev = api.Device(0)
+    t0 = time.perf_counter()
+    scene = api.Scene(dev, desc)
+    upload_s = time.perf_counter() - t0
+    info = scene.info()
+    log(f"[bench] scene {info}, build+upload {upload_s:.1f} s")
+    cams = [scenes.displaced_plane_camera(v, W, H) for v in range(nv)]
+    films = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0") for _ in range(nv)]
+    kw = dict(seed=1, max_depth=depth, flags=flags)
+    for k in range(args.warmup):
+        scene.render_device(films[0].data_ptr(), W, H, spp, camera=cams[0], **kw)
+        log(f"[bench] warmup {k} done")
+    for f in films:
+        f.zero_()
+    agg = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "extension_rays": 0,
+           "shadow_rays": 0, "ms_total": 0.0}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for v in range(nv):
+            scene.render_device(films[v].data_ptr(), W, H, spp, camera=cams[v], spp_offset=k * spp, **kw)
+            st = scene.stats()
+            for key in agg:
+                agg[key] += st[key]
+        log(f"[bench] step {k} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    paths = W * H * spp * nv * args.steps
+    means = [float(f.mean()) / (spp * args.steps) for f in films]
+    finite = all(bool(torch.isfinite(f).all()) for f in films)
+    stages = {k: round(agg[k] / args.steps, 2) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")}
+    out = {
+        "metric": "Mpaths/s (displaced plane, microfacet normal mapping)",
+        "value": round(paths / wall / 1e6, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"displaced_plane{args.plane_grid}_{W}x{H}_{spp}spp_x{nv}variants_depth{depth}_latlong2048",
+                   "triangles": info["num_prims"], "width": W, "height": H, "spp": spp, "variants": nv,
+                   "max_depth": depth, "material": "MicrofacetNormalMapping over GGX alpha 0.2, 1024^2 32-sine normal map",
+                   "parallelism": "1 GPU"},
+        "mrays_per_s": round((agg["extension_rays"] + agg["shadow_rays"]) / wall / 1e6, 1),
+        "rays_per_path": round((agg["extension_rays"] + agg["shadow_rays"]) / paths, 3),
+        "stage_ms_per_step": stages,
+        "film_means": [round(m, 4) for m in means[:4]],
+        "finite": finite,
+        "build": {"scene_gen_s": round(gen_s, 1), "bvh_build_upload_s": round(upload_s, 1),
+                  "bvh_nodes": info["bvh_nodes"], "bvh_depth": info["bvh_depth"]},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        cdesc = desc
+        cdesc.camera = cams[0]
+        cargs = argparse.Namespace(**vars(args))
+        cargs.width, cargs.height, cargs.spp = W, H, spp
+        out["cpu_baseline"] = cpu_baseline(cdesc, cargs, depth, flags)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -250,6 +338,8 @@ def main():
         return guiding_main(args)
     if args.workload == "batch":
         return batch_main(args)
+    if args.workload == "nmap":
+        return nmap_main(args)
     import numpy as np
     import torch
     from xitils_amd import api, scenes

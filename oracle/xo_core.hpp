@@ -1303,7 +1303,9 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
                     V3 Le = sc.env.radiance(cur.d);
                     if (envn.on && pdfB >= 0.0f) {
                         float pe = envn.pSelect * envn.pdf(normalize(cur.d), isect.sn);
-                        float mis = (pdfB * pdfB) / (pdfB * pdfB + pe * pe);
+                        // power heuristic; a direction the sky sampler cannot produce
+                        // (pe = 0) or an unbounded BSDF pdf keeps the full weight
+                        float mis = (pe > 0.0f && pdfB < Infinity) ? (pdfB * pdfB) / (pdfB * pdfB + pe * pe) : 1.0f;
                         radiance += weight * mis * meval * Le;
                     } else {
                         radiance += weight * meval * Le;

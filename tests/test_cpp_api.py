@@ -13,20 +13,21 @@ from xitils_amd import api, scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "cpp", "_build", "cornell_api")
+NMAP_EXE = os.path.join(ROOT, "tests", "cpp", "_build", "nmap_api")
 
 
-def build_example():
-    os.makedirs(os.path.dirname(EXE), exist_ok=True)
+def build_example(exe=EXE):
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
     libdir = os.path.dirname(api.LIB_PATH)
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "cpp", "cornell_api.cpp"), "-o", EXE,
+                    os.path.join(ROOT, "tests", "cpp", os.path.basename(exe) + ".cpp"), "-o", exe,
                     "-L", libdir, "-lxitils_amd", f"-Wl,-rpath,{libdir}"], check=True)
-    return EXE
+    return exe
 
 
 def test_cpp_api_compiles_and_links():
-    exe = build_example()
-    assert os.path.exists(exe)
+    for exe in (EXE, NMAP_EXE):
+        assert os.path.exists(build_example(exe))
 
 
 @pytest.mark.gpu
@@ -44,4 +45,40 @@ def test_cpp_api_render_matches_python_binding():
     # the two callers derive the camera's tan(fov/2) through different float
     # paths (std::tan on 40 * ToRad vs numpy on radians(40)), so rays can
     # differ in the last ulp: same image up to sampling-noise-free rounding
+    assert rel_mse(fc / 4, fp / 4) < 1e-6
+
+
+def _write_plane(d, desc):
+    o = desc.objects[0]
+    with open(os.path.join(d, "mesh.bin"), "wb") as f:
+        np.array([o.positions.shape[0], o.indices.size], np.int32).tofile(f)
+        for a in (o.positions, o.normals, o.texcoords, o.tangents, o.bitangents):
+            np.ascontiguousarray(a, np.float32).tofile(f)
+        np.ascontiguousarray(o.indices, np.int32).tofile(f)
+    for name, img in (("tex.bin", desc.textures[0]), ("sky.bin", desc.env.map)):
+        with open(os.path.join(d, name), "wb") as f:
+            np.array([img.shape[1], img.shape[0]], np.int32).tofile(f)
+            np.ascontiguousarray(img, np.float32).tofile(f)
+    cam = desc.camera
+    np.concatenate([cam.cam_to_world.astype(np.float32).reshape(16),
+                    np.array([cam.fov, cam.aspect], np.float32)]).tofile(os.path.join(d, "cam.bin"))
+
+
+@pytest.mark.gpu
+def test_cpp_api_microfacet_normal_mapping_matches_python_binding():
+    """MicrofacetNormalMapping + Material::normalmap + setGeometry with
+    texCoords and tangents, through the C++ API, give the Python binding's
+    image of the same arrays."""
+    exe = NMAP_EXE if os.path.exists(NMAP_EXE) else build_example(NMAP_EXE)
+    desc = scenes.displaced_plane(grid=48, width=48, height=32, normal_map_size=128, sky_size=(128, 64), variant=2)
+    with tempfile.TemporaryDirectory() as d:
+        _write_plane(d, desc)
+        out = os.path.join(d, "f.f32")
+        print("[gpu] C++ API normal-mapped plane render", flush=True)
+        r = subprocess.run([exe, out, "48", "32", "4", d], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        fc = np.fromfile(out, np.float32).reshape(32, 48, 3)
+    dev = api.Device(0)
+    fp = api.Scene(dev, desc).render(48, 32, 4, seed=1, max_depth=12, flags=scenes.FLAG_ENV_NEE)
+    assert np.isfinite(fc).all()
     assert rel_mse(fc / 4, fp / 4) < 1e-6

@@ -240,7 +240,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     if (sc.env.has_sky) {                                                  // PathTracer.h:192-195
                         f3 Le = env_radiance(sc.env, d);
                         if (envn.on && pdf_prev >= 0.0f) {
-                            float mis = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe_prev * pe_prev);
+                            // power heuristic; a direction the sky sampler cannot produce
+                            // (pe = 0) or an unbounded BSDF pdf keeps the full weight
+                            const float mis = (pe_prev > 0.0f && pdf_prev < INFINITY)
+                                                  ? (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe_prev * pe_prev)
+                                                  : 1.0f;
                             rad += weight * mis * meval * Le;
                         } else {
                             rad += weight * meval * Le;

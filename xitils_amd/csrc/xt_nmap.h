@@ -31,9 +31,12 @@ XT_HD uint4 walk_block(const WalkKey& k, int block) {
     return philox(make_uint4(k.pixel, k.sample, k.slot, (uint32_t)block), k.key);
 }
 
+// BASE: the base type as a compile-time constant (the walk is instantiated
+// per base, so each instance carries only its base's code and registers).
+template <uint32_t BASE>
 XT_HD xt_material nmap_base(const xt_material& m) {
     xt_material b;
-    b.type = (uint32_t)m.p[0];
+    b.type = BASE;
     for (int k = 0; k < 6; ++k) b.p[k] = m.p[k + 1];
     b.p[6] = 0.0f;
     return b;
@@ -130,8 +133,9 @@ struct NmapSample {
 };
 
 // bsdfCos (Main.cpp:94-170) and getPDF of one direction.
-static __host__ __device__ __noinline__ NmapEval nmap_eval(xt_material m, Surf s, f3 wi, WalkKey wk, int base_dim) {
-    const xt_material base = nmap_base(m);
+template <uint32_t BASE>
+static __host__ __device__ __noinline__ NmapEval nmap_eval_t(xt_material m, Surf s, f3 wi, WalkKey wk, int base_dim) {
+    const xt_material base = nmap_base<BASE>(m);
     f3 wg, wp, wt;
     const int mode = nmap_frame(s, &wg, &wp, &wt);
     NmapEval r;
@@ -168,8 +172,9 @@ static __host__ __device__ __noinline__ NmapEval nmap_eval(xt_material m, Surf s
 // evalAndSample (Main.cpp:172-241): the walk's throughput, exit direction
 // and getPDF of it (-1 when every step sampled a delta lobe).  A NaN tangent
 // facet leaves the reference's pdf unset; here it is 0.
-static __host__ __device__ __noinline__ NmapSample nmap_sample(xt_material m, Surf s, WalkKey wk, int base_dim) {
-    const xt_material base = nmap_base(m);
+template <uint32_t BASE>
+static __host__ __device__ __noinline__ NmapSample nmap_sample_t(xt_material m, Surf s, WalkKey wk, int base_dim) {
+    const xt_material base = nmap_base<BASE>(m);
     f3 wg, wp, wt;
     const int mode = nmap_frame(s, &wg, &wp, &wt);
     const int b0 = base_dim >> 2;
@@ -209,6 +214,19 @@ static __host__ __device__ __noinline__ NmapSample nmap_sample(xt_material m, Su
     r.wi = wr;
     r.pdf = specular ? -1.0f : nmap_pdf_frame(base, s, wr, wg, wp, wt);
     return r;
+}
+
+static __host__ __device__ __noinline__ NmapEval nmap_eval(xt_material m, Surf s, f3 wi, WalkKey wk, int base_dim) {
+    const uint32_t b = (uint32_t)m.p[0];
+    if (b == XT_MAT_METAL) return nmap_eval_t<XT_MAT_METAL>(m, s, wi, wk, base_dim);
+    if (b == XT_MAT_DIFFUSE) return nmap_eval_t<XT_MAT_DIFFUSE>(m, s, wi, wk, base_dim);
+    return nmap_eval_t<XT_MAT_GGX_LAMBERT>(m, s, wi, wk, base_dim);
+}
+static __host__ __device__ __noinline__ NmapSample nmap_sample(xt_material m, Surf s, WalkKey wk, int base_dim) {
+    const uint32_t b = (uint32_t)m.p[0];
+    if (b == XT_MAT_METAL) return nmap_sample_t<XT_MAT_METAL>(m, s, wk, base_dim);
+    if (b == XT_MAT_DIFFUSE) return nmap_sample_t<XT_MAT_DIFFUSE>(m, s, wk, base_dim);
+    return nmap_sample_t<XT_MAT_GGX_LAMBERT>(m, s, wk, base_dim);
 }
 
 // Material dispatch at the shading call sites: the stochastic material reads

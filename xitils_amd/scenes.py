@@ -729,9 +729,15 @@ def displaced_plane(grid=4096, width=3840, height=2160, alpha=0.2, f0=(0.95, 0.9
     sw, sh = sky_size
     sky = procedural_sky_latlong(sw, sh, sun_dir=(0.3, 0.5, 0.8), sun_half_angle_deg=0.5, sun_radiance=2e4, seed=seed)
     s.env = Env(True, (0.0, 0.0, 0.0), map_kind=ENV_MAP_LATLONG, map=sky)
+    s.camera = displaced_plane_camera(variant, width, height)
+    return s
+
+
+def displaced_plane_camera(variant, width, height):
+    """Camera `variant` (0..15) of the displaced-plane workload: eight
+    azimuths at two elevations, looking at the plane's centre."""
     az = 2 * np.pi * (variant % 8) / 8
     el = 0.55 if variant < 8 else 0.9
     r = 2.6
     eye = (r * np.sin(az) * np.cos(el), r * np.sin(el), -r * np.cos(az) * np.cos(el))
-    s.camera = Camera(look_at(eye, (0.0, 0.0, 0.0)), math.radians(45.0), height / width)
-    return s
+    return Camera(look_at(eye, (0.0, 0.0, 0.0)), math.radians(45.0), height / width)
