@@ -29,6 +29,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 NODE_BYTES, TRI_BYTES = 128, 48
 EXT_RAY_IO_BYTES = 32 + 16     # ray in (o, d) + hit out
 SHADOW_RAY_IO_BYTES = 48 + 16  # shadow record in + radiance update
+SHADE_NMAP_BYTES_PER_PATH = 64 + 16 + 48 + 48 + 96 + 32 + 64 + 64 + 48 + 32
 
 
 def log(*a):
@@ -68,11 +69,11 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(kernel_tag):
+def traffic_from_profiles(kernel_tag, name=None):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), scaled by the copy-kernel calibration, if a summary for this
     workload is committed under profiles/."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.txt")
+    path = os.path.join(ROOT, "profiles", name or f"{kernel_tag}_pmc.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -156,8 +157,23 @@ def guiding_main(args):
         "plain": res["plain"],
         "guided": res["guided"],
         "reference_seconds": round(ref_s, 2),
-        "curves": res["curves"],
     }
+    # the network's arithmetic: 33,792 FLOP per query (2 x (96x64 + 64x64 + 64x64 + 64x40)); a training
+    # sample adds the weight gradients (33,792) and the activation gradients of layers 1-3
+    # (2 x (64x40 + 64x64 + 64x64) = 21,504): 89,088 FLOP.  Peak: 2.5 PFLOP/s dense bf16 MFMA.
+    g = res["guided"]
+    mlp = {"peak_tflops_bf16": 2500.0}
+    if g.get("guide_queries") and g.get("ms_guide_infer"):
+        tf = g["guide_queries"] * 33792 / (g["ms_guide_infer"] * 1e-3) / 1e12
+        mlp["infer"] = {"queries": g["guide_queries"], "flop_per_query": 33792, "tflops": round(tf, 2),
+                        "frac": round(tf / 2500.0, 4)}
+    if g.get("train_samples") and g.get("ms_guide_train"):
+        n = min(g["train_samples"], g["adam_steps"] * 65536)   # samples through the gradient kernel
+        tf = n * 89088 / (g["ms_guide_train"] * 1e-3) / 1e12
+        mlp["train"] = {"minibatch_samples": n, "flop_per_sample": 89088, "tflops": round(tf, 2),
+                        "frac": round(tf / 2500.0, 4)}
+    out["mlp"] = mlp
+    out["curves"] = res["curves"]
     print(json.dumps(out), flush=True)
 
 
@@ -270,6 +286,13 @@ def nmap_main(args):
     log(f"[bench] scene {info}, build+upload {upload_s:.1f} s")
     cams = [scenes.displaced_plane_camera(v, W, H) for v in range(nv)]
     films = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0") for _ in range(nv)]
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB read + 1 GiB written
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy done (1 GiB in, 1 GiB out)")
     kw = dict(seed=1, max_depth=depth, flags=flags)
     for k in range(args.warmup):
         scene.render_device(films[0].data_ptr(), W, H, spp, camera=cams[0], **kw)
@@ -277,7 +300,7 @@ def nmap_main(args):
     for f in films:
         f.zero_()
     agg = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "extension_rays": 0,
-           "shadow_rays": 0, "ms_total": 0.0}
+           "shadow_rays": 0, "ms_total": 0.0, "extend_launches": 0}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -293,6 +316,24 @@ def nmap_main(args):
     means = [float(f.mean()) / (spp * args.steps) for f in films]
     finite = all(bool(torch.isfinite(f).all()) for f in films)
     stages = {k: round(agg[k] / args.steps, 2) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")}
+    # roofline of the dominant kernel, the shade step with the normal-map lookup and the random walk:
+    # per queued path 64 B queue in + 16 hit + 48 triangle + 48 normals + 96 normal-map attributes +
+    # 32 material + 64 (four bilinear texels) + 64 queue out + 48 shadow record + 32 radiance/pending
+    shade_bytes = agg["extension_rays"] * SHADE_NMAP_BYTES_PER_PATH
+    achieved = shade_bytes / (agg["ms_shade"] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(None, "shade_displaced_plane4096_nmap.txt")
+        if (args.plane_grid, W, H) == (4096, 3840, 2160) else None,
+        "kernel": "k_shade<false, true> (shade + normal map + two-facet random walk)",
+        "bytes_per_path": SHADE_NMAP_BYTES_PER_PATH,
+        "avg_launch_ms": round(agg["ms_shade"] / max(1, agg["extend_launches"]), 4),
+        "note": "VALU-issue bound (56% of wave-instruction issue, profiles/shade_displaced_plane4096_nmap.txt)",
+    }
     out = {
         "metric": "Mpaths/s (displaced plane, microfacet normal mapping)",
         "value": round(paths / wall / 1e6, 3),
@@ -313,6 +354,7 @@ def nmap_main(args):
         "mrays_per_s": round((agg["extension_rays"] + agg["shadow_rays"]) / wall / 1e6, 1),
         "rays_per_path": round((agg["extension_rays"] + agg["shadow_rays"]) / paths, 3),
         "stage_ms_per_step": stages,
+        "roofline": roofline,
         "film_means": [round(m, 4) for m in means[:4]],
         "finite": finite,
         "build": {"scene_gen_s": round(gen_s, 1), "bvh_build_upload_s": round(upload_s, 1),

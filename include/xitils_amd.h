@@ -180,10 +180,10 @@ typedef struct xt_render_stats {
     /* BVH node (128 B) and triangle (48 B) fetches of the traversal kernels;
      * filled only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
-    /* guiding: network inference and training kernel time, Adam steps taken
-     * and training samples kept in the call */
+    /* guiding: network inference and training kernel time, Adam steps taken,
+     * training samples kept and queue entries run through the network */
     double ms_guide_infer, ms_guide_train;
-    uint64_t guide_steps, guide_samples;
+    uint64_t guide_steps, guide_samples, guide_queries;
 } xt_render_stats;
 
 typedef struct xt_ray {        /* 32 bytes */

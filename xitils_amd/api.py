@@ -87,7 +87,7 @@ class xt_render_stats(C.Structure):
                 ("ext_node_fetches", C.c_uint64), ("ext_tri_fetches", C.c_uint64),
                 ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64),
                 ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double), ("guide_steps", C.c_uint64),
-                ("guide_samples", C.c_uint64)]
+                ("guide_samples", C.c_uint64), ("guide_queries", C.c_uint64)]
 
 
 class xt_batch_item(C.Structure):

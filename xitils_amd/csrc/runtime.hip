@@ -955,6 +955,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                                        rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity);
                     XT_LAUNCHED(result);
                     tm.add(4, a, tm.mark());
+                    s->stats.guide_queries += live;
                     gr.lobes4 = l4;
                     gr.lobes2 = l2;
                     gr.stride = rb.capacity;

@@ -32,7 +32,7 @@ def progressive(scene, width, height, budget_s, ref, *, guided=False, train_frac
     curve = []
     t = 0.0
     spp = 0
-    steps = samples = 0
+    steps = samples = queries = 0
     infer_ms = train_ms = 0.0
     while t < budget_s and spp < max_spp:
         training = guided and t < train_fraction * budget_s
@@ -48,6 +48,7 @@ def progressive(scene, width, height, budget_s, ref, *, guided=False, train_frac
         steps += st["guide_steps"]
         samples += st["guide_samples"]
         infer_ms += st["ms_guide_infer"]
+        queries += st["guide_queries"]
         train_ms += st["ms_guide_train"]
         img = film.cpu().numpy().astype(np.float64) / spp
         curve.append((t, spp, rel_mse(img, ref)))
@@ -55,7 +56,7 @@ def progressive(scene, width, height, budget_s, ref, *, guided=False, train_frac
         log(f"[equal-time] guided={guided}: {spp} spp in {t:.3f} s, relMSE {curve[-1][2]:.4e}, "
             f"adam steps {steps}, samples {samples}, infer {infer_ms:.1f} ms, train {train_ms:.1f} ms")
     return curve, {"adam_steps": steps, "train_samples": samples, "ms_guide_infer": infer_ms,
-                   "ms_guide_train": train_ms}
+                   "ms_guide_train": train_ms, "guide_queries": queries}
 
 
 def time_to_reach(curve, target):
