@@ -167,8 +167,8 @@ def guiding_main(args):
         tf = g["guide_queries"] * 33792 / (g["ms_guide_infer"] * 1e-3) / 1e12
         mlp["infer"] = {"queries": g["guide_queries"], "flop_per_query": 33792, "tflops": round(tf, 2),
                         "frac": round(tf / 2500.0, 4)}
-    if g.get("train_samples") and g.get("ms_guide_train"):
-        n = min(g["train_samples"], g["adam_steps"] * 65536)   # samples through the gradient kernel
+    if g.get("grad_samples") and g.get("ms_guide_train"):
+        n = g["grad_samples"]   # samples through the gradient kernels, all epochs
         tf = n * 89088 / (g["ms_guide_train"] * 1e-3) / 1e12
         mlp["train"] = {"minibatch_samples": n, "flop_per_sample": 89088, "tflops": round(tf, 2),
                         "frac": round(tf / 2500.0, 4)}

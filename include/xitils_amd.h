@@ -181,9 +181,10 @@ typedef struct xt_render_stats {
      * filled only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
     /* guiding: network inference and training kernel time, Adam steps taken,
-     * training samples kept and queue entries run through the network */
+     * training samples kept, queue entries run through the network and
+     * samples run through the gradient kernels (all epochs) */
     double ms_guide_infer, ms_guide_train;
-    uint64_t guide_steps, guide_samples, guide_queries;
+    uint64_t guide_steps, guide_samples, guide_queries, guide_grad_samples;
 } xt_render_stats;
 
 typedef struct xt_ray {        /* 32 bytes */
@@ -304,12 +305,14 @@ typedef struct xt_guide_config {
     float guide_prob;            /* one-sample MIS probability of sampling the guide (0.5) */
     float learning_rate;         /* Adam step size (1e-2) */
     float beta1, beta2, adam_eps;/* Adam (0.9, 0.999, 1e-8) */
-    uint32_t minibatch;          /* samples per Adam step (65536) */
-    uint32_t max_steps_per_batch;/* Adam steps after one render batch (32) */
+    uint32_t minibatch;          /* samples per Adam step (32768) */
+    uint32_t max_steps_per_batch;/* Adam steps after one render batch (256) */
     float weight_clip;           /* cap on sample weights normalised to mean 1 (20) */
     uint32_t seed;               /* parameter initialisation */
     uint32_t max_records;        /* training records kept per render batch (8 Mi) */
     uint32_t train_target;       /* 0: incident radiance L_i, 1: product f cos L_i (default) */
+    uint32_t epochs;             /* passes over each render batch's samples, capped by max_steps_per_batch (3; 0 = 1) */
+    uint32_t max_guided_vertex;  /* guide surface vertices 1..n of each path only (0 = all) */
 } xt_guide_config;
 
 void xt_guide_config_default(xt_guide_config* cfg);

@@ -87,7 +87,7 @@ class xt_render_stats(C.Structure):
                 ("ext_node_fetches", C.c_uint64), ("ext_tri_fetches", C.c_uint64),
                 ("shadow_node_fetches", C.c_uint64), ("shadow_tri_fetches", C.c_uint64),
                 ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double), ("guide_steps", C.c_uint64),
-                ("guide_samples", C.c_uint64), ("guide_queries", C.c_uint64)]
+                ("guide_samples", C.c_uint64), ("guide_queries", C.c_uint64), ("guide_grad_samples", C.c_uint64)]
 
 
 class xt_batch_item(C.Structure):
@@ -99,7 +99,7 @@ class xt_guide_config(C.Structure):
     _fields_ = [("guide_prob", C.c_float), ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("adam_eps", C.c_float), ("minibatch", C.c_uint32), ("max_steps_per_batch", C.c_uint32),
                 ("weight_clip", C.c_float), ("seed", C.c_uint32), ("max_records", C.c_uint32),
-                ("train_target", C.c_uint32)]
+                ("train_target", C.c_uint32), ("epochs", C.c_uint32), ("max_guided_vertex", C.c_uint32)]
 
 
 _lib = None

@@ -152,6 +152,15 @@ __global__ void k_guide_prepare(const float* __restrict__ P, GuideNet net) {
         const int o = output_of_row(i);
         ch.b3[i] = o >= 0 ? P[G_OFF_B3 + o] : 0.0f;
     }
+    // backward copy (GuideBack): bk.wL[m][p] = W_L[unit at k position p][input unit m]
+    GuideBack& bk = *net.back;
+    if (i < G_HID * GC_LD) {
+        const int m = i / GC_LD, p = i % GC_LD, u = p < G_HID ? unit_of_kpos(p) : -1;
+        const int o = u >= 0 && u < G_OUT_PAD ? output_of_row(u) : -1;
+        bk.w3[i] = (__bf16)(o >= 0 ? P[G_OFF_W3 + o * G_HID + m] : 0.0f);
+        bk.w2[i] = (__bf16)(u >= 0 ? P[G_OFF_W2 + u * G_HID + m] : 0.0f);
+        bk.w1[i] = (__bf16)(u >= 0 ? P[G_OFF_W1 + u * G_HID + m] : 0.0f);
+    }
 }
 
 // ---------------------------------------------------------------- inference
@@ -313,7 +322,7 @@ __global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, GuideBounds
 __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue pq,
                                                      const uint32_t* __restrict__ count, const float4* __restrict__ hits,
                                                      float4* __restrict__ lobes4, float2* __restrict__ lobes2,
-                                                     uint32_t stride) {
+                                                     uint32_t stride, uint32_t max_vertex) {
     __shared__ __attribute__((aligned(16))) GuideChain s;
     load_chain(net.chain, s);
     const uint32_t n = *count;
@@ -326,7 +335,8 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
         if (i < n) {
             const float4 h = hits[i];
             const uint32_t slot = __float_as_uint(h.y);
-            if (slot != INVALID) {
+            const uint32_t v = __float_as_uint(pq.w[i].w) & 0xFFFFFFu;   // vertex index (shaded at v)
+            if (slot != INVALID && (max_vertex == 0 || v <= max_vertex)) {
                 const uint32_t mat_id = __float_as_uint(sc.tris[(size_t)slot * TRI_F4 + 1].w);
                 const uint32_t mt = sc.mats[mat_id].type;
                 if (mt != XT_MAT_EMISSION && !mat_is_delta(mt)) {
@@ -537,6 +547,361 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
     wave_sync();
     float* out = partial + (size_t)blockIdx.x * G_NPARAMS;
     for (int i = l; i < G_NPARAMS; i += 64) out[i] = s.acc[i];
+}
+
+// ---------------------------------------------------------------- training: per-sample chains
+
+// The four layers for NT tiles of 16 samples keeping every activation (the
+// input fragments of the next layer) for the backward pass.
+template <int NT>
+__device__ __forceinline__ void chain_forward_keep(const GuideChain& s, const bf16x8 (&x)[NT][3], bf16x8 (&h1)[NT][2],
+                                                   bf16x8 (&h2)[NT][2], bf16x8 (&h3)[NT][2], floatx4 (&y)[NT][3]) {
+    const int g = __lane_id() >> 4;
+    floatx4 acc[NT][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const floatx4 bias = *(const floatx4*)&s.b0[16 * b + 4 * g];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bf16x8 a = frag(s.w0, GC_LD0, 16 * b, 32 * c);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, x[t][c], acc[t][b]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        h1[t][0] = relu_pack(acc[t][0], acc[t][1]);
+        h1[t][1] = relu_pack(acc[t][2], acc[t][3]);
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        const __bf16* w = layer == 0 ? s.w1 : s.w2;
+        const float* bl = layer == 0 ? s.b1 : s.b2;
+        bf16x8 (&hin)[NT][2] = layer == 0 ? h1 : h2;
+        bf16x8 (&hout)[NT][2] = layer == 0 ? h2 : h3;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const floatx4 bias = *(const floatx4*)&bl[16 * b + 4 * g];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const bf16x8 a = frag(w, GC_LD, 16 * b, 32 * c);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, hin[t][c], acc[t][b]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            hout[t][0] = relu_pack(acc[t][0], acc[t][1]);
+            hout[t][1] = relu_pack(acc[t][2], acc[t][3]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const floatx4 bias = *(const floatx4*)&s.b3[16 * b + 4 * g];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) y[t][b] = bias;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bf16x8 a = frag(s.w3, GC_LD, 16 * b, 32 * c);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) y[t][b] = mfma(a, h3[t][c], y[t][b]);
+        }
+    }
+}
+
+// Two adjacent 16-unit gradient tiles -> one input fragment (no activation).
+__device__ __forceinline__ bf16x8 grad_pack(floatx4 a, floatx4 b) {
+    bf16x8 h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h[i] = (__bf16)a[i];
+        h[4 + i] = (__bf16)b[i];
+    }
+    return h;
+}
+
+// d = W^T z for one layer of the backward chain (weights: GuideBack rows
+// [64][GC_LD], z: input fragments over the layer's output units), masked by
+// the layer input's activation (h > 0) -> dz of the layer below in output
+// tiles (units 16b + 4g + i in natural order).
+template <int NT>
+__device__ __forceinline__ void back_step(const __bf16* w, const bf16x8 (&z)[NT][2], const bf16x8 (&h)[NT][2],
+                                          floatx4 (&dz)[NT][4]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) dz[t][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bf16x8 a = frag(w, GC_LD, 16 * b, 32 * c);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) dz[t][b] = mfma(a, z[t][c], dz[t][b]);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (!((float)h[t][b >> 1][4 * (b & 1) + i] > 0.0f)) dz[t][b][i] = 0.0f;
+    }
+}
+
+// Per-wave staging of T rows in LDS: [row][32 samples] bf16 with an 80-byte
+// row stride (16-byte aligned rows; the four lane groups of a store land 16
+// banks apart).  Lanes store single values at immediate offsets, then the
+// wave copies whole 64-byte rows to T with 16-byte stores.
+constexpr int GS_STRIDE = 40;                 // bf16 per staged row
+constexpr int GS_ROWS = 96;                   // rows per flush
+struct GradStage {
+    __bf16 v[GS_ROWS * GS_STRIDE];
+};
+// this lane's units of NB output tiles: row 16b + 4g + i, sample column col (0..31)
+template <int NB>
+__device__ __forceinline__ void stage_tiles(GradStage& st, int col, const floatx4 (&v)[NB]) {
+    const int g = __lane_id() >> 4;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) st.v[(16 * b + 4 * g + i) * GS_STRIDE + col] = (__bf16)v[b][i];
+}
+// one input fragment set: k position 32c + 8g + j holds unit unit_of_kpos(...)
+__device__ __forceinline__ void stage_frags(GradStage& st, int col, const bf16x8 (&h)[2]) {
+    const int g = __lane_id() >> 4;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) st.v[unit_of_kpos(32 * c + 8 * g + j) * GS_STRIDE + col] = h[c][j];
+}
+// rows [0, R) of the stage -> T[row0 + row][base .. base + 31]
+template <int R>
+__device__ __forceinline__ void flush_stage(GradStage& st, __bf16* T, uint32_t ld, int row0, uint32_t base) {
+    wave_sync();
+    const int l = __lane_id();
+#pragma unroll
+    for (int k = l; k < 4 * R; k += 64) {
+        const int row = k >> 2, part = k & 3;
+        const uint4 v = *(const uint4*)&st.v[row * GS_STRIDE + 8 * part];
+        *(uint4*)&T[(size_t)(row0 + row) * ld + base + 8 * part] = v;
+    }
+    wave_sync();
+}
+
+// Per-sample part of the gradient of sum_i -min(w_i * weight_scale,
+// weight_clip) log q(dir_i | query_i): forward and backward chains with
+// activations in registers, 32 samples per wave pass; writes the features,
+// hidden activations and pre-activation gradients of every sample to T (see
+// GT_*) for the weight-gradient products (k_guide_wgrad).  Samples >= n are
+// written as zeros up to the next multiple of 32.
+__global__ __launch_bounds__(GB_BLOCK) void k_guide_backprop(GuideNet net, GuideBounds gb,
+                                                             const GuideSample* __restrict__ smp, uint32_t n,
+                                                             float weight_scale, float weight_clip, __bf16* __restrict__ T,
+                                                             uint32_t ld) {
+    constexpr int NT = 2;
+    __shared__ __attribute__((aligned(16))) GuideChain s;
+    __shared__ __attribute__((aligned(16))) GuideBack bk;
+    __shared__ __attribute__((aligned(16))) GradStage stage[GB_BLOCK / 64];
+    {
+        const uint4* src = (const uint4*)net.back;
+        uint4* dst = (uint4*)&bk;
+        for (int k = threadIdx.x; k < (int)(sizeof(GuideBack) / 16); k += blockDim.x) dst[k] = src[k];
+    }
+    load_chain(net.chain, s);   // includes the block barrier
+    GradStage& st = stage[threadIdx.x >> 6];
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t npad = (n + 31u) & ~31u;
+    for (uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 32u; base < npad;
+         base += nwaves * 32u) {
+        // compiler-only fence: the weight fragments are re-read from LDS in each
+        // pass instead of being hoisted into registers for the whole loop
+        asm volatile("" ::: "memory");
+        bf16x8 x[NT][3];
+        f3 dir[NT];
+        float wgt[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const uint32_t si = base + 16 * t + r;
+            f3 P = mk(0, 0, 0), N = mk(0, 0, 0), WO = mk(0, 0, 0);
+            dir[t] = mk(0, 0, 1);
+            wgt[t] = 0.0f;
+            const bool valid = si < n;
+            if (valid) {
+                const float4* sv = (const float4*)&smp[si];
+                const float4 s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3];
+                P = mk(s0.x, s0.y, s0.z);
+                N = mk(s0.w, s1.x, s1.y);
+                WO = mk(s1.z, s1.w, s2.x);
+                dir[t] = mk(s2.y, s2.z, s2.w);
+                wgt[t] = fminf(s3.x * weight_scale, weight_clip);
+            }
+            chain_features(gb, P, N, WO, x[t]);
+            if (!valid)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) x[t][c] = bf16x8{};
+            // features: k position 32c + 8g + j holds feature 24g + 8c + j (g < 3) or 72 + 8c + j (g = 3)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int f = g < 3 ? 24 * g + 8 * c + j : 72 + 8 * c + j;
+                    st.v[f * GS_STRIDE + 16 * t + r] = x[t][c][j];
+                }
+        }
+        flush_stage<G_IN>(st, T, ld, GT_X, base);
+        bf16x8 h1[NT][2], h2[NT][2], h3[NT][2];
+        floatx4 y[NT][3];
+        chain_forward_keep<NT>(s, x, h1, h2, h3, y);
+        // loss gradient: lane group g holds lobes 2g, 2g + 1 (slots 0..9) of sample 16t + r
+        floatx4 dy[NT][3];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
+            float mx = fmaxf(la, lb);
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float ea = expf(la - mx), eb = expf(lb - mx);
+            float esum = ea + eb;
+            esum += __shfl_xor(esum, 16);
+            esum += __shfl_xor(esum, 32);
+            VmfLobe lob[2];
+            float len[2], v[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
+                len[k] = length(m);
+                lob[k].mu = len[k] > 1e-12f ? div(m, len[k]) : mk(0.0f, 0.0f, 1.0f);
+                lob[k].kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
+                lob[k].alpha = (k == 0 ? ea : eb) / esum;
+                v[k] = vmf_lobe_pdf(lob[k], dir[t]);
+            }
+            float q = lob[0].alpha * v[0] + lob[1].alpha * v[1];
+            q += __shfl_xor(q, 16);
+            q += __shfl_xor(q, 32);
+            q = fmaxf(q, 1e-30f);
+            float d[12];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const float rsp = lob[k].alpha * v[k] / q;
+                const float kap = lob[k].kappa;
+                const f3 mu = lob[k].mu;
+                const float mw = dot(mu, dir[t]);
+                const f3 gm = (dir[t] - mw * mu) * (rsp * kap / fmaxf(len[k], 1e-12f));
+                const float dk = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (mw - 1.0f);
+                const float kl = out_slot(y[t], 5 * k + 3);
+                const float gk = (kl > G_KLOG_MIN && kl < G_KLOG_MAX) ? rsp * dk * kap : 0.0f;
+                const float ga = rsp - lob[k].alpha;
+                d[5 * k] = -wgt[t] * gm.x;
+                d[5 * k + 1] = -wgt[t] * gm.y;
+                d[5 * k + 2] = -wgt[t] * gm.z;
+                d[5 * k + 3] = -wgt[t] * gk;
+                d[5 * k + 4] = -wgt[t] * ga;
+            }
+            d[10] = d[11] = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) dy[t][b][i] = d[4 * b + i];
+        }
+        // activations, then dy (rows GT_DY + raw output index, rounded as the backward chain reads it)
+        const bf16x8 (*hs[3])[2] = {h1, h2, h3};
+#pragma unroll
+        for (int layer = 0; layer < 3; ++layer) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) stage_frags(st, 16 * t + r, hs[layer][t]);
+            flush_stage<G_HID>(st, T, ld, GT_H1 + G_HID * layer, base);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int k = 0; k < 10; ++k) st.v[(10 * g + k) * GS_STRIDE + 16 * t + r] = (__bf16)out_slot(dy[t], k);
+        flush_stage<G_OUT>(st, T, ld, GT_DY, base);
+        // backward chain: dz3 = (W3^T dy) * [h3 > 0], dz2 = (W2^T dz3) * [h2 > 0], dz1 = (W1^T dz2) * [h1 > 0]
+        bf16x8 z[NT][2];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            z[t][0] = grad_pack(dy[t][0], dy[t][1]);
+            z[t][1] = grad_pack(dy[t][2], floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        }
+        floatx4 dz[NT][4];
+        back_step<NT>(bk.w3, z, h3, dz);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            stage_tiles<4>(st, 16 * t + r, dz[t]);
+            z[t][0] = grad_pack(dz[t][0], dz[t][1]);
+            z[t][1] = grad_pack(dz[t][2], dz[t][3]);
+        }
+        flush_stage<G_HID>(st, T, ld, GT_D3, base);
+        back_step<NT>(bk.w2, z, h2, dz);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            stage_tiles<4>(st, 16 * t + r, dz[t]);
+            z[t][0] = grad_pack(dz[t][0], dz[t][1]);
+            z[t][1] = grad_pack(dz[t][2], dz[t][3]);
+        }
+        flush_stage<G_HID>(st, T, ld, GT_D2, base);
+        back_step<NT>(bk.w1, z, h1, dz);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) stage_tiles<4>(st, 16 * t + r, dz[t]);
+        flush_stage<G_HID>(st, T, ld, GT_D1, base);
+    }
+}
+
+// ---------------------------------------------------------------- training: weight-gradient products
+
+// One 16-row strip of a layer's gradient over samples [k0, k1): dW[m][n] =
+// sum_k D[m][k] A[n][k] (D = dz rows, A = activation rows of T), db[m] =
+// sum_k D[m][k]; written to row[...] of this chunk.
+template <int IN>
+__device__ __forceinline__ void wgrad_strip(const __bf16* T, uint32_t ld, int drow, int arow, uint32_t k0, uint32_t k1,
+                                            float* row, int off_w, int off_b, int rows_valid) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+    floatx4 acc[IN / 16];
+#pragma unroll
+    for (int c = 0; c < IN / 16; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (uint32_t k = k0; k < k1; k += 32) {
+        const bf16x8 a = frag(T + (size_t)drow * ld, (int)ld, 0, (int)k);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bsum += (float)a[j];
+#pragma unroll
+        for (int c = 0; c < IN / 16; ++c) acc[c] = mfma(a, frag(T + (size_t)(arow + 16 * c) * ld, (int)ld, 0, (int)k), acc[c]);
+    }
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+#pragma unroll
+    for (int c = 0; c < IN / 16; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 4 * g + i;
+            if (m < rows_valid) row[off_w + m * IN + 16 * c + r] = acc[c][i];
+        }
+    if (g == 0 && r < rows_valid) row[off_b + r] = bsum;
+}
+
+// Weight and bias gradients of one chunk of `chunk` samples (block x) from
+// T: 15 strips of 16 output rows (4 per hidden layer, 3 for the output
+// layer) spread over the block's 4 waves; one partial row per block, summed
+// by k_guide_adam.
+__global__ __launch_bounds__(GW_BLOCK) void k_guide_wgrad(const __bf16* __restrict__ T, uint32_t ld, uint32_t n,
+                                                          uint32_t chunk, float* __restrict__ partial) {
+    const uint32_t npad = (n + 31u) & ~31u;
+    const uint32_t k0 = blockIdx.x * chunk, k1 = min(npad, k0 + chunk);
+    float* row = partial + (size_t)blockIdx.x * G_NPARAMS;
+    const int wave = threadIdx.x >> 6;
+    for (int strip = wave; strip < 15; strip += GW_BLOCK / 64) {
+        const int layer = strip >> 2, sb = strip & 3, m0 = 16 * sb;
+        if (layer == 0)
+            wgrad_strip<G_IN>(T, ld, GT_D1 + m0, GT_X, k0, k1, row, G_OFF_W0 + m0 * G_IN, G_OFF_B0 + m0, 16);
+        else if (layer == 1)
+            wgrad_strip<G_HID>(T, ld, GT_D2 + m0, GT_H1, k0, k1, row, G_OFF_W1 + m0 * G_HID, G_OFF_B1 + m0, 16);
+        else if (layer == 2)
+            wgrad_strip<G_HID>(T, ld, GT_D3 + m0, GT_H2, k0, k1, row, G_OFF_W2 + m0 * G_HID, G_OFF_B2 + m0, 16);
+        else
+            wgrad_strip<G_HID>(T, ld, GT_DY + m0, GT_H3, k0, k1, row, G_OFF_W3 + m0 * G_HID, G_OFF_B3 + m0, 16);
+    }
 }
 
 // Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the sum

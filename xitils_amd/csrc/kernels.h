@@ -107,6 +107,7 @@ struct GuideRender {
     TrainRec* recs;        // null: no training records
     uint32_t rec_base;     // record index of this bounce's first queue entry
     uint32_t rec_cap;
+    uint32_t max_vertex;   // guide surface vertices 1..max_vertex only (0: all)
 };
 
 template <bool ANY, int LDS_DEPTH, bool COUNT>

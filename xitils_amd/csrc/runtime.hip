@@ -64,8 +64,10 @@ struct GuideState {
     float* P = nullptr;              // fp32 master parameters
     float* m = nullptr;              // Adam moments
     float* v = nullptr;
-    float* partial = nullptr;        // per-block gradient rows of k_guide_grad
+    float* partial = nullptr;        // per-chunk gradient rows of k_guide_wgrad
     uint32_t partial_rows = 0;
+    __bf16* T = nullptr;             // per-sample activations / gradients of a minibatch (GT_ROWS x t_ld)
+    uint32_t t_ld = 0;
     void* net_mem = nullptr;         // bf16 weight copies + fp32 biases
     GuideNet net{};
     uint64_t step = 0;               // Adam steps taken
@@ -76,7 +78,7 @@ struct GuideState {
     float* d_wsum = nullptr;         // their weight sum
     float* host_io = nullptr;        // pinned: count + weight sum read-back
     void release() {
-        hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem);
+        hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem); hipFree(T);
         hipFree(recs); hipFree(samples); hipFree(d_count); hipFree(d_wsum);
         if (host_io) hipHostFree(host_io);
         *this = GuideState();
@@ -775,8 +777,31 @@ static xt_status guide_need(xt_scene* s) {
     return XT_OK;
 }
 
-static uint32_t guide_grad_blocks(const xt_scene* s, uint32_t n) {
-    return std::max(1u, std::min((n + 63) / 64, s->guide.partial_rows));
+// Gradient of the minibatch d_samples[0..n) into g.partial rows [0, *rows):
+// the per-sample chains write T, then one weight-gradient product per chunk
+// of samples (at most partial_rows chunks).
+static xt_status guide_gradient_rows(xt_scene* s, const GuideSample* d_samples, uint32_t n, float weight_scale,
+                                     float weight_clip, uint32_t* rows) {
+    GuideState& g = s->guide;
+    const uint32_t npad = (n + 31u) & ~31u;
+    if (npad > g.t_ld || !g.T) {
+        hipFree(g.T);
+        g.T = nullptr;
+        g.t_ld = std::max<uint32_t>((npad + 255u) & ~255u, 256u);
+        XT_HIP(dalloc(&g.T, (size_t)GT_ROWS * g.t_ld));
+        XT_HIP(hipMemsetAsync(g.T, 0, (size_t)GT_ROWS * g.t_ld * sizeof(__bf16), s->stream));   // dy rows 40..47 stay 0
+    }
+    const uint32_t waves = std::max(1u, npad / 32u);
+    const uint32_t bp_blocks = std::max(1u, std::min((waves + GB_BLOCK / 64 - 1) / (GB_BLOCK / 64),
+                                                     (uint32_t)s->dev->num_cu));
+    hipLaunchKernelGGL(k_guide_backprop, dim3(bp_blocks), dim3(GB_BLOCK), 0, s->stream, g.net, g.gb, d_samples, n,
+                       weight_scale, weight_clip, g.T, g.t_ld);
+    uint32_t chunk = std::max(256u, (((npad + g.partial_rows - 1) / g.partial_rows) + 31u) & ~31u);
+    const uint32_t nchunks = std::max(1u, (npad + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_guide_wgrad, dim3(nchunks), dim3(GW_BLOCK), 0, s->stream, g.T, g.t_ld, n, chunk, g.partial);
+    XT_HIP(hipGetLastError());
+    *rows = nchunks;
+    return XT_OK;
 }
 
 // Adam steps over d_samples[0..n) in minibatches, weights scaled by weight_scale.
@@ -784,12 +809,14 @@ static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t
                              uint32_t max_steps, uint64_t* steps_taken) {
     GuideState& g = s->guide;
     const uint32_t M = std::max(64u, g.cfg.minibatch);
-    const uint32_t steps = std::min<uint32_t>((n + M - 1) / M, std::max(1u, max_steps));
+    const uint32_t per_epoch = (n + M - 1) / M;
+    const uint32_t steps = std::min<uint32_t>(per_epoch * std::max(1u, g.cfg.epochs), std::max(1u, max_steps));
     for (uint32_t k = 0; k < steps; ++k) {
-        const uint32_t off = k * M, cnt = std::min(M, n - off);
-        const uint32_t grid = guide_grad_blocks(s, cnt);
-        hipLaunchKernelGGL(k_guide_grad, dim3(grid), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, d_samples + off, cnt,
-                           weight_scale, g.cfg.weight_clip, g.partial);
+        const uint32_t off = (k % per_epoch) * M, cnt = std::min(M, n - off);
+        uint32_t grid = 0;
+        xt_status gst = guide_gradient_rows(s, d_samples + off, cnt, weight_scale, g.cfg.weight_clip, &grid);
+        if (gst != XT_OK) return gst;
+        s->stats.guide_grad_samples += cnt;
         g.step++;
         const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
         const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
@@ -952,7 +979,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     float4* l4 = (float4*)rb.lobes;
                     float2* l2 = (float2*)(rb.lobes + (size_t)4 * G_LOBES * rb.capacity);
                     hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
-                                       rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity);
+                                       rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
                     XT_LAUNCHED(result);
                     tm.add(4, a, tm.mark());
                     s->stats.guide_queries += live;
@@ -960,6 +987,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     gr.lobes2 = l2;
                     gr.stride = rb.capacity;
                     gr.prob = g.cfg.guide_prob;
+                    gr.max_vertex = g.cfg.max_guided_vertex;
                     if (train) {
                         gr.recs = g.recs;
                         gr.rec_base = rec_base;
@@ -1144,11 +1172,12 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
         GuideRender gr{};
         if (guide) {
             hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, rb.q[cur],
-                               rb.counts + cur, rb.hits, l4, l2, rb.capacity);
+                               rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
             gr.lobes4 = l4;
             gr.lobes2 = l2;
             gr.stride = rb.capacity;
             gr.prob = g.cfg.guide_prob;
+            gr.max_vertex = g.cfg.max_guided_vertex;
         }
         hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
                            rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
@@ -1357,12 +1386,14 @@ void xt_guide_config_default(xt_guide_config* c) {
     c->beta1 = 0.9f;
     c->beta2 = 0.999f;
     c->adam_eps = 1e-8f;
-    c->minibatch = 65536;
-    c->max_steps_per_batch = 32;
+    c->minibatch = 32768;
+    c->max_steps_per_batch = 256;
     c->weight_clip = 20.0f;
     c->seed = 1;
     c->max_records = 1u << 23;
     c->train_target = 1;
+    c->epochs = 3;
+    c->max_guided_vertex = 0;
 }
 
 xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
@@ -1373,6 +1404,7 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     if (!(c.guide_prob >= 0.0f && c.guide_prob < 1.0f))
         return fail(XT_ERR_INVALID_ARGUMENT, "guide_prob must be in [0, 1)");
     if (c.train_target > 1) return xt_fail(XT_ERR_INVALID_ARGUMENT, "train_target must be 0 or 1");
+    if (c.epochs == 0) c.epochs = 1;
     if (!(c.learning_rate > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f))
         return fail(XT_ERR_INVALID_ARGUMENT, "bad Adam settings");
     XT_HIP(hipSetDevice(s->dev->ordinal));
@@ -1405,11 +1437,11 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     chk(dalloc(&g.d_wsum, 1));
     chk(hipHostMalloc((void**)&g.host_io, 2 * sizeof(float), hipHostMallocDefault));
     // bf16 copies and biases in one allocation, 256-byte aligned pieces
-    const size_t sizes[13] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
+    const size_t sizes[14] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
                               2 * G_IN * G_HID, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_HID * 64,
-                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD, sizeof(GuideChain)};
-    size_t off[13], total = 0;
-    for (int k = 0; k < 13; ++k) {
+                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD, sizeof(GuideChain), sizeof(GuideBack)};
+    size_t off[14], total = 0;
+    for (int k = 0; k < 14; ++k) {
         off[k] = total;
         total += (sizes[k] + 255) / 256 * 256;
     }
@@ -1426,6 +1458,7 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     g.net.b0 = (float*)(b + off[8]); g.net.b1 = (float*)(b + off[9]);
     g.net.b2 = (float*)(b + off[10]); g.net.b3 = (float*)(b + off[11]);
     g.net.chain = (GuideChain*)(b + off[12]);
+    g.net.back = (GuideBack*)(b + off[13]);
     g.ready = true;
     std::vector<float> P;
     guide_init_params(c.seed, P);
@@ -1502,10 +1535,9 @@ xt_status xt_guide_gradient(xt_scene* s, const float* queries, const float* dirs
         if (st != XT_OK) return st;
     }
     GuideState& g = s->guide;
-    const uint32_t grid = guide_grad_blocks(s, n);
-    hipLaunchKernelGGL(k_guide_grad, dim3(grid), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, ds, n, weight_scale,
-                       weight_clip, g.partial);
-    XT_HIP(hipGetLastError());
+    uint32_t grid = 0;
+    st = guide_gradient_rows(s, ds, n, weight_scale, weight_clip, &grid);
+    if (st != XT_OK) return st;
     std::vector<float> rows((size_t)grid * G_NPARAMS);
     XT_HIP(hipMemcpyAsync(rows.data(), g.partial, rows.size() * 4, hipMemcpyDeviceToHost, s->stream));
     XT_HIP(hipStreamSynchronize(s->stream));

@@ -25,17 +25,27 @@ from .quality import rel_mse
 
 
 def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=0.25, spp_per_call=4,
-                train_spp_per_call=1, seed=1, flags=0, max_depth=16, max_spp=1 << 20, log=None):
-    """Returns the error curve [(seconds, spp, relMSE)] of one progressive run."""
+                train_spp_per_call=1, seed=1, flags=0, max_depth=16, max_spp=1 << 20, keep_training_samples=True,
+                log=None):
+    """Returns the error curve [(seconds, spp, relMSE)] of one progressive run.
+    keep_training_samples=False restarts the film when training ends (the
+    samples rendered with the untrained network are dropped; the sample
+    index keeps counting, so no sample is rendered twice)."""
     import torch
     film = torch.zeros((height, width, 3), dtype=torch.float32, device="cuda:0")
     curve = []
     t = 0.0
-    spp = 0
-    steps = samples = queries = 0
+    spp = 0          # sample index of the next call
+    film_spp = 0     # samples in the film
+    was_training = False
+    steps = samples = queries = grad_samples = 0
     infer_ms = train_ms = 0.0
     while t < budget_s and spp < max_spp:
         training = guided and t < train_fraction * budget_s
+        if was_training and not training and not keep_training_samples:
+            film.zero_()
+            film_spp = 0
+        was_training = training
         f = flags | (api.XT_FLAG_GUIDE if guided else 0) | (api.XT_FLAG_GUIDE_TRAIN if training else 0)
         n = train_spp_per_call if training else spp_per_call
         torch.cuda.synchronize()
@@ -44,19 +54,21 @@ def progressive(scene, width, height, budget_s, ref, *, guided=False, train_frac
         torch.cuda.synchronize()
         t += time.perf_counter() - t0
         spp += n
+        film_spp += n
         st = scene.stats()
         steps += st["guide_steps"]
         samples += st["guide_samples"]
         infer_ms += st["ms_guide_infer"]
         queries += st["guide_queries"]
+        grad_samples += st["guide_grad_samples"]
         train_ms += st["ms_guide_train"]
-        img = film.cpu().numpy().astype(np.float64) / spp
+        img = film.cpu().numpy().astype(np.float64) / film_spp
         curve.append((t, spp, rel_mse(img, ref)))
     if log:
         log(f"[equal-time] guided={guided}: {spp} spp in {t:.3f} s, relMSE {curve[-1][2]:.4e}, "
             f"adam steps {steps}, samples {samples}, infer {infer_ms:.1f} ms, train {train_ms:.1f} ms")
     return curve, {"adam_steps": steps, "train_samples": samples, "ms_guide_infer": infer_ms,
-                   "ms_guide_train": train_ms, "guide_queries": queries}
+                   "ms_guide_train": train_ms, "guide_queries": queries, "grad_samples": grad_samples}
 
 
 def time_to_reach(curve, target):
