@@ -9,14 +9,10 @@
 
 namespace xt {
 
-// Device copies of the network used by the kernels: bf16 weights in natural
-// [out][in] order (forward) and transposed [in][out] order (backward), fp32
-// biases.
+// Device copies of the network used by the kernels (bf16 weights laid out as
+// the kernels hold them in LDS, fp32 biases).
 struct GuideNet {
-    __bf16 *w0, *w1, *w2, *w3;
-    __bf16 *w0t, *w1t, *w2t, *w3t;
-    float *b0, *b1, *b2, *b3;
-    struct GuideChain* chain;   // inference copy (below)
+    struct GuideChain* chain;   // forward copy (below)
     struct GuideBack* back;     // backward copy (below)
 };
 
@@ -71,8 +67,6 @@ __global__ void k_guide_prepare(const float* P, GuideNet net);
 __global__ void k_guide_forward(GuideNet net, GuideBounds gb, const float* qin, uint32_t n, float* out);
 __global__ void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue q, const uint32_t* count,
                               const float4* hits, float4* lobes4, float2* lobes2, uint32_t stride, uint32_t max_vertex);
-__global__ void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
-                             float weight_clip, float* partial);
 __global__ void k_guide_backprop(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
                                  float weight_clip, __bf16* T, uint32_t ld);
 __global__ void k_guide_wgrad(const __bf16* T, uint32_t ld, uint32_t n, uint32_t chunk, float* partial);
@@ -82,7 +76,6 @@ __global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4*
                                  uint32_t* count, float* weight_sum);
 
 constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
-constexpr int GG_BLOCK = 64;    // threads per block of k_guide_grad
 constexpr int GB_BLOCK = 512;   // threads per block of k_guide_backprop (32 samples per wave pass)
 constexpr int GW_BLOCK = 256;   // threads per block of k_guide_wgrad (one k chunk per block)
 constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize

@@ -6,8 +6,10 @@
 //   A[m0 + l%16][k0 + 8(l/16) + j], B[k0 + 8(l/16) + j][n0 + l%16] (j = 0..7),
 //   C[m0 + 4(l/16) + i][n0 + l%16] (i = 0..3)
 // (checked on the device).  Every operand is stored with its K index
-// contiguous, so a fragment is one 16-byte load.  Activations travel
-// query-major through LDS: X/H[query][feature].
+// contiguous, so a fragment is one 16-byte load.  Activations stay in
+// registers between layers (the output fragment of one layer is the input
+// fragment of the next, see GuideChain); training writes them unit-major to
+// global memory for the weight-gradient products.
 #include <hip/hip_runtime.h>
 
 #include "guide.h"
@@ -31,45 +33,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One dense layer for 64 queries: out[q][o] = act(sum_k W[o][k] in[q][k] + b[o])
-// W: [OUT][IN] bf16 (OUT multiple of 16, IN multiple of 32), in: [64][IN] bf16
-// (query-major); writes bf16 [64][OUT] (hidden) or fp32 [64][OUT] (last), and
-// optionally the bf16 transpose [OUT][64].
-template <int IN, int OUT, bool RELU, bool F32OUT>
-__device__ __forceinline__ void dense64(const __bf16* W, const float* bias, const __bf16* in, void* out,
-                                        __bf16* out_t = nullptr) {
-    const int l = __lane_id(), g = l >> 4, r = l & 15;
-    // all weight fragments of the layer first: one memory round trip per layer
-    bf16x8 wf[OUT / 16][IN / 32];
-#pragma unroll
-    for (int b = 0; b < OUT / 16; ++b)
-#pragma unroll
-        for (int c = 0; c < IN / 32; ++c) wf[b][c] = frag(W, IN, 16 * b, 32 * c);
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        bf16x8 xf[IN / 32];
-#pragma unroll
-        for (int c = 0; c < IN / 32; ++c) xf[c] = frag(in, IN, 16 * t, 32 * c);
-        const int q = 16 * t + r;
-#pragma unroll
-        for (int b = 0; b < OUT / 16; ++b) {
-            floatx4 acc;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] = bias[16 * b + 4 * g + i];
-#pragma unroll
-            for (int c = 0; c < IN / 32; ++c) acc = mfma(wf[b][c], xf[c], acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int o = 16 * b + 4 * g + i;
-                float v = RELU ? fmaxf(acc[i], 0.0f) : acc[i];
-                if (F32OUT) ((float*)out)[q * OUT + o] = v;
-                else ((__bf16*)out)[q * OUT + o] = (__bf16)v;
-                if (out_t) out_t[o * 64 + q] = (__bf16)v;
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------- parameters
@@ -96,38 +59,10 @@ __host__ __device__ __forceinline__ int output_of_row(int m) {      // -1: unuse
     return s < 10 ? 10 * g + s : -1;
 }
 
-// fp32 master parameters -> bf16 copies: natural [out][in] for the forward
-// pass and transposed [in][out] (w3t padded to [64][64]) for the backward
-// pass; biases stay fp32.
+// fp32 master parameters -> the inference copy (GuideChain) and the
+// backward copy (GuideBack), both bf16 with fp32 biases.
 __global__ void k_guide_prepare(const float* __restrict__ P, GuideNet net) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < G_HID * G_IN) {
-        int o = i / G_IN, k = i % G_IN;
-        net.w0[i] = (__bf16)P[G_OFF_W0 + i];
-        net.w0t[k * G_HID + o] = (__bf16)P[G_OFF_W0 + i];
-    }
-    if (i < G_HID * G_HID) {
-        int o = i / G_HID, k = i % G_HID;
-        net.w1[i] = (__bf16)P[G_OFF_W1 + i];
-        net.w1t[k * G_HID + o] = (__bf16)P[G_OFF_W1 + i];
-        net.w2[i] = (__bf16)P[G_OFF_W2 + i];
-        net.w2t[k * G_HID + o] = (__bf16)P[G_OFF_W2 + i];
-    }
-    if (i < G_OUT_PAD * G_HID) {
-        int o = i / G_HID, k = i % G_HID;
-        net.w3[i] = (__bf16)P[G_OFF_W3 + i];
-        net.w3t[k * 64 + o] = (__bf16)P[G_OFF_W3 + i];
-    }
-    if (i < G_HID * 16) {
-        int k = i / 16, o = G_OUT_PAD + i % 16;
-        net.w3t[k * 64 + o] = (__bf16)0.0f;
-    }
-    if (i < G_HID) {
-        net.b0[i] = P[G_OFF_B0 + i];
-        net.b1[i] = P[G_OFF_B1 + i];
-        net.b2[i] = P[G_OFF_B2 + i];
-    }
-    if (i < G_OUT_PAD) net.b3[i] = P[G_OFF_B3 + i];
     // inference copy (GuideChain)
     GuideChain& ch = *net.chain;
     if (i < G_HID * GC_LD0) {
@@ -164,10 +99,6 @@ __global__ void k_guide_prepare(const float* __restrict__ P, GuideNet net) {
 }
 
 // ---------------------------------------------------------------- inference
-
-__device__ __forceinline__ void zero_row(__bf16* x) {
-    for (int k = 0; k < G_IN; ++k) x[k] = (__bf16)0.0f;
-}
 
 __device__ __forceinline__ void load_chain(const GuideChain* src, GuideChain& dst) {
     const uint4* s = (const uint4*)src;
@@ -388,165 +319,6 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
             }
         }
     }
-}
-
-// ---------------------------------------------------------------- training
-
-// C[m0..m0+15][n0..n0+15] = sum_k A[m][k] B[k][n], A stored [M][K] (lda), B
-// stored transposed [N][K] (ldb); K a multiple of 32.
-template <int K>
-__device__ __forceinline__ floatx4 prod_tile(const __bf16* A, int lda, int m0, const __bf16* Bt, int ldb, int n0) {
-    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int c = 0; c < K / 32; ++c) acc = mfma(frag(A, lda, m0, 32 * c), frag(Bt, ldb, n0, 32 * c), acc);
-    return acc;
-}
-
-// acc[m][n] += C tile for rows m < mlim.  Lanes of the wave touch distinct
-// elements, and the accumulator belongs to this wave alone.
-__device__ __forceinline__ void add_tile(float* acc, int ld, int m0, int n0, floatx4 c, int mlim) {
-    const int l = __lane_id(), g = l >> 4, r = l & 15;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int m = m0 + 4 * g + i;
-        if (m < mlim) acc[m * ld + n0 + r] += c[i];
-    }
-}
-
-// dW[out][in] += dZ^T H over the 64 samples: dZt [OUTP][64], Ht [IN][64].
-template <int OUT, int IN>
-__device__ __forceinline__ void weight_grad(float* acc, const __bf16* dZt, const __bf16* Ht) {
-#pragma unroll 1
-    for (int b = 0; b < (OUT + 15) / 16; ++b)
-#pragma unroll 1
-        for (int t = 0; t < IN / 16; ++t) add_tile(acc, IN, 16 * b, 16 * t, prod_tile<64>(dZt, 64, 16 * b, Ht, 64, 16 * t), OUT);
-}
-
-// db[o] += sum_q dZ[q][o] (lane o; dZt [OUT][64]).
-template <int OUT>
-__device__ __forceinline__ void bias_grad(float* acc, const __bf16* dZt) {
-    const int l = __lane_id();
-    if (l < OUT) {
-        float sum = 0.0f;
-        for (int q = 0; q < 64; ++q) sum += (float)dZt[l * 64 + q];
-        acc[l] += sum;
-    }
-}
-
-// dZ' = (dZ W) masked by H > 0, in both layouts: dZ [64 q][OUTP], W^T
-// stored [IN][OUTP], Ht [IN][64]; writes dZp [64][IN] and dZpt [IN][64].
-template <int IN, int OUTP>
-__device__ __forceinline__ void back_layer(const __bf16* Wt, const __bf16* dZ, const __bf16* Ht, __bf16* dZp,
-                                           __bf16* dZpt) {
-    const int l = __lane_id(), g = l >> 4, r = l & 15;
-    bf16x8 wf[IN / 16][OUTP / 32];
-#pragma unroll
-    for (int b = 0; b < IN / 16; ++b)
-#pragma unroll
-        for (int c = 0; c < OUTP / 32; ++c) wf[b][c] = frag(Wt, OUTP, 16 * b, 32 * c);
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        bf16x8 zf[OUTP / 32];
-#pragma unroll
-        for (int c = 0; c < OUTP / 32; ++c) zf[c] = frag(dZ, OUTP, 16 * t, 32 * c);
-        const int q = 16 * t + r;
-#pragma unroll
-        for (int b = 0; b < IN / 16; ++b) {
-            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int c = 0; c < OUTP / 32; ++c) acc = mfma(wf[b][c], zf[c], acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int u = 16 * b + 4 * g + i;
-                const float v = (float)Ht[u * 64 + q] > 0.0f ? acc[i] : 0.0f;
-                dZp[q * IN + u] = (__bf16)v;
-                dZpt[u * 64 + q] = (__bf16)v;
-            }
-        }
-    }
-}
-
-struct GradLds {
-    float acc[G_NPARAMS];                 // this block's gradient, flat parameter layout
-    __bf16 xy[64 * G_IN];                 // X [64][96] bf16, later Y [64][48] fp32
-    __bf16 xt[G_IN * 64];                 // X^T
-    __bf16 ht[3][64 * 64];                // H1^T, H2^T, H3^T
-    __bf16 p[2][64 * 64];                 // forward ping-pong, then dZ / dZ^T
-    __bf16 r[2][64 * 64];                 // dZ / dZ^T
-};
-
-// Gradient of sum_i -min(w_i * weight_scale, weight_clip) log q(dir_i | query_i)
-// over the samples.  One wave per block; each block adds its share into LDS
-// and writes it to partial[blockIdx.x][...] (reduced by k_guide_adam).
-__global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* __restrict__ smp,
-                                                         uint32_t n, float weight_scale, float weight_clip,
-                                                         float* __restrict__ partial) {
-    __shared__ __attribute__((aligned(16))) GradLds s;
-    const int l = __lane_id();
-    for (int i = l; i < G_NPARAMS; i += 64) s.acc[i] = 0.0f;
-    __bf16* X = s.xy;
-    float* Y = (float*)s.xy;
-    __bf16 *Ha = s.p[0], *Hb = s.p[1];
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
-        const uint32_t si = base + l;
-        const bool valid = si < n;
-        float4 s0 = make_float4(0, 0, 0, 0), s1 = s0, s2 = s0, s3 = s0;
-        if (valid) {
-            const float4* sv = (const float4*)&smp[si];
-            s0 = sv[0];
-            s1 = sv[1];
-            s2 = sv[2];
-            s3 = sv[3];
-            guide_encode(gb, mk(s0.x, s0.y, s0.z), mk(s0.w, s1.x, s1.y), mk(s1.z, s1.w, s2.x), X + l * G_IN);
-        } else {
-            zero_row(X + l * G_IN);
-        }
-        for (int k = 0; k < G_IN; ++k) s.xt[k * 64 + l] = X[l * G_IN + k];
-        wave_sync();
-        // forward, keeping H^T of every hidden layer
-        dense64<G_IN, G_HID, true, false>(net.w0, net.b0, X, Ha, s.ht[0]);
-        wave_sync();
-        dense64<G_HID, G_HID, true, false>(net.w1, net.b1, Ha, Hb, s.ht[1]);
-        wave_sync();
-        dense64<G_HID, G_HID, true, false>(net.w2, net.b2, Hb, Ha, s.ht[2]);
-        wave_sync();
-        dense64<G_HID, G_OUT_PAD, false, true>(net.w3, net.b3, Ha, Y);
-        wave_sync();
-        // dL/dy -> dY [64][64] and dY^T (outputs >= 40 and padding lanes zero)
-        __bf16 *dY = s.r[0], *dYt = s.r[1];
-        {
-            for (int o = 0; o < 64; ++o) {
-                dY[l * 64 + o] = (__bf16)0.0f;
-                dYt[o * 64 + l] = (__bf16)0.0f;
-            }
-            if (valid) {
-                const float w = fminf(s3.x * weight_scale, weight_clip);
-                guide_loss_grad(Y + l * G_OUT_PAD, mk(s2.y, s2.z, s2.w), w, [&](int o, float v) {
-                    dY[l * 64 + o] = (__bf16)v;
-                    dYt[o * 64 + l] = (__bf16)v;
-                });
-            }
-        }
-        wave_sync();
-        weight_grad<G_OUT, G_HID>(s.acc + G_OFF_W3, dYt, s.ht[2]);
-        bias_grad<G_OUT>(s.acc + G_OFF_B3, dYt);
-        back_layer<G_HID, 64>(net.w3t, dY, s.ht[2], s.p[0], s.p[1]);     // dZ3
-        wave_sync();
-        weight_grad<G_HID, G_HID>(s.acc + G_OFF_W2, s.p[1], s.ht[1]);
-        bias_grad<G_HID>(s.acc + G_OFF_B2, s.p[1]);
-        back_layer<G_HID, G_HID>(net.w2t, s.p[0], s.ht[1], s.r[0], s.r[1]);   // dZ2
-        wave_sync();
-        weight_grad<G_HID, G_HID>(s.acc + G_OFF_W1, s.r[1], s.ht[0]);
-        bias_grad<G_HID>(s.acc + G_OFF_B1, s.r[1]);
-        back_layer<G_HID, G_HID>(net.w1t, s.r[0], s.ht[0], s.p[0], s.p[1]);   // dZ1
-        wave_sync();
-        weight_grad<G_HID, G_IN>(s.acc + G_OFF_W0, s.p[1], s.xt);
-        bias_grad<G_HID>(s.acc + G_OFF_B0, s.p[1]);
-        wave_sync();
-    }
-    wave_sync();
-    float* out = partial + (size_t)blockIdx.x * G_NPARAMS;
-    for (int i = l; i < G_NPARAMS; i += 64) out[i] = s.acc[i];
 }
 
 // ---------------------------------------------------------------- training: per-sample chains

@@ -68,7 +68,7 @@ struct GuideState {
     uint32_t partial_rows = 0;
     __bf16* T = nullptr;             // per-sample activations / gradients of a minibatch (GT_ROWS x t_ld)
     uint32_t t_ld = 0;
-    void* net_mem = nullptr;         // bf16 weight copies + fp32 biases
+    void* net_mem = nullptr;         // GuideChain + GuideBack
     GuideNet net{};
     uint64_t step = 0;               // Adam steps taken
     TrainRec* recs = nullptr;
@@ -1436,12 +1436,10 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     chk(dalloc(&g.d_count, 1));
     chk(dalloc(&g.d_wsum, 1));
     chk(hipHostMalloc((void**)&g.host_io, 2 * sizeof(float), hipHostMallocDefault));
-    // bf16 copies and biases in one allocation, 256-byte aligned pieces
-    const size_t sizes[14] = {2 * G_HID * G_IN, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_OUT_PAD * G_HID,
-                              2 * G_IN * G_HID, 2 * G_HID * G_HID, 2 * G_HID * G_HID, 2 * G_HID * 64,
-                              4 * G_HID, 4 * G_HID, 4 * G_HID, 4 * G_OUT_PAD, sizeof(GuideChain), sizeof(GuideBack)};
-    size_t off[14], total = 0;
-    for (int k = 0; k < 14; ++k) {
+    // the forward and backward copies in one allocation, 256-byte aligned pieces
+    const size_t sizes[2] = {sizeof(GuideChain), sizeof(GuideBack)};
+    size_t off[2], total = 0;
+    for (int k = 0; k < 2; ++k) {
         off[k] = total;
         total += (sizes[k] + 255) / 256 * 256;
     }
@@ -1451,14 +1449,8 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
         return st;
     }
     char* b = (char*)g.net_mem;
-    g.net.w0 = (__bf16*)(b + off[0]); g.net.w1 = (__bf16*)(b + off[1]);
-    g.net.w2 = (__bf16*)(b + off[2]); g.net.w3 = (__bf16*)(b + off[3]);
-    g.net.w0t = (__bf16*)(b + off[4]); g.net.w1t = (__bf16*)(b + off[5]);
-    g.net.w2t = (__bf16*)(b + off[6]); g.net.w3t = (__bf16*)(b + off[7]);
-    g.net.b0 = (float*)(b + off[8]); g.net.b1 = (float*)(b + off[9]);
-    g.net.b2 = (float*)(b + off[10]); g.net.b3 = (float*)(b + off[11]);
-    g.net.chain = (GuideChain*)(b + off[12]);
-    g.net.back = (GuideBack*)(b + off[13]);
+    g.net.chain = (GuideChain*)(b + off[0]);
+    g.net.back = (GuideBack*)(b + off[1]);
     g.ready = true;
     std::vector<float> P;
     guide_init_params(c.seed, P);
