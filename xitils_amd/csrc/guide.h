@@ -66,7 +66,7 @@ static_assert(sizeof(GuideSample) == 64, "GuideSample is read as four float4");
 __global__ void k_guide_prepare(const float* P, GuideNet net);
 __global__ void k_guide_forward(GuideNet net, GuideBounds gb, const float* qin, uint32_t n, float* out);
 __global__ void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue q, const uint32_t* count,
-                              const float4* hits, float4* lobes4, float2* lobes2, uint32_t stride, uint32_t max_vertex);
+                              const float4* hits, uint2* lobes_mk, float2* lobes2, uint32_t stride, uint32_t max_vertex);
 __global__ void k_guide_backprop(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
                                  float weight_clip, __bf16* T, uint32_t ld);
 __global__ void k_guide_wgrad(const __bf16* T, uint32_t ld, uint32_t n, uint32_t chunk, float* partial);

@@ -248,11 +248,11 @@ __global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, GuideBounds
 
 // Decoded mixtures for the extension-queue entries of one bounce that the
 // shade kernel may guide (hit, not emissive, not a delta material):
-// lobes4[j * stride + i] = (mu, kappa), lobes2[j * stride + i] =
-// (alpha, alpha * vmf_norm(kappa)) of lobe j.
+// lobes_mk[j * stride + i] = (mu, kappa) as halves, lobes2[j * stride + i] =
+// (alpha, alpha * vmf_norm(kappa)) of lobe j, kappa as stored.
 __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue pq,
                                                      const uint32_t* __restrict__ count, const float4* __restrict__ hits,
-                                                     float4* __restrict__ lobes4, float2* __restrict__ lobes2,
+                                                     uint2* __restrict__ lobes_mk, float2* __restrict__ lobes2,
                                                      uint32_t stride, uint32_t max_vertex) {
     __shared__ __attribute__((aligned(16))) GuideChain s;
     load_chain(net.chain, s);
@@ -313,8 +313,9 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
                     const float kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
                     const float alpha = (k == 0 ? ea : eb) / sum;
                     const size_t o = (size_t)(2 * g + k) * stride + q;
-                    lobes4[o] = make_float4(mu.x, mu.y, mu.z, kappa);
-                    lobes2[o] = make_float2(alpha, alpha * vmf_norm(kappa));
+                    const uint2 mk2 = make_uint2(pack_half2(mu.x, mu.y), pack_half2(mu.z, kappa));
+                    lobes_mk[o] = mk2;
+                    lobes2[o] = make_float2(alpha, alpha * vmf_norm(half_hi(mk2.y)));
                 }
             }
         }

@@ -95,12 +95,25 @@ struct TrainRec {
 constexpr uint32_t TREC_GUIDED = 1u << 31;
 
 // Decoded mixture per path-queue entry, SoA per lobe j:
-// lobes4[j * stride + i] = (mu, kappa), lobes2[j * stride + i] =
-// (alpha, alpha * vmf_norm(kappa)); 192 bytes per entry.
-constexpr int G_LOBE_FIELDS = 6 * G_LOBES;
+// lobes_mk[j * stride + i] = (mu, kappa) as four halves, lobes2[j * stride +
+// i] = (alpha, alpha * vmf_norm(kappa)) in fp32 with kappa the stored half
+// value; 128 bytes per entry.  Readers renormalise mu, so sampling and pdf
+// see the same distribution.
+constexpr int G_LOBE_FIELDS = 4 * G_LOBES;
+__host__ __device__ __forceinline__ uint32_t pack_half2(float a, float b) {
+    const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
+__host__ __device__ __forceinline__ float half_lo(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xFFFFu)); }
+__host__ __device__ __forceinline__ float half_hi(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16)); }
+// (normalised mu, kappa) of one stored lobe
+__host__ __device__ __forceinline__ float4 unpack_lobe(uint2 v) {
+    const f3 mu = normalize(mk(half_lo(v.x), half_hi(v.x), half_lo(v.y)));
+    return make_float4(mu.x, mu.y, mu.z, half_hi(v.y));
+}
 
 struct GuideRender {
-    const float4* lobes4;  // null: guiding off
+    const uint2* lobes_mk; // null: guiding off
     const float2* lobes2;
     uint32_t stride;
     float prob;            // probability of sampling the guide (one-sample MIS with the BSDF)

@@ -302,10 +302,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene s
                     // Guided vertex: one-sample MIS of the learned mixture (probability
                     // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
                     // pdf wherever the BSDF pdf is positive, and zero elsewhere.
-                    guided = GUIDE && gr.lobes4 != nullptr && !mat_is_delta(mat.type) &&
+                    guided = GUIDE && gr.lobes_mk != nullptr && !mat_is_delta(mat.type) &&
                              (gr.max_vertex == 0 || v <= gr.max_vertex);
                     const WalkKey wk{key, rng.pixel, rng.sample, v};
-                    auto l4 = [&](int j) { return gr.lobes4[(size_t)j * gr.stride + i]; };
+                    auto l4 = [&](int j) { return unpack_lobe(gr.lobes_mk[(size_t)j * gr.stride + i]); };
                     auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
                     // (a NaN BSDF pdf -- GGX D at a grazing half vector -- stays NaN, so
                     // the light-sample MIS below drops that sample as the plain path does)

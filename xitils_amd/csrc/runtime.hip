@@ -976,14 +976,14 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     a = tm.mark();
                     const uint32_t ib = std::max(1u, std::min((live + GI_BLOCK - 1) / GI_BLOCK,
                                                               (uint32_t)s->dev->num_cu * 4));
-                    float4* l4 = (float4*)rb.lobes;
-                    float2* l2 = (float2*)(rb.lobes + (size_t)4 * G_LOBES * rb.capacity);
+                    uint2* l4 = (uint2*)rb.lobes;
+                    float2* l2 = (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity);
                     hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
                                        rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
                     XT_LAUNCHED(result);
                     tm.add(4, a, tm.mark());
                     s->stats.guide_queries += live;
-                    gr.lobes4 = l4;
+                    gr.lobes_mk = l4;
                     gr.lobes2 = l2;
                     gr.stride = rb.capacity;
                     gr.prob = g.cfg.guide_prob;
@@ -1157,8 +1157,8 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
     const uint32_t sblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((maxpaths + SHADE_BLOCK - 1) / SHADE_BLOCK, s->shade_grid));
     const uint32_t tblocks = trace_blocks(s, maxpaths);
     GuideState& g = s->guide;
-    float4* l4 = guide ? (float4*)rb.lobes : nullptr;
-    float2* l2 = guide ? (float2*)(rb.lobes + (size_t)4 * G_LOBES * rb.capacity) : nullptr;
+    uint2* l4 = guide ? (uint2*)rb.lobes : nullptr;
+    float2* l2 = guide ? (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity) : nullptr;
     const uint32_t ib = std::max(1u, std::min((maxpaths + GI_BLOCK - 1) / GI_BLOCK, (uint32_t)s->dev->num_cu * 4));
     XT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, s->stream, wp, rb.q[0], rb.radiance);
@@ -1173,7 +1173,7 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
         if (guide) {
             hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, rb.q[cur],
                                rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
-            gr.lobes4 = l4;
+            gr.lobes_mk = l4;
             gr.lobes2 = l2;
             gr.stride = rb.capacity;
             gr.prob = g.cfg.guide_prob;
