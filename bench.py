@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-NODE_BYTES, TRI_BYTES = 128, 48
+TRI_BYTES = 48                 # one triangle record; a node's size comes from the scene (bytes_nodes / nodes)
 EXT_RAY_IO_BYTES = 32 + 16     # ray in (o, d) + hit out
 SHADOW_RAY_IO_BYTES = 48 + 16  # shadow record in + radiance update
 SHADE_NMAP_BYTES_PER_PATH = 64 + 16 + 48 + 48 + 96 + 32 + 64 + 64 + 48 + 32
@@ -439,7 +439,8 @@ def main():
     rays = agg["extension_rays"] + agg["shadow_rays"]
     # roofline of the closest-hit traversal (dominant kernel)
     n_ext = cst["extension_rays"]
-    bytes_ext = n_ext * EXT_RAY_IO_BYTES + cst["ext_node_fetches"] * NODE_BYTES + cst["ext_tri_fetches"] * TRI_BYTES
+    node_bytes = info["bytes_nodes"] // max(1, info["bvh_nodes"])
+    bytes_ext = n_ext * EXT_RAY_IO_BYTES + cst["ext_node_fetches"] * node_bytes + cst["ext_tri_fetches"] * TRI_BYTES
     per_step_ext_bytes = bytes_ext
     launches = agg["extend_launches"] / args.steps
     achieved = per_step_ext_bytes / (agg["ms_extend"] / args.steps * 1e-3) / 1e9

@@ -1,4 +1,4 @@
-// Host-side structural checks of the BVH4 builder (xitils_amd/csrc/bvh.cpp):
+// Host-side structural checks of the BVH builder (xitils_amd/csrc/bvh.cpp):
 // every triangle lands in exactly one leaf, every child box contains its
 // subtree, leaf sizes respect the 3-bit count, and the reported depth bounds
 // the traversal stack.  Prints "OK <nodes> <depth>" or the first violation.
@@ -14,7 +14,7 @@
 using namespace xt;
 
 struct Ctx {
-    const BVH4Result& b;
+    const BVHResult& b;
     const std::vector<BuildTri>& t;
     std::vector<int> seen;
     int max_level = 0;
@@ -39,12 +39,13 @@ static int walk(Ctx& c, int code, const float* lo, const float* hi, int level) {
         return 0;
     }
     if (level > c.max_level) c.max_level = level;
-    const float* n = &c.b.nodes[(size_t)code * 32];
-    for (int k = 0; k < 4; ++k) {
-        float clo[3] = {n[0 * 4 + k], n[2 * 4 + k], n[4 * 4 + k]};
-        float chi[3] = {n[1 * 4 + k], n[3 * 4 + k], n[5 * 4 + k]};
+    const int W = BVH_WIDTH;
+    const float* n = &c.b.nodes[(size_t)code * BVH_NODE_FLOATS];
+    for (int k = 0; k < W; ++k) {
+        float clo[3] = {n[0 * W + k], n[2 * W + k], n[4 * W + k]};
+        float chi[3] = {n[1 * W + k], n[3 * W + k], n[5 * W + k]};
         int ch;
-        memcpy(&ch, &n[6 * 4 + k], 4);
+        memcpy(&ch, &n[6 * W + k], 4);
         if (clo[0] > chi[0]) continue;  // empty slot
         if (lo && (!inside(lo, hi, clo) || !inside(lo, hi, chi))) { printf("FAIL child box\n"); return 1; }
         if (walk(c, ch, clo, chi, level + 1)) return 1;
@@ -66,7 +67,7 @@ int main(int argc, char** argv) {
             for (int k = 0; k < 3; ++k) t[i].p[v][k] = c[k] + (mode == 1 ? 0.0f : 0.01f * (U(rng) - 0.5f));
     }
     BVHBuildOptions opt;
-    BVH4Result b = build_bvh4(t, opt);
+    BVHResult b = build_bvh(t, opt);
     Ctx c{b, t, std::vector<int>(n, 0)};
     if ((int)b.slot_prim.size() != n) { printf("FAIL slot count\n"); return 1; }
     if (walk(c, 0, nullptr, nullptr, 1)) return 1;

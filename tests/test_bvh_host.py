@@ -1,4 +1,4 @@
-"""Host BVH4 builder invariants (no GPU): compiled with g++ from the product's
+"""Host BVH builder invariants (no GPU): compiled with g++ from the product's
 bvh.cpp and run on random, degenerate and flat triangle sets."""
 import os
 import subprocess
@@ -24,5 +24,5 @@ def test_bvh_invariants(exe, n, mode):
     r = subprocess.run([exe, str(n), str(mode)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
     nodes, depth = map(int, r.stdout.split()[1:3])
-    # the traversal stack bound used by the kernels: 3 * depth + 1 entries
+    # the traversal stack bound used by the kernels: (width - 1) * depth + 1 entries
     assert depth >= 1 and nodes >= 1

@@ -1,4 +1,4 @@
-// Host binned-SAH BVH build, collapsed to BVH4 (see bvh.h).
+// Host binned-SAH BVH build, collapsed to a BVH_WIDTH-wide tree (see bvh.h).
 #include "bvh.h"
 
 #include <algorithm>
@@ -150,17 +150,18 @@ struct Builder {
 struct Collapser {
     const std::vector<BNode>& bn;
     const std::vector<uint32_t>& idx;
-    BVH4Result& out;
+    BVHResult& out;
     uint32_t depth = 0;
 
     static void set_child(float* node, int k, const Box& b, int code) {
-        node[0 * 4 + k] = b.lo[0];
-        node[1 * 4 + k] = b.hi[0];
-        node[2 * 4 + k] = b.lo[1];
-        node[3 * 4 + k] = b.hi[1];
-        node[4 * 4 + k] = b.lo[2];
-        node[5 * 4 + k] = b.hi[2];
-        std::memcpy(&node[6 * 4 + k], &code, 4);
+        constexpr int W = BVH_WIDTH;
+        node[0 * W + k] = b.lo[0];
+        node[1 * W + k] = b.hi[0];
+        node[2 * W + k] = b.lo[1];
+        node[3 * W + k] = b.hi[1];
+        node[4 * W + k] = b.lo[2];
+        node[5 * W + k] = b.hi[2];
+        std::memcpy(&node[6 * W + k], &code, 4);
     }
 
     int leaf(const BNode& n) {
@@ -170,18 +171,18 @@ struct Collapser {
         return ~(int)((first << 3) | (n.count - 1));
     }
 
-    // Emits a BVH4 node whose children are the binary subtrees `kids`.
+    // Emits a node whose children are the binary subtrees `kids`.
     uint32_t emit(std::vector<int> kids, uint32_t level) {
         depth = std::max(depth, level);
         uint32_t me = out.num_nodes++;
-        out.nodes.resize((size_t)out.num_nodes * 32);
+        out.nodes.resize((size_t)out.num_nodes * BVH_NODE_FLOATS);
         {
-            float* node = &out.nodes[(size_t)me * 32];
-            for (int k = 0; k < 4; ++k) {
+            float* node = &out.nodes[(size_t)me * BVH_NODE_FLOATS];
+            for (int k = 0; k < BVH_WIDTH; ++k) {
                 Box empty;
                 set_child(node, k, empty, 0);
             }
-            std::memset(&node[7 * 4], 0, 16);
+            std::memset(&node[7 * BVH_WIDTH], 0, 4 * BVH_WIDTH);
         }
         for (size_t k = 0; k < kids.size(); ++k) {
             const BNode& c = bn[kids[k]];
@@ -193,14 +194,15 @@ struct Collapser {
                 expand(g);
                 code = (int)emit(g, level + 1);
             }
-            set_child(&out.nodes[(size_t)me * 32], (int)k, c.box, code);
+            set_child(&out.nodes[(size_t)me * BVH_NODE_FLOATS], (int)k, c.box, code);
         }
         return me;
     }
 
-    // Replaces the largest-area internal child by its two children until four.
+    // Replaces the largest-area internal child by its two children until
+    // the node is full.
     void expand(std::vector<int>& g) {
-        while (g.size() < 4) {
+        while (g.size() < (size_t)BVH_WIDTH) {
             int best = -1;
             float ba = -1.0f;
             for (size_t k = 0; k < g.size(); ++k) {
@@ -220,8 +222,8 @@ struct Collapser {
 
 }  // namespace
 
-BVH4Result build_bvh4(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt) {
-    BVH4Result out;
+BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt) {
+    BVHResult out;
     const uint32_t n = (uint32_t)tris.size();
     Builder bld(opt);
     int hw = opt.threads > 0 ? opt.threads : (int)std::max(1u, std::thread::hardware_concurrency());
@@ -237,9 +239,9 @@ BVH4Result build_bvh4(const std::vector<BuildTri>& tris, const BVHBuildOptions& 
         bld.idx[i] = i;
     }
     if (n == 0) {
-        out.nodes.assign(32, 0.0f);
+        out.nodes.assign(BVH_NODE_FLOATS, 0.0f);
         float* node = out.nodes.data();
-        for (int k = 0; k < 4; ++k) Collapser::set_child(node, k, Box(), 0);
+        for (int k = 0; k < BVH_WIDTH; ++k) Collapser::set_child(node, k, Box(), 0);
         out.num_nodes = 1;
         out.depth = 1;
         return out;

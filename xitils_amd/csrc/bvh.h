@@ -1,5 +1,5 @@
-// Host BVH builder: binned SAH binary tree collapsed to a 4-wide BVH with
-// 128-byte nodes and leaf-ordered triangles (layout in xt_device.h).
+// Host BVH builder: binned SAH binary tree collapsed to an XT_BVH_WIDTH-wide
+// BVH (bvh_layout.h) with leaf-ordered triangles (layout in xt_device.h).
 //
 // Replaces the reference's build (AccelerationStructure.h:163-291: sort +
 // 8 equal-count buckets, "SAH" on 2xyz, one primitive per leaf).  Hit results
@@ -10,17 +10,21 @@
 #include <cstdint>
 #include <vector>
 
+#include "bvh_layout.h"
+
 namespace xt {
 
 struct BuildTri {
     float p[3][3];   // world-space vertices
 };
 
-struct BVH4Result {
-    std::vector<float> nodes;          // NODE_F4 * 4 floats per node
+constexpr int BVH_NODE_FLOATS = 8 * BVH_WIDTH;   // 8 rows of BVH_WIDTH floats
+
+struct BVHResult {
+    std::vector<float> nodes;          // BVH_NODE_FLOATS per node
     std::vector<uint32_t> slot_prim;   // leaf slot -> prim index (input order)
     uint32_t num_nodes = 0;
-    uint32_t depth = 0;                // levels of BVH4 nodes (root = 1)
+    uint32_t depth = 0;                // levels of nodes (root = 1)
     uint32_t max_leaf = 0;
 };
 
@@ -31,6 +35,6 @@ struct BVHBuildOptions {
     int threads = 0;           // 0 = hardware concurrency
 };
 
-BVH4Result build_bvh4(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt);
+BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt);
 
 }  // namespace xt

@@ -1,5 +1,5 @@
 // Host runtime behind include/xitils_amd.h: device handles, scene upload
-// (instance flattening, light tables, BVH4 build), the ray-query entry points
+// (instance flattening, light tables, BVH build), the ray-query entry points
 // and the wavefront render loop.
 #include <hip/hip_runtime.h>
 
@@ -155,7 +155,9 @@ static ShadeKernel shade_kernel(bool guide, bool nmap) {
 
 extern "C" {
 
-const char* xt_version(void) { return "xitils_amd 0.1.0 (gfx950, BVH4 wavefront)"; }
+#define XT_STR2(x) #x
+#define XT_STR(x) XT_STR2(x)
+const char* xt_version(void) { return "xitils_amd 0.1.0 (gfx950, BVH" XT_STR(XT_BVH_WIDTH) " wavefront)"; }
 const char* xt_last_error(void) { return g_last_error.c_str(); }
 
 xt_status xt_device_open(int ordinal, xt_device** out) {
@@ -385,7 +387,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     }
     // ---- BVH
     BVHBuildOptions bopt;
-    BVH4Result bvh = build_bvh4(bt, bopt);
+    BVHResult bvh = build_bvh(bt, bopt);
     std::vector<uint32_t> prim_slot(nprim);
     for (uint32_t s = 0; s < nprim; ++s) prim_slot[bvh.slot_prim[s]] = s;
     // ---- leaf-ordered triangle records
@@ -522,7 +524,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
         XT_HIP_S(hipMemcpy(s->d_light_slots, lslots.data(), lslots.size() * 4, hipMemcpyHostToDevice));
     }
     // traversal stack: at most 3 pushes per level plus one
-    const uint32_t stack_bound = 3 * bvh.depth + 1;
+    const uint32_t stack_bound = (BVH_WIDTH - 1) * bvh.depth + 1;
     s->ovf_depth = stack_bound > (uint32_t)TRACE_LDS_DEPTH ? stack_bound - TRACE_LDS_DEPTH : 0;
     s->trace_grid_max = (uint32_t)dev->num_cu * 8;
     s->trace_grid = s->trace_grid_max;
@@ -583,7 +585,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     }
     s->bvh_nodes = bvh.num_nodes;
     s->bvh_depth = bvh.depth;
-    s->bytes_nodes = (uint64_t)bvh.num_nodes * 128;
+    s->bytes_nodes = (uint64_t)bvh.num_nodes * NODE_F4 * 16;
     s->bytes_tris = (uint64_t)nprim * 48;
     for (int a = 0; a < 3; ++a) {
         s->bounds_lo[a] = INFINITY;

@@ -2,6 +2,7 @@
 // shared by the gfx950 kernels (and host code that prepares the tables).
 #pragma once
 
+#include "bvh_layout.h"
 #include "xt_math.h"
 #include "../../include/xitils_amd.h"
 
@@ -9,10 +10,11 @@ namespace xt {
 
 // ------------------------------------------------------------------ HBM layout
 //
-// BVH4 node: 128 bytes (one L2 line on gfx950), 8 float4:
-//   [0] lo.x[4]  [1] hi.x[4]  [2] lo.y[4]  [3] hi.y[4]  [4] lo.z[4]  [5] hi.z[4]
-//   [6] child[4] as int: >= 0 internal node index; < 0 leaf ~(first << 3 | (count - 1))
-//   [7] reserved
+// BVH node (W = BVH_WIDTH children, bvh_layout.h): 8 rows of W floats, 32 W
+// bytes (256 bytes, two L2 lines, for W = 8):
+//   rows lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W],
+//   child[W] as int: >= 0 internal node index; < 0 leaf ~(first << 3 | (count - 1)),
+//   reserved
 // Empty child slots have lo = +inf, hi = -inf and never pass the slab test.
 //
 // Triangles, in leaf order ("slots"), 3 float4 = 48 bytes each:
@@ -22,7 +24,8 @@ namespace xt {
 // map), 6 float4: (tangent k.xyz, uv) for k = 0..2 then (bitangent k.xyz, uv):
 //   [0] t0, u0  [1] t1, v0  [2] t2, u1  [3] b0, v1  [4] b1, u2  [5] b2, v2
 // Flags bits 8..31 hold 1 + the normal map's texture index (0 = none).
-constexpr int NODE_F4 = 8;
+constexpr int NODE_ROW_F4 = BVH_WIDTH / 4;     // float4 per row
+constexpr int NODE_F4 = 8 * NODE_ROW_F4;
 constexpr int TRI_F4 = 3;
 constexpr int NMAP_F4 = 6;
 constexpr uint32_t TRI_FLAG_NORMALS = 1u;

@@ -1,10 +1,11 @@
-// BVH4 traversal and the watertight triangle test on gfx950.
+// BVH traversal (BVH_WIDTH-wide nodes) and the watertight triangle test on
+// gfx950.
 //
 // One lane per ray, wave64.  The traversal stack lives in LDS
 // (stack[level][lane], conflict-free) for the first LDS_DEPTH entries and
 // spills to a per-lane global column beyond that; the host sizes the column
-// from the built tree (3 * depth + 1 entries bound any BVH4 traversal), so no
-// push can leave its buffer.
+// from the built tree ((W - 1) * depth + 1 entries bound any traversal of a
+// W-wide tree), so no push can leave its buffer.
 #pragma once
 
 #include "xt_device.h"
@@ -94,21 +95,34 @@ struct TraceResult {
 };
 
 
-// Four child slab tests; returns entry distances (inf = miss).  The near and
-// far planes of each axis are picked by address (the ray's direction signs),
-// so no per-child selects are needed.
-__device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const RayPre& r, float cull, float tn[4]) {
-    float4 nx = nd[r.px], fx = nd[r.px ^ 1];
-    float4 ny = nd[r.py], fy = nd[r.py ^ 1];
-    float4 nz = nd[r.pz], fz = nd[r.pz ^ 1];
-    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
-    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
-    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+// One float of row `row` (0..7) for child k of a node.
+__device__ __forceinline__ float node_row(const float4* nd, int row, int k) {
+    const float4 v = nd[row * NODE_ROW_F4 + (k >> 2)];
+    const int c = k & 3;
+    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+
+// The node's child slab tests; returns entry distances (inf = miss).  The
+// near and far planes of each axis are picked by address (the ray's
+// direction signs), so no per-child selects are needed.
+__device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const RayPre& r, float cull,
+                                           float tn[BVH_WIDTH]) {
+    float4 rows[6][NODE_ROW_F4];
+    const int pr[6] = {r.px, r.px ^ 1, r.py, r.py ^ 1, r.pz, r.pz ^ 1};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        float t0x = (nxa[k] - r.o.x) * r.ix, t1x = (fxa[k] - r.o.x) * r.ix;
-        float t0y = (nya[k] - r.o.y) * r.iy, t1y = (fya[k] - r.o.y) * r.iy;
-        float t0z = (nza[k] - r.o.z) * r.iz, t1z = (fza[k] - r.o.z) * r.iz;
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+        for (int h = 0; h < NODE_ROW_F4; ++h) rows[q][h] = nd[pr[q] * NODE_ROW_F4 + h];
+#pragma unroll
+    for (int k = 0; k < BVH_WIDTH; ++k) {
+        auto at = [&](int q) {
+            const float4 v = rows[q][k >> 2];
+            const int c = k & 3;
+            return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+        };
+        float t0x = (at(0) - r.o.x) * r.ix, t1x = (at(1) - r.o.x) * r.ix;
+        float t0y = (at(2) - r.o.y) * r.iy, t1y = (at(3) - r.o.y) * r.iy;
+        float t0z = (at(4) - r.o.z) * r.iz, t1z = (at(5) - r.o.z) * r.iz;
         float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         float tmax = fminf(fminf(t1x, t1y), fminf(t1z, cull)) * BOX_MARGIN;
         tn[k] = tmin <= tmax ? tmin : INFINITY;
@@ -153,23 +167,40 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, T
     if (t.node >= 0) {
         const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
         if (COUNT) vc.nodes++;
-        float tn[4];
+        float tn[BVH_WIDTH];
         node_slabs(nd, r, t.cull, tn);
-        float4 chf = nd[6];
-        int ch[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z), __float_as_int(chf.w)};
+        int ch[BVH_WIDTH];
+#pragma unroll
+        for (int k = 0; k < BVH_WIDTH; ++k) ch[k] = __float_as_int(node_row(nd, 6, k));
 #define XT_CSWAP(i, j)                                   \
     if (tn[j] < tn[i]) {                                 \
         float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;     \
         int cc = ch[i]; ch[i] = ch[j]; ch[j] = cc;       \
     }
-        XT_CSWAP(0, 1) XT_CSWAP(2, 3) XT_CSWAP(0, 2) XT_CSWAP(1, 3) XT_CSWAP(1, 2)
+        static_assert(BVH_WIDTH == 4 || BVH_WIDTH == 8, "nodes are 4 or 8 wide");
+        if constexpr (BVH_WIDTH == 4) {
+            XT_CSWAP(0, 1) XT_CSWAP(2, 3) XT_CSWAP(0, 2) XT_CSWAP(1, 3) XT_CSWAP(1, 2)
+        } else {   // Batcher's odd-even merge sort for 8 keys (19 comparators)
+            XT_CSWAP(0, 1) XT_CSWAP(2, 3) XT_CSWAP(4, 5) XT_CSWAP(6, 7)
+            XT_CSWAP(0, 2) XT_CSWAP(1, 3) XT_CSWAP(4, 6) XT_CSWAP(5, 7)
+            XT_CSWAP(1, 2) XT_CSWAP(5, 6)
+            XT_CSWAP(0, 4) XT_CSWAP(1, 5) XT_CSWAP(2, 6) XT_CSWAP(3, 7)
+            XT_CSWAP(2, 4) XT_CSWAP(3, 5)
+            XT_CSWAP(1, 2) XT_CSWAP(3, 4) XT_CSWAP(5, 6)
+        }
 #undef XT_CSWAP
-        int nhit = (tn[0] < INFINITY) + (tn[1] < INFINITY) + (tn[2] < INFINITY) + (tn[3] < INFINITY);
+        int nhit = 0;
+#pragma unroll
+        for (int k = 0; k < BVH_WIDTH; ++k) nhit += tn[k] < INFINITY;
         if (nhit > 0) {
-            for (int k = nhit - 1; k >= 1; --k) {
-                if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
-                else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
-                ++t.sp;
+            // far children first; static indices keep ch[] in registers
+#pragma unroll
+            for (int k = BVH_WIDTH - 1; k >= 1; --k) {
+                if (k < nhit) {
+                    if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
+                    else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
+                    ++t.sp;
+                }
             }
             t.node = ch[0];
             return false;
