@@ -29,6 +29,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 TRI_BYTES = 48                 # one triangle record; a node's size comes from the scene (bytes_nodes / nodes)
 EXT_RAY_IO_BYTES = 32 + 16     # ray in (o, d) + hit out
 SHADOW_RAY_IO_BYTES = 48 + 16  # shadow record in + radiance update
+QUERY_IO_BYTES = {"closest": 32 + 16, "any": 32 + 1}   # xt_ray in + xt_hit / occlusion byte out
 SHADE_NMAP_BYTES_PER_PATH = 64 + 16 + 48 + 48 + 96 + 32 + 64 + 64 + 48 + 32
 
 
@@ -48,7 +49,7 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch", "nmap"], default="heightfield",
+    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch", "nmap", "rays"], default="heightfield",
                     help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room; "
                          "batch: 64 small scenes rendered one by one and as one graph-replayed batch; "
                          "nmap: the displaced plane with microfacet-based normal mapping, 16 camera variants")
@@ -370,6 +371,127 @@ def nmap_main(args):
     print(json.dumps(out), flush=True)
 
 
+def rays_main(args):
+    """Ray-query microbenchmark on the 1024^2-cell height field: one batch
+    each of 1080p primary rays, cosine-weighted diffuse bounce rays from
+    random surface points and shadow rays into the sun's cone (seed 7,
+    2,073,600 rays each), resident in HBM, through xt_trace_closest
+    (primary, diffuse) and xt_trace_any (shadow).  A step traces all three
+    batches; value = rays / summed kernel time (HIP events on the library's
+    stream)."""
+    import numpy as np
+    import torch
+    from xitils_amd import api, scenes
+    W, H = args.width, args.height
+    log(f"[bench] building heightfield {args.cells}^2 cells")
+    desc = scenes.heightfield(cells=args.cells, width=W, height=H)
+    dev = api.Device(0)
+    scene = api.Scene(dev, desc)
+    info = scene.info()
+    node_bytes = info["bytes_nodes"] // max(1, info["bvh_nodes"])
+    log(f"[bench] scene {info}")
+    batches, per_kind = {}, {}
+    for kind in scenes.RAY_BATCH_KINDS:
+        rays = scenes.ray_batch(desc, kind, W, H)
+        batches[kind] = rays
+        n = rays.shape[0]
+        q = "any" if kind == "shadow" else "closest"
+        d_rays = torch.from_numpy(rays).to("cuda:0")
+        out = torch.empty(n, dtype=torch.uint8, device="cuda:0") if q == "any" else \
+            torch.empty((n, 4), dtype=torch.float32, device="cuda:0")
+        call = scene.trace_any_dev if q == "any" else scene.trace_closest_dev
+        torch.cuda.synchronize()
+        scene.set_counting(True)
+        call(d_rays.data_ptr(), n, out.data_ptr())
+        cst = scene.stats()
+        scene.set_counting(False)
+        for _ in range(max(1, args.warmup)):
+            call(d_rays.data_ptr(), n, out.data_ptr())
+        key = "ms_shadow" if q == "any" else "ms_extend"
+        ms = []
+        for _ in range(args.steps):
+            call(d_rays.data_ptr(), n, out.data_ptr())
+            ms.append(scene.stats()[key])
+        avg = float(np.mean(ms))
+        nodes = cst["shadow_node_fetches" if q == "any" else "ext_node_fetches"]
+        tris = cst["shadow_tri_fetches" if q == "any" else "ext_tri_fetches"]
+        algo = n * QUERY_IO_BYTES[q] + nodes * node_bytes + tris * TRI_BYTES
+        res = out.cpu().numpy()
+        hit = float(res.mean()) if q == "any" else float(np.mean(res[:, 1].view(np.uint32) != 0xFFFFFFFF))
+        per_kind[kind] = {
+            "query": q, "rays": n, "avg_kernel_ms": round(avg, 4), "mrays_per_s": round(n / avg / 1e3, 1),
+            "node_fetches_per_ray": round(nodes / n, 3), "tri_fetches_per_ray": round(tris / n, 3),
+            "hit_fraction": round(hit, 4), "algorithmic_bytes_per_ray": round(algo / n, 1),
+            "achieved_gbs": round(algo / (avg * 1e-3) / 1e9, 1),
+        }
+        log(f"[bench] {kind}: {per_kind[kind]}")
+        del d_rays, out
+    total_ms = sum(v["avg_kernel_ms"] for v in per_kind.values())
+    total_rays = sum(v["rays"] for v in per_kind.values())
+    dom_kind = max(per_kind, key=lambda k: per_kind[k]["avg_kernel_ms"])
+    dom = per_kind[dom_kind]
+    roofline = {
+        "bound": "hbm",
+        "achieved": dom["achieved_gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(f"rays_{dom_kind}_heightfield1080p")
+        if (args.cells, W, H) == (1024, 1920, 1080) else None,
+        "kernel": f"k_trace<{'any' if dom['query'] == 'any' else 'closest'}> ({dom_kind} rays)",
+        "bytes_per_launch": round(dom["algorithmic_bytes_per_ray"] * dom["rays"]),
+        "avg_launch_ms": dom["avg_kernel_ms"],
+    }
+    out = {
+        "metric": "Mrays/s (ray-query microbenchmark: primary, diffuse, shadow)",
+        "value": round(total_rays / total_ms / 1e3, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"raybatch_heightfield{args.cells}_{W}x{H}_seed7", "triangles": info["num_prims"],
+                   "rays_per_kind": W * H, "parallelism": "1 GPU"},
+        "per_kind": per_kind,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = pyoracle.OracleScene(desc)
+        budget = args.cpu_seconds / len(batches)
+        done = secs = 0.0
+        agree = []
+        for kind, rays in batches.items():
+            fn = (lambda r: o.trace_any(r, mode=0, nthreads=threads)) if kind == "shadow" else \
+                (lambda r: o.trace_closest(r, mode=0, nthreads=threads))
+            t0 = time.perf_counter()
+            fn(rays[:20000])
+            one = (time.perf_counter() - t0) / 20000
+            m = int(min(rays.shape[0], max(20000, budget / max(one, 1e-9))))
+            t0 = time.perf_counter()
+            ref = fn(rays[:m])
+            secs += time.perf_counter() - t0
+            done += m
+            got = scene.trace_any(rays[:m]) if kind == "shadow" else scene.trace_closest(rays[:m])
+            same = np.array_equal(ref, got) if kind == "shadow" else \
+                np.array_equal(ref[:, 1].view(np.uint32), got[:, 1].view(np.uint32))
+            agree.append(f"{kind} {m} rays{'' if same else ' (MISMATCH)'}")
+        out["cpu_baseline"] = {
+            "value": round(done / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first rays of each batch ({', '.join(agree)}; GPU results identical where not marked) "
+                      f"through the oracle C++ restatement's reference-shaped binary BVH, {secs:.1f} s wall",
+        }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -382,6 +504,8 @@ def main():
         return batch_main(args)
     if args.workload == "nmap":
         return nmap_main(args)
+    if args.workload == "rays":
+        return rays_main(args)
     import numpy as np
     import torch
     from xitils_amd import api, scenes

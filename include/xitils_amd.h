@@ -171,14 +171,18 @@ typedef struct xt_render_params {
     float shadow_margin;     /* 1e-4 (PathTracer.h:257) */
 } xt_render_params;
 
-/* Counters of the last xt_render on a scene. */
+/* Counters of the last xt_render, xt_trace_closest or xt_trace_any on a
+ * scene.  After a ray query only its fields are set: extension_rays (closest)
+ * or shadow_rays (any) = n, ms_extend or ms_shadow = ms_total = the traversal
+ * kernel's time, and with counting on its node and triangle fetches. */
 typedef struct xt_render_stats {
     uint64_t camera_rays, extension_rays, shadow_rays;  /* extension_rays includes camera rays */
     double ms_total;         /* wall time of the call's GPU work (HIP events) */
     double ms_extend, ms_shade, ms_shadow, ms_other;    /* per-stage kernel time (HIP events) */
     uint32_t extend_launches;
-    /* BVH node (128 B) and triangle (48 B) fetches of the traversal kernels;
-     * filled only after xt_scene_set_counting(scene, 1). */
+    /* BVH node (bytes_nodes / bvh_nodes of xt_scene_info: 128 B for 4-wide
+     * nodes) and triangle (48 B) fetches of the traversal kernels; filled
+     * only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
     /* guiding: network inference and training kernel time, Adam steps taken,
      * training samples kept, queue entries run through the network and
@@ -209,9 +213,11 @@ xt_status xt_device_close(xt_device* dev);
 
 /* Scene::addObject x N + Scene::buildAccelerationStructure (Scene.h:17-28):
  * flattens instances to world space, builds the BVH on the host and uploads
- * everything to HBM. */
+ * everything to HBM.  Rejects (XT_ERR_INVALID_ARGUMENT) a referenced vertex
+ * whose world-space position is not finite or exceeds 1e30 in magnitude. */
 xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** out);
 xt_status xt_scene_destroy(xt_scene* scene);
+
 /* Host-side facts about the built scene: num_prims, bvh_nodes, bvh_depth,
  * bytes_nodes, bytes_tris, build_ms (build_ms as double bits). */
 xt_status xt_scene_info(const xt_scene* scene, uint64_t out[6]);
@@ -256,9 +262,10 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n);
  * out = {traversal blocks per CU, shade blocks per CU}. */
 xt_status xt_scene_autotune(xt_scene* scene, const xt_camera* camera, const xt_render_params* params, uint32_t out[2]);
 
-/* Measurement hook (no reference counterpart): renders of this scene use the
- * traversal variant that counts node and triangle fetches into the stats
- * (algorithmic-traffic accounting for the roofline).  Results are identical. */
+/* Measurement hook (no reference counterpart): renders and ray queries of
+ * this scene use the traversal variant that counts node and triangle fetches
+ * into the stats (algorithmic-traffic accounting for the roofline).  Results
+ * are identical. */
 xt_status xt_scene_set_counting(xt_scene* scene, int on);
 
 /* ---- VonMisesFisherDistribution<N> (VonMisesFisherDistribution.h:9-217) -----

@@ -1,7 +1,9 @@
 // Host-side structural checks of the BVH builder (xitils_amd/csrc/bvh.cpp):
-// every triangle lands in exactly one leaf, every child box contains its
-// subtree, leaf sizes respect the 3-bit count, and the reported depth bounds
-// the traversal stack.  Prints "OK <nodes> <depth>" or the first violation.
+// every triangle lands in exactly one leaf, every child box (as the traversal
+// decodes it, bvh_child) contains the exact bounds of its subtree's triangles
+// with the builder's guaranteed outward margin, leaf sizes respect the 3-bit
+// count, and the reported depth bounds the traversal stack.  Prints "OK
+// <nodes> <depth>" or the first violation.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,13 +22,12 @@ struct Ctx {
     int max_level = 0;
 };
 
-static bool inside(const float* lo, const float* hi, const float* p) {
-    for (int k = 0; k < 3; ++k)
-        if (p[k] < lo[k] || p[k] > hi[k]) return false;
-    return true;
-}
-
-static int walk(Ctx& c, int code, const float* lo, const float* hi, int level) {
+// Walks the subtree of `code`; writes the exact bounds of its triangles.
+static int walk(Ctx& c, int code, int level, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
     if (code < 0) {
         unsigned u = ~(unsigned)code;
         unsigned first = u >> 3, cnt = (u & 7u) + 1;
@@ -34,21 +35,28 @@ static int walk(Ctx& c, int code, const float* lo, const float* hi, int level) {
             unsigned prim = c.b.slot_prim[first + k];
             c.seen[prim]++;
             for (int v = 0; v < 3; ++v)
-                if (!inside(lo, hi, c.t[prim].p[v])) { printf("FAIL leaf box\n"); return 1; }
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::fmin(lo[a], (double)c.t[prim].p[v][a]);
+                    hi[a] = std::fmax(hi[a], (double)c.t[prim].p[v][a]);
+                }
         }
         return 0;
     }
     if (level > c.max_level) c.max_level = level;
-    const int W = BVH_WIDTH;
-    const float* n = &c.b.nodes[(size_t)code * BVH_NODE_FLOATS];
-    for (int k = 0; k < W; ++k) {
-        float clo[3] = {n[0 * W + k], n[2 * W + k], n[4 * W + k]};
-        float chi[3] = {n[1 * W + k], n[3 * W + k], n[5 * W + k]};
+    for (int k = 0; k < BVH_WIDTH; ++k) {
+        double clo[3], chi[3], m[3], slo[3], shi[3];
         int ch;
-        memcpy(&ch, &n[6 * W + k], 4);
-        if (clo[0] > chi[0]) continue;  // empty slot
-        if (lo && (!inside(lo, hi, clo) || !inside(lo, hi, chi))) { printf("FAIL child box\n"); return 1; }
-        if (walk(c, ch, clo, chi, level + 1)) return 1;
+        if (!bvh_child(c.b, (uint32_t)code, k, clo, chi, &ch, m)) continue;
+        if (walk(c, ch, level + 1, slo, shi)) return 1;
+        for (int a = 0; a < 3; ++a) {
+            if (!(clo[a] <= slo[a] - m[a] && chi[a] >= shi[a] + m[a])) {
+                printf("FAIL child box node %d slot %d axis %d: [%.9g, %.9g] margin %.3g vs [%.9g, %.9g]\n", code, k, a,
+                       clo[a], chi[a], m[a], slo[a], shi[a]);
+                return 1;
+            }
+            lo[a] = std::fmin(lo[a], slo[a]);
+            hi[a] = std::fmax(hi[a], shi[a]);
+        }
     }
     return 0;
 }
@@ -63,14 +71,19 @@ int main(int argc, char** argv) {
         float c[3] = {U(rng), U(rng), U(rng)};
         if (mode == 1) c[0] = c[1] = c[2] = 0.5f;          // all centroids equal
         if (mode == 2) c[1] = 0.0f;                        // flat sheet
+        // 3: far from the origin (the grid origin's rounding is coarse against the step)
+        // 4 / 5: huge / tiny coordinates
+        const float off = mode == 3 ? 1000.0f : 0.0f, scale = mode == 4 ? 1e20f : (mode == 5 ? 1e-20f : 1.0f);
         for (int v = 0; v < 3; ++v)
-            for (int k = 0; k < 3; ++k) t[i].p[v][k] = c[k] + (mode == 1 ? 0.0f : 0.01f * (U(rng) - 0.5f));
+            for (int k = 0; k < 3; ++k)
+                t[i].p[v][k] = off + scale * (c[k] + (mode == 1 ? 0.0f : 0.01f * (U(rng) - 0.5f)));
     }
     BVHBuildOptions opt;
     BVHResult b = build_bvh(t, opt);
     Ctx c{b, t, std::vector<int>(n, 0)};
     if ((int)b.slot_prim.size() != n) { printf("FAIL slot count\n"); return 1; }
-    if (walk(c, 0, nullptr, nullptr, 1)) return 1;
+    double rlo[3], rhi[3];
+    if (walk(c, 0, 1, rlo, rhi)) return 1;
     for (int i = 0; i < n; ++i)
         if (c.seen[i] != 1) { printf("FAIL prim %d seen %d\n", i, c.seen[i]); return 1; }
     if (c.max_level != (int)b.depth) { printf("FAIL depth %d vs %u\n", c.max_level, b.depth); return 1; }

@@ -9,17 +9,22 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.fixture(scope="module")
-def exe():
+# the default layout (quantised 4-wide nodes) and the float 4- and 8-wide ones
+LAYOUTS = {"quant4": [], "float4": ["-DXT_BVH_QUANT=0"], "float8": ["-DXT_BVH_QUANT=0", "-DXT_BVH_WIDTH=8"]}
+
+
+@pytest.fixture(scope="module", params=list(LAYOUTS))
+def exe(request):
     d = tempfile.mkdtemp()
-    out = os.path.join(d, "bvh_check")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-o", out,
+    out = os.path.join(d, "bvh_check_" + request.param)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", *LAYOUTS[request.param], "-o", out,
                     os.path.join(ROOT, "tests", "cpp", "bvh_check.cpp"),
                     os.path.join(ROOT, "xitils_amd", "csrc", "bvh.cpp")], check=True)
     return out
 
 
-@pytest.mark.parametrize("n,mode", [(1, 0), (3, 0), (5, 0), (1000, 0), (200000, 0), (5000, 1), (20000, 2)])
+@pytest.mark.parametrize("n,mode", [(0, 0), (1, 0), (3, 0), (5, 0), (1000, 0), (200000, 0), (5000, 1), (20000, 2),
+                                    (20000, 3), (5000, 4), (5000, 5)])
 def test_bvh_invariants(exe, n, mode):
     r = subprocess.run([exe, str(n), str(mode)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr

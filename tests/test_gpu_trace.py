@@ -112,6 +112,21 @@ def test_unsupported_material_is_rejected(dev):
     assert e.value.status == 2
 
 
+def test_non_finite_vertices_are_rejected(dev):
+    """Positions the BVH's quantisation grid cannot hold (not finite, or
+    beyond 1e30) fail scene creation with XT_ERR_INVALID_ARGUMENT."""
+    for bad in (np.nan, np.inf, 3e30):
+        desc = scenes.cornell()
+        obj = desc.objects[0]
+        pos = obj.positions.copy()
+        pos[obj.indices[0, 0], 1] = bad
+        obj.positions = pos
+        print(f"[gpu] scene with a vertex coordinate {bad}", flush=True)
+        with pytest.raises(api.XtError) as e:
+            api.Scene(dev, desc)
+        assert e.value.status == 1 and "finite" in str(e.value)
+
+
 def test_large_heightfield_properties(dev):
     # full 1024^2-cell heightfield (2.1 M triangles): closest hits agree with
     # brute force on a sample, and any-hit with tmax = t_hit * (1 + 2e-6) is
@@ -136,3 +151,29 @@ def test_large_heightfield_properties(dev):
     hb = o.trace_closest(rays[sel], mode=1)
     assert np.array_equal(hit_prims(h[sel]), hit_prims(hb))
     assert np.array_equal(h[sel, 0], hb[:, 0])
+
+
+def test_ray_batches_at_full_size_match_oracle(dev):
+    """The microbenchmark's three batches on the full 1024^2-cell height field
+    (2,097,152 triangles): GPU hit ids and occlusion bits equal the oracle's
+    reference-shaped BVH on every 103rd ray of each (20,133 rays); the query reports
+    its ray count, kernel time and fetch counts."""
+    desc = scenes.heightfield(cells=1024, materials=False)
+    g = api.Scene(dev, desc)
+    o = po.OracleScene(desc)
+    for kind in scenes.RAY_BATCH_KINDS:
+        full = scenes.ray_batch(desc, kind)
+        rays = np.ascontiguousarray(full[:: len(full) // 20000])   # spread over the frame / surface
+        print(f"[gpu] ray batch {kind} n={len(rays)}", flush=True)
+        g.set_counting(True)
+        if kind == "shadow":
+            got = g.trace_any(rays)
+            st = g.stats()
+            assert np.array_equal(got, o.trace_any(rays, mode=0))
+            assert st["shadow_rays"] == len(rays) and st["ms_shadow"] > 0 and st["shadow_node_fetches"] >= len(rays)
+        else:
+            got = g.trace_closest(rays)
+            st = g.stats()
+            assert np.array_equal(hit_prims(got), hit_prims(o.trace_closest(rays, mode=0)))
+            assert st["extension_rays"] == len(rays) and st["ms_extend"] > 0 and st["ext_node_fetches"] >= len(rays)
+        g.set_counting(False)

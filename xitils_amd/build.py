@@ -43,7 +43,7 @@ def build(verbose: bool = False, out_lib: str = LIB, defines=()) -> str:
         cmd = [HIPCC, "--offload-arch=gfx950", *COMMON, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, *COMMON, "-x", "c++", "-I", os.path.join(ROOT, "include"), "-c",
+            cmd = [HIPCC, *COMMON, *[f"-D{d}" for d in defines], "-x", "c++", "-I", os.path.join(ROOT, "include"), "-c",
                    os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))

@@ -175,9 +175,9 @@ struct Collapser {
     uint32_t emit(std::vector<int> kids, uint32_t level) {
         depth = std::max(depth, level);
         uint32_t me = out.num_nodes++;
-        out.nodes.resize((size_t)out.num_nodes * BVH_NODE_FLOATS);
+        out.nodes.resize((size_t)out.num_nodes * BVH_FLOAT_NODE_WORDS);
         {
-            float* node = &out.nodes[(size_t)me * BVH_NODE_FLOATS];
+            float* node = &out.nodes[(size_t)me * BVH_FLOAT_NODE_WORDS];
             for (int k = 0; k < BVH_WIDTH; ++k) {
                 Box empty;
                 set_child(node, k, empty, 0);
@@ -194,7 +194,7 @@ struct Collapser {
                 expand(g);
                 code = (int)emit(g, level + 1);
             }
-            set_child(&out.nodes[(size_t)me * BVH_NODE_FLOATS], (int)k, c.box, code);
+            set_child(&out.nodes[(size_t)me * BVH_FLOAT_NODE_WORDS], (int)k, c.box, code);
         }
         return me;
     }
@@ -220,7 +220,99 @@ struct Collapser {
     }
 };
 
+// Float node (8 rows of 4 floats) -> quantised node (bvh_layout.h).  Per
+// axis the grid step is the smallest power of two 2^e for which every valid
+// child fits in bytes with one extra step on each side: lo = floor((lo_c -
+// origin) / 2^e) - 1 >= 0, hi = ceil((hi_c - origin) / 2^e) + 1 <= 255, origin =
+// the node's lower bound minus one step, rounded down to a float.  The step
+// is at least 2^-22 of the node's magnitude, so the origin's rounding stays
+// below half a step.
+void quantize_node(const float* f, uint32_t* q) {
+    constexpr int W = 4;
+    int nvalid = 0;
+    for (int k = 0; k < W; ++k)
+        if (f[k] <= f[W + k]) nvalid = k + 1;   // valid slots come first (Collapser::emit)
+    std::memset(q, 0, 16 * sizeof(uint32_t));
+    uint32_t meta = (uint32_t)nvalid << 24;
+    for (int a = 0; a < 3; ++a) {
+        const float* lo_r = f + (2 * a) * W;
+        const float* hi_r = f + (2 * a + 1) * W;
+        double lo = 0.0, hi = 0.0;
+        if (nvalid > 0) {
+            lo = lo_r[0];
+            hi = hi_r[0];
+            for (int k = 1; k < nvalid; ++k) {
+                lo = std::min(lo, (double)lo_r[k]);
+                hi = std::max(hi, (double)hi_r[k]);
+            }
+        }
+        const double mag = std::max(std::fabs(lo), std::fabs(hi));
+        int e = std::max(-126, (mag > 0.0 ? std::ilogb(mag) : -126) - 22);
+        if (hi - lo > 0.0) e = std::max(e, (int)std::ceil(std::log2((hi - lo) / 250.0)));
+        float org = 0.0f;
+        uint32_t lo_b = 0, hi_b = 0;
+        for (; e <= 127; ++e) {
+            const double step = std::ldexp(1.0, e);
+            org = (float)(lo - step);
+            if ((double)org > lo - step) org = std::nextafter(org, -INFINITY);
+            bool fits = true;
+            lo_b = hi_b = 0;
+            for (int k = 0; k < W; ++k) {
+                uint32_t ql = 255, qh = 0;   // empty slot: lo > hi
+                if (k < nvalid) {
+                    const double l = std::floor(((double)lo_r[k] - (double)org) / step) - 1.0;
+                    const double h = std::ceil(((double)hi_r[k] - (double)org) / step) + 1.0;
+                    if (!(l >= 0.0 && h <= 255.0)) {
+                        fits = false;
+                        break;
+                    }
+                    ql = (uint32_t)l;
+                    qh = (uint32_t)h;
+                }
+                lo_b |= ql << (8 * k);
+                hi_b |= qh << (8 * k);
+            }
+            if (fits) break;
+        }
+        std::memcpy(&q[a], &org, 4);
+        meta |= (uint32_t)(uint8_t)(int8_t)e << (8 * a);
+        q[8 + 2 * a] = lo_b;
+        q[9 + 2 * a] = hi_b;
+    }
+    q[3] = meta;
+    for (int k = 0; k < W; ++k) std::memcpy(&q[4 + k], &f[6 * W + k], 4);
+}
+
 }  // namespace
+
+bool bvh_child(const BVHResult& b, uint32_t node, int k, double lo[3], double hi[3], int* code, double margin[3]) {
+    const float* f = &b.nodes[(size_t)node * BVH_NODE_WORDS];
+    if constexpr (BVH_QUANT) {
+        uint32_t w[16];
+        std::memcpy(w, f, sizeof w);
+        if (k >= (int)(w[3] >> 24)) return false;
+        for (int a = 0; a < 3; ++a) {
+            float org;
+            std::memcpy(&org, &w[a], 4);
+            const int e = (int)(int8_t)(uint8_t)(w[3] >> (8 * a));
+            lo[a] = (double)org + std::ldexp((double)((w[8 + 2 * a] >> (8 * k)) & 255u), e);
+            hi[a] = (double)org + std::ldexp((double)((w[9 + 2 * a] >> (8 * k)) & 255u), e);
+            if (margin) margin[a] = std::ldexp(1.0, e);
+        }
+        std::memcpy(code, &w[4 + k], 4);
+        return true;
+    } else {
+        constexpr int W = BVH_WIDTH;
+        if (!(f[k] <= f[W + k])) return false;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = f[(2 * a) * W + k];
+            hi[a] = f[(2 * a + 1) * W + k];
+            if (margin) margin[a] = 0.0;
+        }
+        std::memcpy(code, &f[6 * W + k], 4);
+        return true;
+    }
+}
 
 BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt) {
     BVHResult out;
@@ -239,11 +331,16 @@ BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& op
         bld.idx[i] = i;
     }
     if (n == 0) {
-        out.nodes.assign(BVH_NODE_FLOATS, 0.0f);
+        out.nodes.assign(BVH_FLOAT_NODE_WORDS, 0.0f);
         float* node = out.nodes.data();
         for (int k = 0; k < BVH_WIDTH; ++k) Collapser::set_child(node, k, Box(), 0);
         out.num_nodes = 1;
         out.depth = 1;
+        if constexpr (BVH_QUANT) {
+            std::vector<float> qn(BVH_NODE_WORDS);
+            quantize_node(out.nodes.data(), reinterpret_cast<uint32_t*>(qn.data()));
+            out.nodes.swap(qn);
+        }
         return out;
     }
     bld.nodes.resize(2 * (size_t)n);
@@ -262,6 +359,12 @@ BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& op
     }
     col.emit(g, 1);
     out.depth = col.depth;
+    if constexpr (BVH_QUANT) {
+        std::vector<float> qn((size_t)out.num_nodes * BVH_NODE_WORDS);
+        for (uint32_t i = 0; i < out.num_nodes; ++i)
+            quantize_node(&out.nodes[(size_t)i * BVH_FLOAT_NODE_WORDS], reinterpret_cast<uint32_t*>(&qn[(size_t)i * BVH_NODE_WORDS]));
+        out.nodes.swap(qn);
+    }
     return out;
 }
 

@@ -18,10 +18,8 @@ struct BuildTri {
     float p[3][3];   // world-space vertices
 };
 
-constexpr int BVH_NODE_FLOATS = 8 * BVH_WIDTH;   // 8 rows of BVH_WIDTH floats
-
 struct BVHResult {
-    std::vector<float> nodes;          // BVH_NODE_FLOATS per node
+    std::vector<float> nodes;          // BVH_NODE_WORDS 32-bit words per node (bvh_layout.h)
     std::vector<uint32_t> slot_prim;   // leaf slot -> prim index (input order)
     uint32_t num_nodes = 0;
     uint32_t depth = 0;                // levels of nodes (root = 1)
@@ -36,5 +34,12 @@ struct BVHBuildOptions {
 };
 
 BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& opt);
+
+// Child k of a node as the traversal sees it: the box (decoded exactly in
+// double for quantised nodes), the child code and the outward margin the
+// builder guarantees beyond the child's exact bounds per axis (one grid step
+// for quantised nodes, 0 for float nodes); false for an empty slot.
+bool bvh_child(const BVHResult& b, uint32_t node, int k, double lo[3], double hi[3], int* code,
+               double margin[3] = nullptr);
 
 }  // namespace xt

@@ -31,6 +31,8 @@ xt_status xt_fail(xt_status s, const std::string& msg) {
 
 static xt_status fail(xt_status s, const std::string& msg) { return xt_fail(s, msg); }
 
+constexpr int RB_RING = 4;   // bounce rounds whose queue counts can be in flight to the host
+
 struct RenderBuffers {
     uint32_t capacity = 0;
     PathQueue q[2] = {};
@@ -41,7 +43,8 @@ struct RenderBuffers {
     uint32_t* rec[2] = {};           // training-record index per path-queue entry (allocated on demand)
     float* lobes = nullptr;          // decoded guide mixtures per path-queue entry (allocated on demand)
     uint32_t* counts = nullptr;      // [0], [1]: path queues, [2]: shadow queue
-    uint32_t* host_counts = nullptr; // pinned
+    uint32_t* host_counts = nullptr; // pinned, RB_RING slots of 4 counts (bounce rounds in flight)
+    hipEvent_t round_ev[RB_RING] = {};
     unsigned long long* visits = nullptr;  // node, tri fetches: extension [0..1], shadow [2..3]
     void release() {
         for (int k = 0; k < 2; ++k) {
@@ -51,6 +54,8 @@ struct RenderBuffers {
         hipFree(hits); hipFree(radiance); hipFree(pending); hipFree(counts); hipFree(visits);
         hipFree(rec[0]); hipFree(rec[1]); hipFree(lobes);
         if (host_counts) hipHostFree(host_counts);
+        for (hipEvent_t& e : round_ev)
+            if (e) hipEventDestroy(e);
         *this = RenderBuffers();
     }
 };
@@ -142,6 +147,7 @@ struct xt_scene {
     float4* d_qhits = nullptr;
     uint8_t* d_qocc = nullptr;
     size_t qout_cap = 0;
+    unsigned long long* d_qvisits = nullptr;   // node, tri fetches of the last counted ray query
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
     GuideState guide;
 };
@@ -193,7 +199,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_nmap_attr); hipFree(s->d_textures);
     for (float4* t : s->d_texels) hipFree(t);
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
-    hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc);
+    hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc); hipFree(s->d_qvisits);
     hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
     for (RenderGraph& g : s->graphs) hipGraphExecDestroy(g.exec);
     hipFree(s->d_cursor);
@@ -342,6 +348,10 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
                     return fail(XT_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) + " index out of range");
                 p[k] = mk(od.positions[3 * vi], od.positions[3 * vi + 1], od.positions[3 * vi + 2]);
                 w[k] = M ? xf_point(M, p[k]) : p[k];
+                // the BVH's quantisation grid needs finite world positions of sane magnitude
+                if (!(std::fabs(w[k].x) <= 1e30f && std::fabs(w[k].y) <= 1e30f && std::fabs(w[k].z) <= 1e30f))
+                    return fail(XT_ERR_INVALID_ARGUMENT, "object " + std::to_string(o) +
+                                                             " has a vertex that is not finite or exceeds 1e30 in world space");
                 bt[base + t].p[k][0] = w[k].x;
                 bt[base + t].p[k][1] = w[k].y;
                 bt[base + t].p[k][2] = w[k].z;
@@ -641,17 +651,60 @@ static xt_status query(xt_scene* s, const xt_ray* rays, uint32_t n, void* out, i
         d_rays = s->d_qrays;
         d_out = any ? (void*)s->d_qocc : (void*)s->d_qhits;
     }
+    const bool count = s->counting != 0;
+    if (count) {
+        if (!s->d_qvisits) XT_HIP(dalloc(&s->d_qvisits, 2));
+        XT_HIP(hipMemsetAsync(s->d_qvisits, 0, 2 * sizeof(unsigned long long), s->stream));
+    }
+    hipEvent_t ev[2];
+    XT_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        hipEventDestroy(ev[0]);
+        return fail(XT_ERR_DEVICE, "hipEventCreate failed");
+    }
     uint32_t blocks = trace_blocks(s, n);
-    if (any)
-        hipLaunchKernelGGL((k_trace<true, TRACE_LDS_DEPTH, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
-                           d_rays, d_rays + 1, 2, nullptr, n, nullptr, (uint8_t*)d_out, 1, 1, s->d_ovf, nullptr);
-    else
-        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream,
-                           s->ds, d_rays, d_rays + 1, 2, nullptr, n, (float4*)d_out, nullptr, 1, 1, s->d_ovf, nullptr);
-    XT_HIP(hipGetLastError());
-    if (!on_device)
-        XT_HIP(hipMemcpyAsync(out, d_out, (size_t)n * (any ? 1 : sizeof(xt_hit)), hipMemcpyDeviceToHost, s->stream));
-    XT_HIP(hipStreamSynchronize(s->stream));
+    float4* hits = any ? nullptr : (float4*)d_out;
+    uint8_t* occ = any ? (uint8_t*)d_out : nullptr;
+    hipEventRecord(ev[0], s->stream);
+#define XT_QUERY(ANY, COUNT)                                                                                     \
+    hipLaunchKernelGGL((k_trace<ANY, TRACE_LDS_DEPTH, COUNT>), dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, \
+                       d_rays, d_rays + 1, 2, nullptr, n, hits, occ, 1, 1, s->d_ovf, s->d_qvisits)
+    if (any && count) {
+        XT_QUERY(true, true);
+    } else if (any) {
+        XT_QUERY(true, false);
+    } else if (count) {
+        XT_QUERY(false, true);
+    } else {
+        XT_QUERY(false, false);
+    }
+#undef XT_QUERY
+    hipEventRecord(ev[1], s->stream);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess && !on_device)
+        le = hipMemcpyAsync(out, d_out, (size_t)n * (any ? 1 : sizeof(xt_hit)), hipMemcpyDeviceToHost, s->stream);
+    hipError_t se = hipStreamSynchronize(s->stream);
+    float ms = 0.0f;
+    if (le == hipSuccess && se == hipSuccess) hipEventElapsedTime(&ms, ev[0], ev[1]);
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (le != hipSuccess) return fail(XT_ERR_DEVICE, std::string("ray query failed: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(XT_ERR_DEVICE, std::string("ray query failed: ") + hipGetErrorString(se));
+    s->stats = xt_render_stats{};
+    s->stats.ms_total = ms;
+    unsigned long long v[2] = {0, 0};
+    if (count) XT_HIP(hipMemcpy(v, s->d_qvisits, sizeof v, hipMemcpyDeviceToHost));
+    if (any) {
+        s->stats.shadow_rays = n;
+        s->stats.ms_shadow = ms;
+        s->stats.shadow_node_fetches = v[0];
+        s->stats.shadow_tri_fetches = v[1];
+    } else {
+        s->stats.extension_rays = n;
+        s->stats.ms_extend = ms;
+        s->stats.ext_node_fetches = v[0];
+        s->stats.ext_tri_fetches = v[1];
+    }
     return XT_OK;
 }
 
@@ -690,7 +743,8 @@ static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
     XT_HIP(dalloc(&rb.pending, c));
     XT_HIP(dalloc(&rb.counts, 4));
     XT_HIP(dalloc(&rb.visits, 4));
-    XT_HIP(hipHostMalloc((void**)&rb.host_counts, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    XT_HIP(hipHostMalloc((void**)&rb.host_counts, RB_RING * 4 * sizeof(uint32_t), hipHostMallocDefault));
+    for (hipEvent_t& e : rb.round_ev) XT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rb.capacity = capacity;
     return XT_OK;
 }
@@ -951,79 +1005,108 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             XT_LAUNCHED(result);
             tm.add(3, a, tm.mark());
             s->stats.camera_rays += wp.num_paths;
-            uint32_t live = wp.num_paths;
-            int cur = 0;
+            // Bounce rounds.  Every kernel reads its queue length from device
+            // memory; the host reads each round's counts from a pinned ring
+            // `lag` rounds late, so the GPU does not wait for the host between
+            // bounces.  Grids are sized from the last count read, an upper
+            // bound of every later queue (a path adds at most one extension
+            // and one shadow ray per round).  Rounds launched after the queue
+            // ran empty exit at once.  Training renders read each round's
+            // counts before the next (the record offsets must be exact).
+            const int lag = train ? 0 : 2;
+            uint32_t bound = wp.num_paths;         // >= every queue length from here on
+            uint32_t queue_len[RB_RING];           // queue length of each unverified round
+            int round = 0, verified = 0, cur = 0;
             uint32_t rec_base = 0;
-            while (live > 0) {
-                s->stats.extension_rays += live;
-                s->stats.extend_launches++;
-                a = tm.mark();
-                if (s->counting)
-                    hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, true>), dim3(trace_blocks(s, live)),
-                                       dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                       rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, rb.visits);
-                else
-                    hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(trace_blocks(s, live)),
-                                       dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                       rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr);
-                XT_LAUNCHED(result);
-                hipEvent_t b = tm.mark();
-                tm.add(0, a, b);
-                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + (1 - cur), 1u);
-                XT_LAUNCHED(result);
-                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
-                XT_LAUNCHED(result);
-                GuideRender gr{};
-                if (guide) {
-                    a = tm.mark();
-                    const uint32_t ib = std::max(1u, std::min((live + GI_BLOCK - 1) / GI_BLOCK,
-                                                              (uint32_t)s->dev->num_cu * 4));
-                    uint2* l4 = (uint2*)rb.lobes;
-                    float2* l2 = (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity);
-                    hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
-                                       rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
-                    XT_LAUNCHED(result);
-                    tm.add(4, a, tm.mark());
-                    s->stats.guide_queries += live;
-                    gr.lobes_mk = l4;
-                    gr.lobes2 = l2;
-                    gr.stride = rb.capacity;
-                    gr.prob = g.cfg.guide_prob;
-                    gr.max_vertex = g.cfg.max_guided_vertex;
-                    if (train) {
-                        gr.recs = g.recs;
-                        gr.rec_base = rec_base;
-                        gr.rec_cap = g.rec_cap;
-                    }
-                }
-                a = tm.mark();
-                hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(shade_blocks(live)), dim3(SHADE_BLOCK), 0,
-                                   s->stream, wp, s->ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
-                                   rb.counts + (1 - cur), rb.sq, rb.counts + 2, rb.radiance, rb.pending, gr);
-                XT_LAUNCHED(result);
-                b = tm.mark();
-                tm.add(1, a, b);
-                hipMemcpyAsync(rb.host_counts, rb.counts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
-                if (hipStreamSynchronize(s->stream) != hipSuccess) {
-                    result = fail(XT_ERR_DEVICE, std::string("render failed: ") + hipGetErrorString(hipGetLastError()));
-                    break;
-                }
-                const uint32_t next = rb.host_counts[1 - cur], nsh = rb.host_counts[2];
-                if (train) rec_base = (uint32_t)std::min<uint64_t>((uint64_t)rec_base + next, g.rec_cap);
-                if (nsh > 0) {
-                    s->stats.shadow_rays += nsh;
+            bool launch = true;
+            queue_len[0] = wp.num_paths;
+            while (result == XT_OK && (launch || verified < round)) {
+                if (launch) {
+                    const int slot = round % RB_RING;
                     a = tm.mark();
                     if (s->counting)
-                        hipLaunchKernelGGL(k_shadow<true>, dim3(trace_blocks(s, nsh)), dim3(TRACE_BLOCK), 0, s->stream,
+                        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, true>), dim3(trace_blocks(s, bound)),
+                                           dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
+                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, rb.visits);
+                    else
+                        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(trace_blocks(s, bound)),
+                                           dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
+                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr);
+                    XT_LAUNCHED(result);
+                    tm.add(0, a, tm.mark());
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + (1 - cur), 1u);
+                    XT_LAUNCHED(result);
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
+                    XT_LAUNCHED(result);
+                    GuideRender gr{};
+                    if (guide) {
+                        a = tm.mark();
+                        const uint32_t ib = std::max(1u, std::min((bound + GI_BLOCK - 1) / GI_BLOCK,
+                                                                  (uint32_t)s->dev->num_cu * 4));
+                        uint2* l4 = (uint2*)rb.lobes;
+                        float2* l2 = (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity);
+                        hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
+                                           rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity,
+                                           g.cfg.max_guided_vertex);
+                        XT_LAUNCHED(result);
+                        tm.add(4, a, tm.mark());
+                        gr.lobes_mk = l4;
+                        gr.lobes2 = l2;
+                        gr.stride = rb.capacity;
+                        gr.prob = g.cfg.guide_prob;
+                        gr.max_vertex = g.cfg.max_guided_vertex;
+                        if (train) {
+                            gr.recs = g.recs;
+                            gr.rec_base = rec_base;
+                            gr.rec_cap = g.rec_cap;
+                        }
+                    }
+                    a = tm.mark();
+                    hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(shade_blocks(bound)), dim3(SHADE_BLOCK), 0,
+                                       s->stream, wp, s->ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
+                                       rb.counts + (1 - cur), rb.sq, rb.counts + 2, rb.radiance, rb.pending, gr);
+                    XT_LAUNCHED(result);
+                    tm.add(1, a, tm.mark());
+                    a = tm.mark();
+                    if (s->counting)
+                        hipLaunchKernelGGL(k_shadow<true>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
                                            s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, rb.visits + 2);
                     else
-                        hipLaunchKernelGGL(k_shadow<false>, dim3(trace_blocks(s, nsh)), dim3(TRACE_BLOCK), 0, s->stream,
+                        hipLaunchKernelGGL(k_shadow<false>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
                                            s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, nullptr);
                     XT_LAUNCHED(result);
                     tm.add(2, a, tm.mark());
+                    hipMemcpyAsync(rb.host_counts + 4 * slot, rb.counts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   s->stream);
+                    if (hipEventRecord(rb.round_ev[slot], s->stream) != hipSuccess) {
+                        result = fail(XT_ERR_DEVICE, "hipEventRecord failed");
+                        break;
+                    }
+                    ++round;
+                    cur = 1 - cur;
                 }
-                live = next;
-                cur = 1 - cur;
+                // read back every round at least `lag` behind the last launch (all of them once launching stopped)
+                while (verified < round && (!launch || verified <= round - 1 - lag)) {
+                    const int vs = verified % RB_RING;
+                    if (hipEventSynchronize(rb.round_ev[vs]) != hipSuccess) {
+                        result = fail(XT_ERR_DEVICE, std::string("render failed: ") + hipGetErrorString(hipGetLastError()));
+                        break;
+                    }
+                    const uint32_t* hc = rb.host_counts + 4 * vs;
+                    const uint32_t len = queue_len[vs];
+                    const uint32_t next = hc[(verified & 1) ? 0 : 1], nsh = hc[2];
+                    if (len > 0) {
+                        s->stats.extension_rays += len;
+                        s->stats.extend_launches++;
+                        if (guide) s->stats.guide_queries += len;
+                    }
+                    s->stats.shadow_rays += nsh;
+                    if (train) rec_base = (uint32_t)std::min<uint64_t>((uint64_t)rec_base + next, g.rec_cap);
+                    queue_len[(verified + 1) % RB_RING] = next;
+                    if (launch) bound = next;
+                    if (next == 0) launch = false;
+                    ++verified;
+                }
             }
             if (result != XT_OK) break;
             a = tm.mark();

@@ -24,8 +24,8 @@ namespace xt {
 // map), 6 float4: (tangent k.xyz, uv) for k = 0..2 then (bitangent k.xyz, uv):
 //   [0] t0, u0  [1] t1, v0  [2] t2, u1  [3] b0, v1  [4] b1, u2  [5] b2, v2
 // Flags bits 8..31 hold 1 + the normal map's texture index (0 = none).
-constexpr int NODE_ROW_F4 = BVH_WIDTH / 4;     // float4 per row
-constexpr int NODE_F4 = 8 * NODE_ROW_F4;
+constexpr int NODE_ROW_F4 = BVH_WIDTH / 4;     // float4 per row of a float node
+constexpr int NODE_F4 = BVH_NODE_WORDS / 4;    // float4 per node (bvh_layout.h)
 constexpr int TRI_F4 = 3;
 constexpr int NMAP_F4 = 6;
 constexpr uint32_t TRI_FLAG_NORMALS = 1u;

@@ -366,6 +366,92 @@ def pack_materials(mats) -> np.ndarray:
     return out
 
 
+# --------------------------------------------------------------------------- ray batches
+
+RAY_BATCH_KINDS = ("primary", "diffuse", "shadow")
+
+
+def _frame(n):
+    """Two unit vectors completing each row of n (unit, (m, 3) float64) to an
+    orthonormal basis."""
+    a = np.where(np.abs(n[:, :1]) < 0.5, np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 1.0, 0.0]]))
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    return t, np.cross(n, t)
+
+
+def _surface_points(desc, rng, n):
+    """n points uniform over each triangle of the first object, triangles
+    chosen uniformly by index; returns positions and geometric normals
+    oriented to +y (height-field-like scenes)."""
+    obj = desc.objects[0]
+    pos = obj.positions.astype(np.float64)
+    tri = obj.indices[rng.integers(0, obj.indices.shape[0], n)]
+    p0, p1, p2 = pos[tri[:, 0]], pos[tri[:, 1]], pos[tri[:, 2]]
+    su = np.sqrt(rng.random(n))[:, None]
+    v = rng.random(n)[:, None]
+    p = (1.0 - su) * p0 + su * (1.0 - v) * p1 + su * v * p2
+    ng = np.cross(p1 - p0, p2 - p0)
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    ng *= np.where(ng[:, 1:2] < 0.0, -1.0, 1.0)
+    return p, ng
+
+
+def ray_batch(desc, kind, width=1920, height=1080, seed=7, eps=1e-5):
+    """width*height rays of one kind as (n, 8) float32 {o, tmax, d, 0}, the
+    inputs of the ray-query microbenchmark on the height field:
+      primary -- the scene camera's rays (PinholeCamera::generateRay,
+                 Camera.h:45-55), one per pixel, jittered inside the pixel as
+                 RenderTargetTile::GenerateFilmPosition (RenderTarget.h:68-80);
+      diffuse -- cosine-weighted bounce directions (Vector.h:817-824) about the
+                 geometric normal at uniformly chosen surface points;
+      shadow  -- directions uniform in the sun's cone from surface points that
+                 face the sun, tmax = inf (any-hit queries).
+    Bounce and shadow origins are offset by eps along d (PathTracer.h:254)."""
+    if kind not in RAY_BATCH_KINDS:
+        raise ValueError(f"ray batch kind must be one of {RAY_BATCH_KINDS}")
+    n = width * height
+    rng = np.random.default_rng([seed, RAY_BATCH_KINDS.index(kind)])
+    if kind == "primary":
+        cam = desc.camera
+        tx, ty = (float(v) for v in cam.tans())
+        ys, xs = np.divmod(np.arange(n), width)
+        fx = xs / width + (rng.random(n) - 0.5) / width
+        fy = 1.0 - ys / height + (rng.random(n) - 0.5) / height
+        dc = np.stack([tx * (fx - 0.5) * 2, ty * (fy - 0.5) * 2, np.ones(n)], axis=1)
+        dc /= np.linalg.norm(dc, axis=1, keepdims=True)
+        M = cam.cam_to_world.astype(np.float64)
+        d = dc @ M[:3, :3].T
+        o = np.broadcast_to(M[:3, 3], d.shape)
+    elif kind == "diffuse":
+        p, ng = _surface_points(desc, rng, n)
+        u, v = rng.random(n), rng.random(n)
+        r, phi = np.sqrt(u), 2.0 * np.pi * v
+        t, b = _frame(ng)
+        d = np.sqrt(1.0 - u)[:, None] * ng + (r * np.cos(phi))[:, None] * t + (r * np.sin(phi))[:, None] * b
+        o = p + eps * d
+    else:
+        env = desc.env
+        sun = np.asarray(env.sun_dir, np.float64)
+        sun /= np.linalg.norm(sun)
+        p, ng = _surface_points(desc, rng, 2 * n)
+        keep = np.nonzero(ng @ sun > 0.0)[0]
+        if keep.size < n:
+            raise ValueError("too few surface points face the sun for a shadow-ray batch")
+        p = p[keep[:n]]
+        cmax = math.cos(math.radians(env.sun_half_angle_deg))
+        ct = 1.0 - rng.random(n) * (1.0 - cmax)
+        st, phi = np.sqrt(np.maximum(0.0, 1.0 - ct * ct)), 2.0 * np.pi * rng.random(n)
+        t, b = _frame(np.broadcast_to(sun, (n, 3)).copy())
+        d = ct[:, None] * sun + (st * np.cos(phi))[:, None] * t + (st * np.sin(phi))[:, None] * b
+        o = p + eps * d
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = np.inf
+    rays[:, 4:7] = d
+    return rays
+
+
 # --------------------------------------------------------------------------- small test scenes
 
 def uv_sphere(center, radius, nu=24, nv=12, smooth=True):
