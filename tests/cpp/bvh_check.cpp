@@ -19,6 +19,7 @@ struct Ctx {
     const BVHResult& b;
     const std::vector<BuildTri>& t;
     std::vector<int> seen;
+    std::vector<int> level_of;   // per node, 0 = not reached
     int max_level = 0;
 };
 
@@ -43,6 +44,11 @@ static int walk(Ctx& c, int code, int level, double lo[3], double hi[3]) {
         return 0;
     }
     if (level > c.max_level) c.max_level = level;
+    if (c.level_of[code]) {
+        printf("FAIL node %d reached twice\n", code);
+        return 1;
+    }
+    c.level_of[code] = level;
     for (int k = 0; k < BVH_WIDTH; ++k) {
         double clo[3], chi[3], m[3], slo[3], shi[3];
         int ch;
@@ -80,7 +86,7 @@ int main(int argc, char** argv) {
     }
     BVHBuildOptions opt;
     BVHResult b = build_bvh(t, opt);
-    Ctx c{b, t, std::vector<int>(n, 0)};
+    Ctx c{b, t, std::vector<int>(n, 0), std::vector<int>(b.num_nodes, 0)};
     if ((int)b.slot_prim.size() != n) { printf("FAIL slot count\n"); return 1; }
     double rlo[3], rhi[3];
     if (walk(c, 0, 1, rlo, rhi)) return 1;
@@ -88,6 +94,21 @@ int main(int argc, char** argv) {
         if (c.seen[i] != 1) { printf("FAIL prim %d seen %d\n", i, c.seen[i]); return 1; }
     if (c.max_level != (int)b.depth) { printf("FAIL depth %d vs %u\n", c.max_level, b.depth); return 1; }
     if (b.max_leaf > 8) { printf("FAIL leaf size\n"); return 1; }
+    // the top levels come first, breadth-first (the kernels stage [0, top_nodes) in LDS)
+    const int top_levels = (BVH_QUANT && BVH_TOP_LEVELS > 0) ? BVH_TOP_LEVELS : 0;
+    uint32_t in_top = 0;
+    for (uint32_t v = 0; v < b.num_nodes; ++v) {
+        if (!c.level_of[v]) { printf("FAIL node %u unreachable\n", v); return 1; }
+        if (c.level_of[v] <= top_levels) {
+            ++in_top;
+            if (v >= b.top_nodes) { printf("FAIL top-level node %u at index >= %u\n", v, b.top_nodes); return 1; }
+        }
+        if (v > 0 && v < b.top_nodes && c.level_of[v] < c.level_of[v - 1]) { printf("FAIL top levels not breadth-first\n"); return 1; }
+    }
+    if (in_top != b.top_nodes || (int)b.top_nodes > (BVH_QUANT ? BVH_TOP_MAX_NODES : 0)) {
+        printf("FAIL top_nodes %u vs %u\n", b.top_nodes, in_top);
+        return 1;
+    }
     printf("OK %u %u\n", b.num_nodes, b.depth);
     return 0;
 }

@@ -9,8 +9,11 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-# the default layout (quantised 4-wide nodes) and the float 4- and 8-wide ones
-LAYOUTS = {"quant4": [], "float4": ["-DXT_BVH_QUANT=0"], "float8": ["-DXT_BVH_QUANT=0", "-DXT_BVH_WIDTH=8"]}
+# the default layout (quantised 4-wide nodes), the same with the top four
+# levels numbered breadth-first for LDS staging, and the float 4- and 8-wide
+# ones
+LAYOUTS = {"quant4": [], "quant4_top4": ["-DXT_BVH_TOP_LEVELS=4"], "float4": ["-DXT_BVH_QUANT=0"],
+           "float8": ["-DXT_BVH_QUANT=0", "-DXT_BVH_WIDTH=8"]}
 
 
 @pytest.fixture(scope="module", params=list(LAYOUTS))

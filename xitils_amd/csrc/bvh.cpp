@@ -222,11 +222,11 @@ struct Collapser {
 
 // Float node (8 rows of 4 floats) -> quantised node (bvh_layout.h).  Per
 // axis the grid step is the smallest power of two 2^e for which every valid
-// child fits in bytes with one extra step on each side: lo = floor((lo_c -
-// origin) / 2^e) - 1 >= 0, hi = ceil((hi_c - origin) / 2^e) + 1 <= 255, origin =
-// the node's lower bound minus one step, rounded down to a float.  The step
-// is at least 2^-22 of the node's magnitude, so the origin's rounding stays
-// below half a step.
+// child fits in bytes: lo = floor((lo_c - origin) / 2^e) >= 0, hi =
+// ceil((hi_c - origin) / 2^e) <= 255, with the origin the node's lower bound
+// rounded down to a multiple of the step.  The step is at least 2^-22 of the
+// node's magnitude, so origin / 2^e + 255 < 2^24 and every grid plane is a
+// float.
 void quantize_node(const float* f, uint32_t* q) {
     constexpr int W = 4;
     int nvalid = 0;
@@ -247,21 +247,22 @@ void quantize_node(const float* f, uint32_t* q) {
             }
         }
         const double mag = std::max(std::fabs(lo), std::fabs(hi));
+        // step >= 2^-22 of the magnitude: origin / step and origin / step + 255 stay below 2^24, so every
+        // grid plane origin + q step is a float (exact layout)
         int e = std::max(-126, (mag > 0.0 ? std::ilogb(mag) : -126) - 22);
-        if (hi - lo > 0.0) e = std::max(e, (int)std::ceil(std::log2((hi - lo) / 250.0)));
+        if (hi - lo > 0.0) e = std::max(e, (int)std::ceil(std::log2((hi - lo) / 253.0)));
         float org = 0.0f;
         uint32_t lo_b = 0, hi_b = 0;
         for (; e <= 127; ++e) {
             const double step = std::ldexp(1.0, e);
-            org = (float)(lo - step);
-            if ((double)org > lo - step) org = std::nextafter(org, -INFINITY);
+            org = (float)(std::floor(lo / step) * step);   // a multiple of the step, exact in float
             bool fits = true;
             lo_b = hi_b = 0;
             for (int k = 0; k < W; ++k) {
                 uint32_t ql = 255, qh = 0;   // empty slot: lo > hi
                 if (k < nvalid) {
-                    const double l = std::floor(((double)lo_r[k] - (double)org) / step) - 1.0;
-                    const double h = std::ceil(((double)hi_r[k] - (double)org) / step) + 1.0;
+                    const double l = std::floor(((double)lo_r[k] - (double)org) / step);
+                    const double h = std::ceil(((double)hi_r[k] - (double)org) / step);
                     if (!(l >= 0.0 && h <= 255.0)) {
                         fits = false;
                         break;
@@ -283,6 +284,51 @@ void quantize_node(const float* f, uint32_t* q) {
     for (int k = 0; k < W; ++k) std::memcpy(&q[4 + k], &f[6 * W + k], 4);
 }
 
+// Renumbers the float-layout nodes so the first `levels` levels come first,
+// breadth-first (the traversal kernels copy nodes [0, top_nodes) into LDS);
+// every other node keeps its depth-first position relative to the rest.
+void order_top_levels(BVHResult& out, int levels) {
+    constexpr int W = BVH_WIDTH;
+    const uint32_t n = out.num_nodes;
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    std::vector<uint8_t> placed(n, 0);
+    std::vector<uint32_t> frontier{0};
+    for (int lvl = 1; lvl <= levels && !frontier.empty(); ++lvl) {
+        std::vector<uint32_t> next;
+        for (uint32_t v : frontier) {
+            order.push_back(v);
+            placed[v] = 1;
+            const float* f = &out.nodes[(size_t)v * BVH_FLOAT_NODE_WORDS];
+            for (int k = 0; k < W; ++k) {
+                int code;
+                std::memcpy(&code, &f[6 * W + k], 4);
+                if (f[k] <= f[W + k] && code >= 0) next.push_back((uint32_t)code);
+            }
+        }
+        frontier.swap(next);
+    }
+    out.top_nodes = (uint32_t)order.size();
+    for (uint32_t v = 0; v < n; ++v)
+        if (!placed[v]) order.push_back(v);
+    std::vector<uint32_t> index(n);
+    for (uint32_t i = 0; i < n; ++i) index[order[i]] = i;
+    std::vector<float> re(out.nodes.size());
+    for (uint32_t i = 0; i < n; ++i) {
+        float* d = &re[(size_t)i * BVH_FLOAT_NODE_WORDS];
+        std::memcpy(d, &out.nodes[(size_t)order[i] * BVH_FLOAT_NODE_WORDS], BVH_FLOAT_NODE_WORDS * sizeof(float));
+        for (int k = 0; k < W; ++k) {
+            int code;
+            std::memcpy(&code, &d[6 * W + k], 4);
+            if (d[k] <= d[W + k] && code >= 0) {
+                code = (int)index[(uint32_t)code];
+                std::memcpy(&d[6 * W + k], &code, 4);
+            }
+        }
+    }
+    out.nodes.swap(re);
+}
+
 }  // namespace
 
 bool bvh_child(const BVHResult& b, uint32_t node, int k, double lo[3], double hi[3], int* code, double margin[3]) {
@@ -297,7 +343,7 @@ bool bvh_child(const BVHResult& b, uint32_t node, int k, double lo[3], double hi
             const int e = (int)(int8_t)(uint8_t)(w[3] >> (8 * a));
             lo[a] = (double)org + std::ldexp((double)((w[8 + 2 * a] >> (8 * k)) & 255u), e);
             hi[a] = (double)org + std::ldexp((double)((w[9 + 2 * a] >> (8 * k)) & 255u), e);
-            if (margin) margin[a] = std::ldexp(1.0, e);
+            if (margin) margin[a] = 0.0;
         }
         std::memcpy(code, &w[4 + k], 4);
         return true;
@@ -336,6 +382,7 @@ BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& op
         for (int k = 0; k < BVH_WIDTH; ++k) Collapser::set_child(node, k, Box(), 0);
         out.num_nodes = 1;
         out.depth = 1;
+        if constexpr (BVH_QUANT && BVH_TOP_LEVELS > 0) out.top_nodes = 1;
         if constexpr (BVH_QUANT) {
             std::vector<float> qn(BVH_NODE_WORDS);
             quantize_node(out.nodes.data(), reinterpret_cast<uint32_t*>(qn.data()));
@@ -359,6 +406,7 @@ BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& op
     }
     col.emit(g, 1);
     out.depth = col.depth;
+    if constexpr (BVH_QUANT && BVH_TOP_LEVELS > 0) order_top_levels(out, BVH_TOP_LEVELS);
     if constexpr (BVH_QUANT) {
         std::vector<float> qn((size_t)out.num_nodes * BVH_NODE_WORDS);
         for (uint32_t i = 0; i < out.num_nodes; ++i)

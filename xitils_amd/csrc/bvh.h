@@ -22,6 +22,7 @@ struct BVHResult {
     std::vector<float> nodes;          // BVH_NODE_WORDS 32-bit words per node (bvh_layout.h)
     std::vector<uint32_t> slot_prim;   // leaf slot -> prim index (input order)
     uint32_t num_nodes = 0;
+    uint32_t top_nodes = 0;            // nodes of the first BVH_TOP_LEVELS levels: indices [0, top_nodes), breadth-first
     uint32_t depth = 0;                // levels of nodes (root = 1)
     uint32_t max_leaf = 0;
 };
@@ -37,8 +38,8 @@ BVHResult build_bvh(const std::vector<BuildTri>& tris, const BVHBuildOptions& op
 
 // Child k of a node as the traversal sees it: the box (decoded exactly in
 // double for quantised nodes), the child code and the outward margin the
-// builder guarantees beyond the child's exact bounds per axis (one grid step
-// for quantised nodes, 0 for float nodes); false for an empty slot.
+// builder guarantees beyond the child's exact bounds per axis (0 for both
+// layouts: boxes are only rounded outward); false for an empty slot.
 bool bvh_child(const BVHResult& b, uint32_t node, int k, double lo[3], double hi[3], int* code,
                double margin[3] = nullptr);
 

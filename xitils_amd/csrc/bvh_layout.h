@@ -9,9 +9,11 @@
 //           plane in grid steps above the origin
 //   w14..15 zero
 //   Child k's box is [origin + lo_k 2^e, origin + hi_k 2^e] per axis,
-//   rounded outward on the host with one extra step on each side (the
-//   traversal's rounding stays inside that step).  Four 16-byte loads per
-//   node visit.
+//   rounded outward to the grid on the host.  The origin is a multiple of
+//   the step and the step at least 2^-22 of the node's magnitude, so every
+//   grid plane is a float: the traversal decodes it exactly (one fma) and
+//   runs the float layout's slab test on it.  Four 16-byte loads per node
+//   visit.
 // XT_BVH_QUANT = 0: float nodes of XT_BVH_WIDTH (4 or 8) children as 8 rows
 //   of XT_BVH_WIDTH floats (lo.x, hi.x, lo.y, hi.y, lo.z, hi.z, child codes,
 //   reserved) -- 128 bytes for 4 children, 256 for 8.
@@ -28,6 +30,15 @@
 #ifndef XT_BVH_QUANT
 #define XT_BVH_QUANT 1
 #endif
+// Levels of the tree (root = 1) numbered breadth-first at the front of the
+// node array and staged in LDS by the traversal kernels (quantised layout).
+// Off by default: on the 1080p height field staging 3, 4 or 5 levels made
+// the frame 2%, 2% and 16% slower -- the top nodes already hit in L1, and a
+// wave whose lanes straddle the boundary issues both the LDS and the global
+// loads (5 levels, 21.8 KB, also costs occupancy).
+#ifndef XT_BVH_TOP_LEVELS
+#define XT_BVH_TOP_LEVELS 0
+#endif
 static_assert(XT_BVH_WIDTH == 4 || XT_BVH_WIDTH == 8, "BVH nodes are 4 or 8 wide");
 static_assert(!XT_BVH_QUANT || XT_BVH_WIDTH == 4, "quantised nodes are 4 wide");
 
@@ -36,4 +47,7 @@ constexpr int BVH_WIDTH = XT_BVH_WIDTH;
 constexpr bool BVH_QUANT = XT_BVH_QUANT != 0;
 constexpr int BVH_FLOAT_NODE_WORDS = 8 * BVH_WIDTH;                    // float layout
 constexpr int BVH_NODE_WORDS = BVH_QUANT ? 16 : BVH_FLOAT_NODE_WORDS;  // layout in HBM
+constexpr int BVH_TOP_LEVELS = XT_BVH_TOP_LEVELS;
+// nodes in the top levels of a 4-wide tree: 1 + 4 + ... + 4^(L-1)
+constexpr int BVH_TOP_MAX_NODES = BVH_TOP_LEVELS <= 0 ? 0 : ((1 << (2 * BVH_TOP_LEVELS)) - 1) / 3;
 }

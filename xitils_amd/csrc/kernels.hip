@@ -85,6 +85,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
                                                         int hit_prim_ids, int tmax_from_ray, uint32_t* __restrict__ ovf,
                                                         unsigned long long* __restrict__ visit_totals) {
     __shared__ uint32_t lds[LDS_DEPTH * TRACE_BLOCK];
+    __shared__ float4 top[TOP_F4];
+    stage_top_nodes(top, sc);
     const uint32_t n = count_ptr ? *count_ptr : count_fixed;
     const size_t nthreads = (size_t)gridDim.x * blockDim.x;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
         if (__ballot(active) == 0) break;
         if (active) {
             bool occ = false;
-            if (trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+            if (trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
                 if (ANY) {
                     occluded[rid] = occ ? 1 : 0;
                 } else {
@@ -508,6 +510,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
                                                         uint32_t* __restrict__ ovf,
                                                         unsigned long long* __restrict__ visit_totals) {
     __shared__ uint32_t lds[TRACE_LDS_DEPTH * TRACE_BLOCK];
+    __shared__ float4 top[TOP_F4];
+    stage_top_nodes(top, sc);
     const uint32_t n = *count;
     const size_t nthreads = (size_t)gridDim.x * blockDim.x;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -540,7 +544,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
         if (__ballot(active) == 0) break;
         if (active) {
             bool occ = false;
-            if (trav_step<true, TRACE_LDS_DEPTH, COUNT>(sc, r, t, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+            if (trav_step<true, TRACE_LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
                 // unoccluded contributions go straight to the path radiance, or to
                 // pending[] when the path continues (see k_shade)
                 float4 c4 = sq.c[rid];

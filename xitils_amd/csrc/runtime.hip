@@ -536,7 +536,19 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     // traversal stack: at most 3 pushes per level plus one
     const uint32_t stack_bound = (BVH_WIDTH - 1) * bvh.depth + 1;
     s->ovf_depth = stack_bound > (uint32_t)TRACE_LDS_DEPTH ? stack_bound - TRACE_LDS_DEPTH : 0;
-    s->trace_grid_max = (uint32_t)dev->num_cu * 8;
+    // Persistent traversal grid: the waves split the queue into fixed ranges,
+    // so launch no more blocks per CU than can be resident at once (the
+    // kernels' registers and LDS, capped at 8).
+    int resident = 8;
+    {
+        int a = 0, b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, TRACE_LDS_DEPTH, false>, TRACE_BLOCK, 0) ==
+                hipSuccess && a > 0)
+            resident = std::min(resident, a);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, TRACE_BLOCK, 0) == hipSuccess && b > 0)
+            resident = std::min(resident, b);
+    }
+    s->trace_grid_max = (uint32_t)dev->num_cu * (uint32_t)resident;
     s->trace_grid = s->trace_grid_max;
     s->shade_grid = (uint32_t)dev->num_cu * (2048 / SHADE_BLOCK);
     if (s->ovf_depth) XT_HIP_S(dalloc(&s->d_ovf, (size_t)s->ovf_depth * s->trace_grid_max * TRACE_BLOCK));
@@ -555,6 +567,7 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
     ds.light_slots = s->d_light_slots;
     ds.num_lights = (uint32_t)lobjs.size();
     ds.num_prims = nprim;
+    ds.top_nodes = std::min<uint32_t>(bvh.top_nodes, (uint32_t)BVH_TOP_MAX_NODES);
     const xt_env& e = desc->env;
     ds.env.has_sky = e.has_sky;
     ds.env.sky = mk(e.sky[0], e.sky[1], e.sky[2]);

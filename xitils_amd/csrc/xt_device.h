@@ -71,6 +71,7 @@ struct DevScene {
     const uint32_t* light_slots;
     uint32_t num_lights;
     uint32_t num_prims;
+    uint32_t top_nodes;         // nodes [0, top_nodes) are the tree's top levels, staged in LDS by the traversal
     DevEnv env;
 };
 

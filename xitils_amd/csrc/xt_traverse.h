@@ -130,23 +130,17 @@ __device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const 
 }
 
 // Quantised node (bvh_layout.h): entry distances and child codes of its
-// four children from four 16-byte loads.  A plane at origin + q 2^e is
-// crossed at t = q (2^e / d) + (origin - o) / d, evaluated as
-// fma(q, ldexp(1/d, e), (origin - o) * (1/d)): the ldexp is exact and the
-// rounding of the rest stays below the one-step margin the builder adds on
-// each side (or, for rays from far away, inside BOX_MARGIN).  Planes are
-// picked per axis by the ray's direction sign; a zero direction component
-// turns that axis's distances into NaN or infinities, which the max/min
-// below ignore or treat as unbounded (never a cull of a box the ray hits).
-__device__ __forceinline__ void node_slabs_q(const float4* __restrict__ nd, const RayPre& r, float cull,
-                                             float tn[4], int ch[4]) {
-    const float4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3];
+// four children from four 16-byte loads.  The plane origin + q 2^e is a
+// float by construction, so fma(q, 2^e, origin) yields it exactly and the
+// slab test is node_slabs' on the decoded (outward-rounded) box: the same
+// rounding, the same BOX_MARGIN, the same handling of zero direction
+// components.  Planes are picked per axis by the ray's direction sign.
+__device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, const float4 w2, const float4 w3,
+                                             const RayPre& r, float cull, float tn[4], int ch[4]) {
     const uint32_t meta = __float_as_uint(w0.w);
     const int nvalid = (int)(meta >> 24);
-    const float sx = ldexpf(r.ix, (int)(int8_t)(meta & 255u));
-    const float sy = ldexpf(r.iy, (int)(int8_t)((meta >> 8) & 255u));
-    const float sz = ldexpf(r.iz, (int)(int8_t)((meta >> 16) & 255u));
-    const float bx = (w0.x - r.o.x) * r.ix, by = (w0.y - r.o.y) * r.iy, bz = (w0.z - r.o.z) * r.iz;
+    const int ex = (int)(int8_t)(meta & 255u), ey = (int)(int8_t)((meta >> 8) & 255u),
+              ez = (int)(int8_t)((meta >> 16) & 255u);
     const uint32_t lx = __float_as_uint(w2.x), hx = __float_as_uint(w2.y);
     const uint32_t ly = __float_as_uint(w2.z), hy = __float_as_uint(w2.w);
     const uint32_t lz = __float_as_uint(w3.x), hz = __float_as_uint(w3.y);
@@ -157,16 +151,31 @@ __device__ __forceinline__ void node_slabs_q(const float4* __restrict__ nd, cons
     ch[1] = __float_as_int(w1.y);
     ch[2] = __float_as_int(w1.z);
     ch[3] = __float_as_int(w1.w);
+    const float stx = ldexpf(1.0f, ex), sty = ldexpf(1.0f, ey), stz = ldexpf(1.0f, ez);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         auto q = [&](uint32_t v) { return (float)((v >> (8 * k)) & 255u); };
-        const float t0x = fmaf(q(nx), sx, bx), t1x = fmaf(q(fx), sx, bx);
-        const float t0y = fmaf(q(ny), sy, by), t1y = fmaf(q(fy), sy, by);
-        const float t0z = fmaf(q(nz), sz, bz), t1z = fmaf(q(fz), sz, bz);
+        const float t0x = (fmaf(q(nx), stx, w0.x) - r.o.x) * r.ix, t1x = (fmaf(q(fx), stx, w0.x) - r.o.x) * r.ix;
+        const float t0y = (fmaf(q(ny), sty, w0.y) - r.o.y) * r.iy, t1y = (fmaf(q(fy), sty, w0.y) - r.o.y) * r.iy;
+        const float t0z = (fmaf(q(nz), stz, w0.z) - r.o.z) * r.iz, t1z = (fmaf(q(fz), stz, w0.z) - r.o.z) * r.iz;
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         const float tmax = fminf(fminf(t1x, t1y), fminf(t1z, cull)) * BOX_MARGIN;
         tn[k] = (k < nvalid && tmin <= tmax) ? tmin : INFINITY;
     }
+}
+
+// LDS copy of the tree's top levels (nodes [0, top_nodes), numbered
+// breadth-first by the builder): every ray starts there, so the first few
+// node visits of each ray are LDS reads instead of global gathers.
+constexpr int TOP_F4 = BVH_QUANT && BVH_TOP_MAX_NODES > 0 ? BVH_TOP_MAX_NODES * NODE_F4 : 1;
+
+// Block-wide copy into LDS; every thread of the block must call it.
+__device__ __forceinline__ void stage_top_nodes(float4* top, const DevScene& sc) {
+    if constexpr (BVH_QUANT && BVH_TOP_MAX_NODES > 0) {
+        const uint32_t n = min(sc.top_nodes, (uint32_t)BVH_TOP_MAX_NODES) * NODE_F4;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) top[i] = sc.nodes[i];
+    }
+    __syncthreads();
 }
 
 // Fetch counters of the counting build (algorithmic traffic accounting).
@@ -202,15 +211,29 @@ __device__ __forceinline__ void trav_init(Trav& t, float tmax) {
 // One traversal step; returns true when the ray is finished (for ANY, *occ
 // tells whether a blocker was found).
 template <bool ANY, int LDS_DEPTH, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, uint32_t* lds, int lds_stride,
-                                          uint32_t* ovf, size_t ovf_stride, VisitCount& vc, bool* occ) {
+__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
+                                          uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
+                                          VisitCount& vc, bool* occ) {
     if (t.node >= 0) {
         const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
         if (COUNT) vc.nodes++;
         float tn[BVH_WIDTH];
         int ch[BVH_WIDTH];
         if constexpr (BVH_QUANT) {
-            node_slabs_q(nd, r, t.cull, tn, ch);
+            float4 w0, w1, w2, w3;
+            if (BVH_TOP_MAX_NODES > 0 && (uint32_t)t.node < sc.top_nodes) {
+                const float4* p = top + t.node * NODE_F4;
+                w0 = p[0];
+                w1 = p[1];
+                w2 = p[2];
+                w3 = p[3];
+            } else {
+                w0 = nd[0];
+                w1 = nd[1];
+                w2 = nd[2];
+                w3 = nd[3];
+            }
+            node_slabs_q(w0, w1, w2, w3, r, t.cull, tn, ch);
         } else {
             node_slabs(nd, r, t.cull, tn);
 #pragma unroll
