@@ -390,6 +390,13 @@ def rays_main(args):
     info = scene.info()
     node_bytes = info["bytes_nodes"] // max(1, info["bvh_nodes"])
     log(f"[bench] scene {info}")
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB read + 1 GiB written
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy done (1 GiB in, 1 GiB out)")
     batches, per_kind = {}, {}
     for kind in scenes.RAY_BATCH_KINDS:
         rays = scenes.ray_batch(desc, kind, W, H)

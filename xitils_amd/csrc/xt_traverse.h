@@ -244,15 +244,31 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, T
             // children go in node order (the first hit one next) and the
             // sorting network is skipped (5% faster on sun shadow rays)
             int nxt = -1;
+            if (BVH_WIDTH == 4 && t.sp + BVH_WIDTH <= LDS_DEPTH) {
+                // room in LDS: every child is written, the pushed ones (hit,
+                // not next) compacted from the stack top, the rest into the
+                // free slot above them -- no per-push branches
+                int pos = t.sp;
 #pragma unroll
-            for (int k = 0; k < BVH_WIDTH; ++k) {
-                if (tn[k] < INFINITY) {
-                    if (nxt == -1) {
-                        nxt = k;
-                    } else {
-                        if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
-                        else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
-                        ++t.sp;
+                for (int k = 0; k < BVH_WIDTH; ++k) {
+                    const bool hit = tn[k] < INFINITY;
+                    const bool push = hit && nxt != -1;
+                    if (hit && nxt == -1) nxt = k;
+                    lds[(push ? pos : t.sp + BVH_WIDTH - 1) * lds_stride] = (uint32_t)ch[k];
+                    pos += push ? 1 : 0;
+                }
+                t.sp = pos;
+            } else {
+#pragma unroll
+                for (int k = 0; k < BVH_WIDTH; ++k) {
+                    if (tn[k] < INFINITY) {
+                        if (nxt == -1) {
+                            nxt = k;
+                        } else {
+                            if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
+                            else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
+                            ++t.sp;
+                        }
                     }
                 }
             }
@@ -285,6 +301,20 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, T
 #pragma unroll
         for (int k = 0; k < BVH_WIDTH; ++k) nhit += tn[k] < INFINITY;
         if (nhit > 0) {
+            if (BVH_WIDTH == 4 && t.sp + BVH_WIDTH - 1 <= LDS_DEPTH) {
+                // room in LDS: the far-to-near order ch[nhit-1] .. ch[1] goes
+                // to fixed slots sp, sp+1, sp+2 by selects; slots at or above
+                // the new top hold leftovers that are never read
+                const int v0 = nhit == 4 ? ch[3 % BVH_WIDTH] : (nhit == 3 ? ch[2] : ch[1]);
+                const int v1 = nhit == 4 ? ch[2] : ch[1];
+                uint32_t* s = lds + t.sp * lds_stride;
+                s[0] = (uint32_t)v0;
+                s[lds_stride] = (uint32_t)v1;
+                s[2 * lds_stride] = (uint32_t)ch[1];
+                t.sp += nhit - 1;
+                t.node = ch[0];
+                return false;
+            }
             // far children first; static indices keep ch[] in registers
 #pragma unroll
             for (int k = BVH_WIDTH - 1; k >= 1; --k) {
