@@ -27,10 +27,17 @@ struct BVHResult {
     uint32_t max_leaf = 0;
 };
 
+#ifndef XT_BVH_MAX_LEAF
+#define XT_BVH_MAX_LEAF 4
+#endif
+#ifndef XT_BVH_NODE_COST
+#define XT_BVH_NODE_COST 1.0f
+#endif
+
 struct BVHBuildOptions {
     int bins = 32;
-    int max_leaf = 4;          // <= 8 (3-bit count in the leaf code)
-    float cost_node = 1.0f;    // SAH traversal cost relative to one triangle test
+    int max_leaf = XT_BVH_MAX_LEAF;         // <= 8 (3-bit count in the leaf code)
+    float cost_node = XT_BVH_NODE_COST;     // SAH traversal cost relative to one triangle test
     int threads = 0;           // 0 = hardware concurrency
 };
 

@@ -191,6 +191,7 @@ struct VisitCount {
 // Each loop iteration advances every busy lane by one node visit or one leaf.
 constexpr int REFILL_IDLE = 16;   // 8 and 32 measured no better
 
+
 struct Trav {
     int node, sp;
     float best_t, b0, b1, cull, tmax;
@@ -208,78 +209,78 @@ __device__ __forceinline__ void trav_init(Trav& t, float tmax) {
     t.best_slot = INVALID;
 }
 
-// One traversal step; returns true when the ray is finished (for ANY, *occ
-// tells whether a blocker was found).
+// Visits internal node t.node: true when t.node is now the next child to
+// visit (the nearest hit one; the other hit children are pushed), false when
+// no child was hit.
 template <bool ANY, int LDS_DEPTH, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
-                                          uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
-                                          VisitCount& vc, bool* occ) {
-    if (t.node >= 0) {
-        const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
-        if (COUNT) vc.nodes++;
-        float tn[BVH_WIDTH];
-        int ch[BVH_WIDTH];
-        if constexpr (BVH_QUANT) {
-            float4 w0, w1, w2, w3;
-            if (BVH_TOP_MAX_NODES > 0 && (uint32_t)t.node < sc.top_nodes) {
-                const float4* p = top + t.node * NODE_F4;
-                w0 = p[0];
-                w1 = p[1];
-                w2 = p[2];
-                w3 = p[3];
-            } else {
-                w0 = nd[0];
-                w1 = nd[1];
-                w2 = nd[2];
-                w3 = nd[3];
-            }
-            node_slabs_q(w0, w1, w2, w3, r, t.cull, tn, ch);
+__device__ __forceinline__ bool visit_node(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
+                                           uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
+                                           VisitCount& vc) {
+    const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
+    if (COUNT) vc.nodes++;
+    float tn[BVH_WIDTH];
+    int ch[BVH_WIDTH];
+    if constexpr (BVH_QUANT) {
+        float4 w0, w1, w2, w3;
+        if (BVH_TOP_MAX_NODES > 0 && (uint32_t)t.node < sc.top_nodes) {
+            const float4* p = top + t.node * NODE_F4;
+            w0 = p[0];
+            w1 = p[1];
+            w2 = p[2];
+            w3 = p[3];
         } else {
-            node_slabs(nd, r, t.cull, tn);
-#pragma unroll
-            for (int k = 0; k < BVH_WIDTH; ++k) ch[k] = __float_as_int(node_row(nd, 6, k));
+            w0 = nd[0];
+            w1 = nd[1];
+            w2 = nd[2];
+            w3 = nd[3];
         }
-        if constexpr (ANY) {
-            // any hit: the answer does not depend on the visiting order, so
-            // children go in node order (the first hit one next) and the
-            // sorting network is skipped (5% faster on sun shadow rays)
-            int nxt = -1;
-            if (BVH_WIDTH == 4 && t.sp + BVH_WIDTH <= LDS_DEPTH) {
-                // room in LDS: every child is written, the pushed ones (hit,
-                // not next) compacted from the stack top, the rest into the
-                // free slot above them -- no per-push branches
-                int pos = t.sp;
+        node_slabs_q(w0, w1, w2, w3, r, t.cull, tn, ch);
+    } else {
+        node_slabs(nd, r, t.cull, tn);
 #pragma unroll
-                for (int k = 0; k < BVH_WIDTH; ++k) {
-                    const bool hit = tn[k] < INFINITY;
-                    const bool push = hit && nxt != -1;
-                    if (hit && nxt == -1) nxt = k;
-                    lds[(push ? pos : t.sp + BVH_WIDTH - 1) * lds_stride] = (uint32_t)ch[k];
-                    pos += push ? 1 : 0;
-                }
-                t.sp = pos;
-            } else {
+        for (int k = 0; k < BVH_WIDTH; ++k) ch[k] = __float_as_int(node_row(nd, 6, k));
+    }
+    if constexpr (ANY) {
+        // any hit: the answer does not depend on the visiting order, so
+        // children go in node order (the first hit one next) and the
+        // sorting network is skipped (5% faster on sun shadow rays)
+        int nxt = -1;
+        if (BVH_WIDTH == 4 && t.sp + BVH_WIDTH <= LDS_DEPTH) {
+            // room in LDS: every child is written, the pushed ones (hit,
+            // not next) compacted from the stack top, the rest into the
+            // free slot above them -- no per-push branches
+            int pos = t.sp;
 #pragma unroll
-                for (int k = 0; k < BVH_WIDTH; ++k) {
-                    if (tn[k] < INFINITY) {
-                        if (nxt == -1) {
-                            nxt = k;
-                        } else {
-                            if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
-                            else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
-                            ++t.sp;
-                        }
+            for (int k = 0; k < BVH_WIDTH; ++k) {
+                const bool hit = tn[k] < INFINITY;
+                const bool push = hit && nxt != -1;
+                if (hit && nxt == -1) nxt = k;
+                lds[(push ? pos : t.sp + BVH_WIDTH - 1) * lds_stride] = (uint32_t)ch[k];
+                pos += push ? 1 : 0;
+            }
+            t.sp = pos;
+        } else {
+#pragma unroll
+            for (int k = 0; k < BVH_WIDTH; ++k) {
+                if (tn[k] < INFINITY) {
+                    if (nxt == -1) {
+                        nxt = k;
+                    } else {
+                        if (t.sp < LDS_DEPTH) lds[t.sp * lds_stride] = (uint32_t)ch[k];
+                        else ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride] = (uint32_t)ch[k];
+                        ++t.sp;
                     }
                 }
             }
-            if (nxt >= 0) {
-                int c = ch[0];
+        }
+        if (nxt >= 0) {
+            int c = ch[0];
 #pragma unroll
-                for (int k = 1; k < BVH_WIDTH; ++k) c = nxt == k ? ch[k] : c;
-                t.node = c;
-                return false;
-            }
-        } else {
+            for (int k = 1; k < BVH_WIDTH; ++k) c = nxt == k ? ch[k] : c;
+            t.node = c;
+            return true;
+        }
+    } else {
 #define XT_CSWAP(i, j)                                   \
     if (tn[j] < tn[i]) {                                 \
         float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;     \
@@ -313,7 +314,7 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, T
                 s[2 * lds_stride] = (uint32_t)ch[1];
                 t.sp += nhit - 1;
                 t.node = ch[0];
-                return false;
+                return true;
             }
             // far children first; static indices keep ch[] in registers
 #pragma unroll
@@ -325,42 +326,71 @@ __device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, T
                 }
             }
             t.node = ch[0];
-            return false;
+            return true;
         }
+    }
+    return false;
+}
+
+// Tests the first triangle of leaf code `leaf` (< 0); returns the code of the
+// rest of the leaf, or 0 when the leaf is done (or, for ANY, when *occ was set).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ int visit_tri(const DevScene& sc, const RayPre& r, Trav& t, int leaf, VisitCount& vc,
+                                         bool* occ) {
+    uint32_t code = (uint32_t)(~leaf);
+    uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
+    if (COUNT) vc.tris++;
+    const float4* tr = sc.tris + (size_t)first * TRI_F4;
+    float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
+    float th, b0, b1;
+    if (tri_test(xyz(v0), xyz(v1), xyz(v2), r, t.tmax, &th, &b0, &b1)) {
+        if (ANY) {
+            *occ = true;
+            return 0;
         }
+        uint32_t prim = __float_as_uint(v0.w);
+        if (th < t.best_t || (th == t.best_t && prim < t.best_prim)) {
+            t.best_t = th;
+            t.best_prim = prim;
+            t.best_slot = first;
+            t.b0 = b0;
+            t.b1 = b1;
+            t.cull = fminf(t.tmax, th);
+        }
+    }
+    // rest of the leaf: first + 1, count - 1
+    return cnt > 1 ? ~(int)(((first + 1u) << 3) | (cnt - 2u)) : 0;
+}
+
+// Pops the next entry into t.node; false when the stack is empty.
+template <int LDS_DEPTH>
+__device__ __forceinline__ bool trav_pop(Trav& t, const uint32_t* lds, int lds_stride, const uint32_t* ovf,
+                                         size_t ovf_stride) {
+    if (t.sp == 0) return false;
+    --t.sp;
+    t.node = (int)(t.sp < LDS_DEPTH ? lds[t.sp * lds_stride] : ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride]);
+    return true;
+}
+
+// One traversal step (one node or one triangle); returns true when the ray
+// is finished (for ANY, *occ tells whether a blocker was found).
+template <bool ANY, int LDS_DEPTH, bool COUNT>
+__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
+                                          uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
+                                          VisitCount& vc, bool* occ) {
+    if (t.node >= 0) {
+        if (visit_node<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, lds, lds_stride, ovf, ovf_stride, vc)) return false;
     } else {
         // one triangle per step: a divergent wave then pays for one node
         // visit plus one triangle test per iteration, not for a whole leaf
-        uint32_t code = (uint32_t)(~t.node);
-        uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
-        if (COUNT) vc.tris++;
-        const float4* tr = sc.tris + (size_t)first * TRI_F4;
-        float4 v0 = tr[0], v1 = tr[1], v2 = tr[2];
-        float th, b0, b1;
-        if (tri_test(xyz(v0), xyz(v1), xyz(v2), r, t.tmax, &th, &b0, &b1)) {
-            if (ANY) {
-                *occ = true;
-                return true;
-            }
-            uint32_t prim = __float_as_uint(v0.w);
-            if (th < t.best_t || (th == t.best_t && prim < t.best_prim)) {
-                t.best_t = th;
-                t.best_prim = prim;
-                t.best_slot = first;
-                t.b0 = b0;
-                t.b1 = b1;
-                t.cull = fminf(t.tmax, th);
-            }
-        }
-        if (cnt > 1) {  // rest of the leaf: first + 1, count - 1
-            t.node = ~(int)(((first + 1u) << 3) | (cnt - 2u));
+        const int rest = visit_tri<ANY, COUNT>(sc, r, t, t.node, vc, occ);
+        if (ANY && *occ) return true;
+        if (rest) {
+            t.node = rest;
             return false;
         }
     }
-    if (t.sp == 0) return true;
-    --t.sp;
-    t.node = (int)(t.sp < LDS_DEPTH ? lds[t.sp * lds_stride] : ovf[(size_t)(t.sp - LDS_DEPTH) * ovf_stride]);
-    return false;
+    return !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride);
 }
 
 }  // namespace xt
