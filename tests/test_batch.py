@@ -81,6 +81,14 @@ def test_guided_graph_render_matches(dev):
     ref = sc.render(48, 48, 8, **kw)
     films, _ = api.render_batch([(sc, 48, 48, 8, None, kw)])
     assert np.array_equal(ref, films[0])
+    # a new network (new allocation) must not be read through the graph cached for the old one
+    print("[gpu] guided batch after guide_init again", flush=True)
+    sc.guide_init(seed=4)
+    sc.render(48, 48, 4, seed=30, max_depth=8, flags=api.XT_FLAG_GUIDE | api.XT_FLAG_GUIDE_TRAIN)
+    ref2 = sc.render(48, 48, 8, **kw)
+    films2, _ = api.render_batch([(sc, 48, 48, 8, None, kw)])
+    assert np.array_equal(ref2, films2[0])
+    assert not np.array_equal(ref, ref2)
 
 
 def test_autotune_changes_shapes_not_results(dev):

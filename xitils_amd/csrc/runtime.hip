@@ -1506,6 +1506,7 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     if (!(c.learning_rate > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f))
         return fail(XT_ERR_INVALID_ARGUMENT, "bad Adam settings");
     XT_HIP(hipSetDevice(s->dev->ordinal));
+    drop_graphs(s);   // captured guided renders hold the old network's addresses
     GuideState& g = s->guide;
     g.release();
     g.cfg = c;
