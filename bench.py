@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--batch-spp", type=int, default=0, help="batch: override every scene's spp")
     ap.add_argument("--budget", type=float, default=2.0, help="guiding: seconds of rendering per method")
     ap.add_argument("--ref-spp", type=int, default=16384, help="guiding: samples per pixel of the FP64 reference")
+    ap.add_argument("--guided-runs", type=int, default=3,
+                    help="guiding: guided runs from fresh networks (seeds 1..n); the median ratio is reported")
     ap.add_argument("--room-tris", type=int, default=500_000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 1 GiB streaming kernel (out = in * 1) to calibrate rocprof byte counters")
@@ -136,7 +138,8 @@ def guiding_main(args):
     ref = quality.reference_image(scene, W, H, spp=args.ref_spp, max_depth=max_depth)
     ref_s = time.perf_counter() - t0
     log(f"[bench] reference {args.ref_spp} spp in {ref_s:.1f} s ({W * H * args.ref_spp / ref_s / 1e6:.1f} Mpaths/s)")
-    res = equal_time.compare(scene, W, H, args.budget, ref, log=log, max_depth=max_depth)
+    res = equal_time.compare(scene, W, H, args.budget, ref, log=log, max_depth=max_depth,
+                             guide_seeds=list(range(1, args.guided_runs + 1)))
     out = {
         "metric": "equal-time relMSE guided vs plain GPU PT (16k-spp oracle)",
         "value": round(res["relmse_ratio_equal_time"], 4),
@@ -157,6 +160,7 @@ def guiding_main(args):
         "guided_seconds_to_plain_relmse": res["guided_seconds_to_plain_relmse"],
         "plain": res["plain"],
         "guided": res["guided"],
+        "guided_runs": res["runs"],
         "reference_seconds": round(ref_s, 2),
     }
     # the network's arithmetic: 33,792 FLOP per query (2 x (96x64 + 64x64 + 64x64 + 64x40)); a training

@@ -274,3 +274,21 @@ def test_guided_furnace_closed_form():
         assert hit.sum() > 50
         assert abs(px.mean() - a) < 2e-3 * a
         assert np.abs(px - a).max() < 0.05 * a
+
+
+@pytest.mark.gpu
+def test_equal_time_compare_reports_the_median_run():
+    """equal_time.compare with several guide seeds: one plain run, one guided
+    run per fresh network, and the reported ratio is the median of theirs."""
+    import torch  # noqa: F401  (the films are torch tensors on the library's HIP runtime)
+    from xitils_amd import equal_time, quality
+    desc = scenes.cornell(48, 48)
+    print("[gpu] equal-time compare, cornell 48x48, 3 seeds", flush=True)
+    sc = api.Scene(api.Device(0), desc)
+    ref = quality.reference_image(sc, 48, 48, spp=256, chunk=64, max_depth=8)
+    res = equal_time.compare(sc, 48, 48, 0.15, ref, max_depth=8, guide_seeds=[1, 2, 3])
+    ratios = sorted(r["ratio"] for r in res["runs"])
+    assert [r["seed"] for r in res["runs"]] == [1, 2, 3]
+    assert res["relmse_ratio_equal_time"] == pytest.approx(ratios[1], rel=1e-3)
+    assert res["guided"]["seed"] in (1, 2, 3) and res["plain"]["spp"] > 0
+    assert all(np.isfinite(r["relmse"]) and r["spp"] > 0 for r in res["runs"])

@@ -86,20 +86,34 @@ def time_to_reach(curve, target):
     return None
 
 
-def compare(scene, width, height, budget_s, ref, *, log=None, **kw):
-    """Runs plain then guided for the same budget; returns a summary dict."""
+def compare(scene, width, height, budget_s, ref, *, log=None, guide_seeds=None, **kw):
+    """Runs plain, then guided for the same budget; returns a summary dict.
+    With guide_seeds, the guided run is repeated from a fresh network per
+    seed (scene.guide_init(seed=...)) and the run with the median ratio is
+    reported, all runs listed (training is stochastic: single runs spread by
+    about 10%).  Without, one guided run uses the scene's current network."""
     plain, _ = progressive(scene, width, height, budget_s, ref, guided=False, log=log, **kw)
-    guided, gstats = progressive(scene, width, height, budget_s, ref, guided=True, log=log, **kw)
     e_plain = plain[-1][2]
-    e_guided = guided[-1][2]
-    t_reach = time_to_reach(guided, e_plain)
+    runs = []
+    for seed in (guide_seeds or [None]):
+        if seed is not None:
+            scene.guide_init(seed=seed)
+        guided, gstats = progressive(scene, width, height, budget_s, ref, guided=True, log=log, **kw)
+        t_reach = time_to_reach(guided, e_plain)
+        runs.append({"seed": seed, "curve": guided, "stats": gstats, "relmse": guided[-1][2],
+                     "ratio": e_plain / guided[-1][2], "t_reach": t_reach})
+    med = sorted(runs, key=lambda r: r["ratio"])[len(runs) // 2]
+    guided, gstats, t_reach = med["curve"], med["stats"], med["t_reach"]
     return {
         "budget_s": budget_s,
         "plain": {"spp": plain[-1][1], "seconds": round(plain[-1][0], 4), "relmse": e_plain},
-        "guided": {"spp": guided[-1][1], "seconds": round(guided[-1][0], 4), "relmse": e_guided, **gstats},
-        "relmse_ratio_equal_time": e_plain / e_guided,
+        "guided": {"spp": guided[-1][1], "seconds": round(guided[-1][0], 4), "relmse": guided[-1][2],
+                   "seed": med["seed"], **gstats},
+        "relmse_ratio_equal_time": med["ratio"],
         "guided_seconds_to_plain_relmse": t_reach,
         "time_speedup": (plain[-1][0] / t_reach) if t_reach else None,
+        "runs": [{"seed": r["seed"], "spp": r["curve"][-1][1], "relmse": r["relmse"], "ratio": round(r["ratio"], 4),
+                  "time_speedup": (plain[-1][0] / r["t_reach"]) if r["t_reach"] else None} for r in runs],
         "curves": {"plain": [(round(a, 4), b, c) for a, b, c in plain[:: max(1, len(plain) // 40)]],
                    "guided": [(round(a, 4), b, c) for a, b, c in guided[:: max(1, len(guided) // 40)]]},
     }
