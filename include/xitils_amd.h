@@ -180,9 +180,9 @@ typedef struct xt_render_stats {
     double ms_total;         /* wall time of the call's GPU work (HIP events) */
     double ms_extend, ms_shade, ms_shadow, ms_other;    /* per-stage kernel time (HIP events) */
     uint32_t extend_launches;
-    /* BVH node (bytes_nodes / bvh_nodes of xt_scene_info: 128 B for 4-wide
-     * nodes) and triangle (48 B) fetches of the traversal kernels; filled
-     * only after xt_scene_set_counting(scene, 1). */
+    /* BVH node (bytes_nodes / bvh_nodes of xt_scene_info: 64 B with the
+     * default quantised 4-wide nodes) and triangle (48 B) fetches of the
+     * traversal kernels; filled only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
     /* guiding: network inference and training kernel time, Adam steps taken,
      * training samples kept, queue entries run through the network and
