@@ -8,6 +8,14 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# torch's HIP runtime must be loaded before libxitils_amd opens a device, so
+# that tests using torch device tensors (progressive films) share the one
+# runtime whatever the test order.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")

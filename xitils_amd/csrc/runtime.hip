@@ -111,6 +111,7 @@ struct RenderGraph {
 struct xt_scene {
     xt_device* dev = nullptr;
     hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev_pool;   // StageTimer events, reused across renders
     DevScene ds{};
     // device allocations
     float4* d_nodes = nullptr;
@@ -205,6 +206,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_cursor);
     s->rb.release();
     s->guide.release();
+    for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return XT_OK;
@@ -764,12 +766,24 @@ static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
+// Per-stage kernel time of one render from event pairs on its stream.  The
+// events come from the scene's pool (created once, reused by every render
+// after the previous one synchronised), so a bounce round costs the host
+// event records, not creations.
 struct StageTimer {
     hipStream_t st;
+    std::vector<hipEvent_t>* pool;
+    size_t used = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];   // extend, shade, shadow, other, guide infer, guide train
     hipEvent_t mark() {
-        hipEvent_t e;
-        hipEventCreate(&e);
+        hipEvent_t e = nullptr;
+        if (used < pool->size()) {
+            e = (*pool)[used];
+        } else {
+            hipEventCreate(&e);
+            pool->push_back(e);
+        }
+        ++used;
         hipEventRecord(e, st);
         return e;
     }
@@ -782,13 +796,6 @@ struct StageTimer {
             ms += m;
         }
         return ms;
-    }
-    ~StageTimer() {
-        for (auto& v : ev)
-            for (auto& p : v) {
-                hipEventDestroy(p.first);
-                hipEventDestroy(p.second);
-            }
     }
 };
 
@@ -991,7 +998,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     wp.tan_x = cam->tan_x;
     wp.tan_y = cam->tan_y;
     std::memcpy(wp.cam, cam->cam_to_world, sizeof(wp.cam));
-    StageTimer tm{s->stream};
+    StageTimer tm{s->stream, &s->ev_pool};
     hipEvent_t t_begin = tm.mark();
     const uint32_t gblocks_max = (uint32_t)s->dev->num_cu * 8;
     auto blocks256 = [&](uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, gblocks_max)); };
@@ -1178,8 +1185,6 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             }
         }
     }
-    hipEventDestroy(t_begin);
-    hipEventDestroy(t_end);
     if (!film_on_device) {
         if (result == XT_OK) {
             hipError_t ce = hipMemcpy(film, d_film, film_bytes, hipMemcpyDeviceToHost);
