@@ -126,7 +126,8 @@ struct GuideRender {
 template <bool ANY, int LDS_DEPTH, bool COUNT>
 __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, int ray_stride,
                         const uint32_t* count_ptr, uint32_t count_fixed, float4* hits, uint8_t* occluded,
-                        int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals);
+                        int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals,
+                        uint32_t* clear_a, uint32_t* clear_b);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
 template <bool GUIDE, bool NMAP>
@@ -135,7 +136,8 @@ __global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_
                         float4* radiance, float4* pending, GuideRender gr);
 template <bool COUNT>
 __global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending,
-                         uint32_t* ovf, unsigned long long* visit_totals);
+                         uint32_t* ovf, unsigned long long* visit_totals, const uint32_t* counts3,
+                         uint32_t* counts_out);
 __global__ void k_accumulate(WaveParams wp, const float4* radiance, void* film, int film_f64);
 __global__ void k_zero_u32(uint32_t* p, uint32_t n);
 __global__ void k_cursor_init(BatchCursor* c, BatchPlan plan);

@@ -83,8 +83,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
                                                         const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
                                                         float4* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                         int hit_prim_ids, int tmax_from_ray, uint32_t* __restrict__ ovf,
-                                                        unsigned long long* __restrict__ visit_totals) {
+                                                        unsigned long long* __restrict__ visit_totals,
+                                                        uint32_t* __restrict__ clear_a, uint32_t* __restrict__ clear_b) {
     __shared__ uint32_t lds[LDS_DEPTH * TRACE_BLOCK];
+    // the next round's queue counters (their last readers ran before this
+    // kernel in stream order), so the bounce loop needs no clearing launches
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (clear_a) *clear_a = 0u;
+        if (clear_b) *clear_b = 0u;
+    }
     __shared__ float4 top[TOP_F4];
     stage_top_nodes(top, sc);
     const uint32_t n = count_ptr ? *count_ptr : count_fixed;
@@ -139,7 +146,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
 #define XT_INST_TRACE(ANY, COUNT)                                                                                  \
     template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT>(DevScene, const float4*, const float4*, int,     \
                                                                   const uint32_t*, uint32_t, float4*, uint8_t*, int, \
-                                                                  int, uint32_t*, unsigned long long*);
+                                                                  int, uint32_t*, unsigned long long*, uint32_t*,    \
+                                                                  uint32_t*);
 XT_INST_TRACE(false, false)
 XT_INST_TRACE(true, false)
 XT_INST_TRACE(false, true)
@@ -508,8 +516,13 @@ template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         float4* __restrict__ radiance, float4* __restrict__ pending,
                                                         uint32_t* __restrict__ ovf,
-                                                        unsigned long long* __restrict__ visit_totals) {
+                                                        unsigned long long* __restrict__ visit_totals,
+                                                        const uint32_t* __restrict__ counts3,
+                                                        uint32_t* __restrict__ counts_out) {
     __shared__ uint32_t lds[TRACE_LDS_DEPTH * TRACE_BLOCK];
+    // the round's final queue counts (shade, which appends them, ran before
+    // this kernel) to the host's pinned ring, in place of a copy launch
+    if (counts_out && blockIdx.x == 0 && threadIdx.x < 3) counts_out[threadIdx.x] = counts3[threadIdx.x];
     __shared__ float4 top[TOP_F4];
     stage_top_nodes(top, sc);
     const uint32_t n = *count;
@@ -562,9 +575,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
     if (COUNT) flush_counts(visit_totals, vc);
 }
 template __global__ void k_shadow<false>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
-                                         unsigned long long*);
+                                         unsigned long long*, const uint32_t*, uint32_t*);
 template __global__ void k_shadow<true>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
-                                        unsigned long long*);
+                                        unsigned long long*, const uint32_t*, uint32_t*);
 
 // film[pixel] += radiance of each of the batch's samples, in sample order.
 __global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, const float4* __restrict__ radiance, void* film,

@@ -44,6 +44,7 @@ struct RenderBuffers {
     float* lobes = nullptr;          // decoded guide mixtures per path-queue entry (allocated on demand)
     uint32_t* counts = nullptr;      // [0], [1]: path queues, [2]: shadow queue
     uint32_t* host_counts = nullptr; // pinned, RB_RING slots of 4 counts (bounce rounds in flight)
+    uint32_t* host_counts_dev = nullptr;  // the same memory's device address (written by k_shadow)
     hipEvent_t round_ev[RB_RING] = {};
     unsigned long long* visits = nullptr;  // node, tri fetches: extension [0..1], shadow [2..3]
     void release() {
@@ -683,7 +684,7 @@ static xt_status query(xt_scene* s, const xt_ray* rays, uint32_t n, void* out, i
     hipEventRecord(ev[0], s->stream);
 #define XT_QUERY(ANY, COUNT)                                                                                     \
     hipLaunchKernelGGL((k_trace<ANY, TRACE_LDS_DEPTH, COUNT>), dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, \
-                       d_rays, d_rays + 1, 2, nullptr, n, hits, occ, 1, 1, s->d_ovf, s->d_qvisits)
+                       d_rays, d_rays + 1, 2, nullptr, n, hits, occ, 1, 1, s->d_ovf, s->d_qvisits, nullptr, nullptr)
     if (any && count) {
         XT_QUERY(true, true);
     } else if (any) {
@@ -758,7 +759,9 @@ static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
     XT_HIP(dalloc(&rb.pending, c));
     XT_HIP(dalloc(&rb.counts, 4));
     XT_HIP(dalloc(&rb.visits, 4));
-    XT_HIP(hipHostMalloc((void**)&rb.host_counts, RB_RING * 4 * sizeof(uint32_t), hipHostMallocDefault));
+    XT_HIP(hipHostMalloc((void**)&rb.host_counts, RB_RING * 4 * sizeof(uint32_t),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    XT_HIP(hipHostGetDevicePointer((void**)&rb.host_counts_dev, rb.host_counts, 0));
     for (hipEvent_t& e : rb.round_ev) XT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rb.capacity = capacity;
     return XT_OK;
@@ -1040,24 +1043,30 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             uint32_t rec_base = 0;
             bool launch = true;
             queue_len[0] = wp.num_paths;
-            while (result == XT_OK && (launch || verified < round)) {
-                if (launch) {
+            // a path adds one extension ray per round, so max_depth + 1 rounds hold every live queue
+            const int max_rounds = p->max_depth > 0 ? p->max_depth + 2 + lag : (1 << 16);
+            // Once a read count is zero every later round is empty: the host
+            // moves on (accumulate, next batch) without waiting for them.
+            while (result == XT_OK && launch) {
+                if (round >= max_rounds) {
+                    result = fail(XT_ERR_DEVICE, "bounce loop did not drain its queue");
+                    break;
+                }
+                {
                     const int slot = round % RB_RING;
                     a = tm.mark();
                     if (s->counting)
                         hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, true>), dim3(trace_blocks(s, bound)),
                                            dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, rb.visits);
+                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, rb.visits,
+                                           rb.counts + (1 - cur), rb.counts + 2);
                     else
                         hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(trace_blocks(s, bound)),
                                            dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr);
+                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr,
+                                           rb.counts + (1 - cur), rb.counts + 2);   // also clears the next counters
                     XT_LAUNCHED(result);
                     tm.add(0, a, tm.mark());
-                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + (1 - cur), 1u);
-                    XT_LAUNCHED(result);
-                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
-                    XT_LAUNCHED(result);
                     GuideRender gr{};
                     if (guide) {
                         a = tm.mark();
@@ -1090,14 +1099,14 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     a = tm.mark();
                     if (s->counting)
                         hipLaunchKernelGGL(k_shadow<true>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
-                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, rb.visits + 2);
+                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, rb.visits + 2,
+                                           rb.counts, rb.host_counts_dev + 4 * slot);
                     else
                         hipLaunchKernelGGL(k_shadow<false>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
-                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, nullptr);
+                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, nullptr,
+                                           rb.counts, rb.host_counts_dev + 4 * slot);   // counts -> pinned ring
                     XT_LAUNCHED(result);
                     tm.add(2, a, tm.mark());
-                    hipMemcpyAsync(rb.host_counts + 4 * slot, rb.counts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                   s->stream);
                     if (hipEventRecord(rb.round_ev[slot], s->stream) != hipSuccess) {
                         result = fail(XT_ERR_DEVICE, "hipEventRecord failed");
                         break;
@@ -1105,8 +1114,8 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     ++round;
                     cur = 1 - cur;
                 }
-                // read back every round at least `lag` behind the last launch (all of them once launching stopped)
-                while (verified < round && (!launch || verified <= round - 1 - lag)) {
+                // read back every round at least `lag` behind the last launch
+                while (launch && verified <= round - 1 - lag) {
                     const int vs = verified % RB_RING;
                     if (hipEventSynchronize(rb.round_ev[vs]) != hipSuccess) {
                         result = fail(XT_ERR_DEVICE, std::string("render failed: ") + hipGetErrorString(hipGetLastError()));
@@ -1123,7 +1132,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     s->stats.shadow_rays += nsh;
                     if (train) rec_base = (uint32_t)std::min<uint64_t>((uint64_t)rec_base + next, g.rec_cap);
                     queue_len[(verified + 1) % RB_RING] = next;
-                    if (launch) bound = next;
+                    bound = next;
                     if (next == 0) launch = false;
                     ++verified;
                 }
@@ -1269,9 +1278,8 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
     int cur = 0;
     for (int it = 0; it <= p->max_depth; ++it) {
         hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
-                           rb.q[cur].o, rb.q[cur].d, 1, rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + (1 - cur), 1u);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, rb.counts + 2, 1u);
+                           rb.q[cur].o, rb.q[cur].d, 1, rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr,
+                           rb.counts + (1 - cur), rb.counts + 2);
         GuideRender gr{};
         if (guide) {
             hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, rb.q[cur],
@@ -1286,7 +1294,7 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
                            rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
                            rb.counts + 2, rb.radiance, rb.pending, gr);
         hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.sq, rb.counts + 2,
-                           rb.radiance, rb.pending, s->d_ovf, nullptr);
+                           rb.radiance, rb.pending, s->d_ovf, nullptr, rb.counts, nullptr);
         cur = 1 - cur;
     }
     hipLaunchKernelGGL(k_accumulate, dim3(ablocks), dim3(256), 0, s->stream, wp, rb.radiance, d_film, f64);
