@@ -250,10 +250,12 @@ typedef struct xt_batch_item {
 
 /* Many RenderTarget::render calls at once (no reference counterpart; the
  * closest is the tile scheduler of RenderTarget.h:84-127).  Each item gives
- * the film xt_render would give, bit for bit, but its bounce loop runs as a
- * HIP graph replayed once per batch with no host round trip (every kernel
- * reads its queue count from device memory), and items on different scenes
- * run concurrently on their scenes' streams.  Needs a bounce cap
+ * the film xt_render would give, bit for bit.  Items of at most 4 Mi paths
+ * (one graph replay) run their bounce loop as a HIP graph with no host round
+ * trip (every kernel reads its queue count from device memory), items on
+ * different scenes concurrently on their scenes' streams; larger items then
+ * render one after another through xt_render's bounce loop while those
+ * graphs run.  Needs a bounce cap
  * (1 <= max_depth <= 1024) and no XT_FLAG_GUIDE_TRAIN.  Graphs are cached
  * per scene, camera, parameters and film. */
 xt_status xt_render_batch(xt_batch_item* items, uint32_t n);

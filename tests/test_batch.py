@@ -63,6 +63,7 @@ def test_batch_progressive_repeat_and_graph_reuse(dev):
 
 
 def test_batch_spans_several_batches(dev):
+    # more paths than one graph replay: the item renders through the bounce loop
     # 1024 x 512 at 40 spp: 21 M paths, more than one batch of the path buffers
     sc = api.Scene(dev, scenes.cornell(1024, 512))
     kw = dict(seed=11, max_depth=4)

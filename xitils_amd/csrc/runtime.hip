@@ -1356,6 +1356,15 @@ static xt_status enqueue_graph_render(xt_scene* s, const RenderGraph& g, uint32_
     return XT_OK;
 }
 
+// Items with more paths than one graph replay holds render through
+// xt_render's lagged bounce loop: per item on the 64-room batch it is 15-60%
+// faster than replaying the fixed max_depth + 1 rounds per 4 Mi paths, while
+// graphs win below one replay (no host in the loop, concurrent streams).
+static bool batch_item_uses_loop(const xt_batch_item& it) {
+    const uint64_t paths = (uint64_t)it.params.width * it.params.height * it.params.spp;
+    return paths > graph_capacity((uint64_t)it.params.width * it.params.height, it.params.spp);
+}
+
 xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
     if (n && !items) return fail(XT_ERR_INVALID_ARGUMENT, "items is NULL");
     for (uint32_t i = 0; i < n; ++i) {
@@ -1398,6 +1407,7 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
             hipMemcpyAsync(q.d_film, it.film, q.bytes, hipMemcpyHostToDevice, s->stream);
         }
         if (it.params.spp == 0) continue;
+        if (batch_item_uses_loop(it)) continue;   // rendered after every graph item is under way
         RenderGraph* g = nullptr;
         result = find_graph(s, &it.camera, &it.params, q.d_film, it.film_is_f64, &g);
         if (result != XT_OK) break;
@@ -1406,6 +1416,14 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
         hipEventRecord(q.a, s->stream);
         result = enqueue_graph_render(s, *g, it.params.spp_offset);
         hipEventRecord(q.b, s->stream);
+    }
+    // items larger than one graph replay: the lagged bounce loop of xt_render,
+    // one after another, while the graph items run on their own streams
+    for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
+        xt_batch_item& it = items[i];
+        if (it.params.spp == 0 || !batch_item_uses_loop(it)) continue;
+        result = xt_render(it.scene, &it.camera, &it.params, pend[i].d_film, it.film_is_f64, 1);
+        if (result == XT_OK) it.ms = (float)it.scene->stats.ms_total;
     }
     for (uint32_t i = 0; i < n; ++i) {
         xt_batch_item& it = items[i];
