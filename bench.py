@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--plane-height", type=int, default=2160)
     ap.add_argument("--plane-spp", type=int, default=1, help="nmap: samples per pixel per variant per step")
     ap.add_argument("--variants", type=int, default=16)
+    ap.add_argument("--guided", action="store_true",
+                    help="nmap: train the guiding network online, then render every variant guided in one "
+                         "xt_render_batch call per step, and compare guided with plain quality")
     ap.add_argument("--scenes", type=int, default=64, help="batch: number of scenes")
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each scene's launch shapes first")
     ap.add_argument("--batch-side", type=int, default=0, help="batch: override every scene's image side")
@@ -365,6 +368,8 @@ def nmap_main(args):
         "build": {"scene_gen_s": round(gen_s, 1), "bvh_build_upload_s": round(upload_s, 1),
                   "bvh_nodes": info["bvh_nodes"], "bvh_depth": info["bvh_depth"]},
     }
+    if args.guided:
+        out.update(nmap_guided(args, scene, cams, W, H, spp, depth, flags, paths / wall / 1e6))
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         cdesc = desc
@@ -373,6 +378,76 @@ def nmap_main(args):
         cargs.width, cargs.height, cargs.spp = W, H, spp
         out["cpu_baseline"] = cpu_baseline(cdesc, cargs, depth, flags)
     print(json.dumps(out), flush=True)
+
+
+def nmap_guided(args, scene, cams, W, H, spp, depth, flags, plain_mpaths):
+    """The displaced plane with the guiding network: online training (one
+    1-spp training render per camera variant), then timed steps that render
+    every variant guided in one xt_render_batch call, then plain and guided
+    at equal spp on variant 0 against a reference (960x540, 4096 spp: the
+    16,384-spp 4K oracle would be 136 G paths per variant)."""
+    import numpy as np
+    import torch
+    from xitils_amd import api, quality
+    nv = len(cams)
+    scratch = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    scene.guide_init(seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = samples = 0
+    for v in range(nv):
+        scene.render_device(scratch.data_ptr(), W, H, 1, camera=cams[v], seed=2, spp_offset=v, max_depth=depth,
+                            flags=flags | api.XT_FLAG_GUIDE | api.XT_FLAG_GUIDE_TRAIN)
+        st = scene.stats()
+        steps += st["guide_steps"]
+        samples += st["guide_samples"]
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"[bench] guide trained: {nv} training renders, {steps} Adam steps, {samples} samples, {train_s:.2f} s")
+    films = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0") for _ in range(nv)]
+    gflags = flags | api.XT_FLAG_GUIDE
+
+    def batch_step(k):
+        api.render_batch([(scene, W, H, spp, (films[v].data_ptr(), False),
+                           dict(camera=cams[v], seed=1, spp_offset=k * spp, max_depth=depth, flags=gflags))
+                          for v in range(nv)])
+    batch_step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch_step(k + 1)
+        log(f"[bench] guided batch step {k} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    guided_mpaths = W * H * spp * nv * args.steps / wall / 1e6
+    finite = all(bool(torch.isfinite(f).all()) for f in films)
+    # quality on variant 0 at reduced resolution, equal spp
+    qw, qh, qspp = 960, 540, 64
+    ref = quality.reference_image(scene, qw, qh, spp=4096, chunk=512, seed=0x5EED, camera=cams[0], max_depth=depth,
+                                  flags=flags)
+    res = {}
+    for name, f in (("plain", flags), ("guided", gflags)):
+        img = torch.zeros((qh, qw, 3), dtype=torch.float32, device="cuda:0")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.render_device(img.data_ptr(), qw, qh, qspp, camera=cams[0], seed=3, max_depth=depth, flags=f)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"relmse": quality.rel_mse(img.cpu().numpy().astype(np.float64) / qspp, ref), "seconds": dt}
+    eff = (res["plain"]["relmse"] * res["plain"]["seconds"]) / (res["guided"]["relmse"] * res["guided"]["seconds"])
+    log(f"[bench] guided {guided_mpaths:.1f} Mpaths/s (batched), quality {res}, efficiency ratio {eff:.3f}")
+    return {
+        "guided": {
+            "mpaths_per_s": round(guided_mpaths, 2), "plain_mpaths_per_s": round(plain_mpaths, 2),
+            "calls": "one xt_render_batch call of %d variants per step" % nv, "finite": finite,
+            "training": {"renders": nv, "adam_steps": steps, "samples": samples, "seconds": round(train_s, 2)},
+            "quality_variant0": {"resolution": f"{qw}x{qh}", "spp": qspp, "reference_spp": 4096,
+                                 "plain": {k: round(v, 5) for k, v in res["plain"].items()},
+                                 "guided": {k: round(v, 5) for k, v in res["guided"].items()},
+                                 "efficiency_ratio": round(eff, 3),
+                                 "note": "relMSE x render time, plain over guided (>1: guiding pays)"},
+        }
+    }
 
 
 def rays_main(args):
