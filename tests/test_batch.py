@@ -73,6 +73,21 @@ def test_batch_spans_several_batches(dev):
     assert np.array_equal(ref, films[0])
 
 
+def test_batch_mixes_graph_and_loop_items_on_one_scene(dev):
+    # a small item (one graph replay) and a large one (the bounce loop, which
+    # grows the scene's path buffers after the graph was enqueued) on the same
+    # scene, plus a small item on another scene: every film as xt_render's
+    a = api.Scene(dev, scenes.cornell(64, 64))
+    b = api.Scene(dev, scenes.material_zoo(32, 32))
+    jobs = [(a, 64, 64, 8, dict(seed=31, max_depth=6)), (a, 1024, 1024, 6, dict(seed=32, max_depth=5)),
+            (b, 32, 32, 16, dict(seed=33, max_depth=6))]
+    print("[gpu] mixed graph / loop batch", flush=True)
+    films, ms = api.render_batch([(sc, w, h, spp, None, kw) for sc, w, h, spp, kw in jobs])
+    for (sc, w, h, spp, kw), f, t in zip(jobs, films, ms):
+        assert np.array_equal(sc.render(w, h, spp, **kw), f), (w, h, spp)
+        assert t > 0
+
+
 def test_guided_graph_render_matches(dev):
     sc = api.Scene(dev, scenes.material_zoo(48, 48))
     sc.guide_init(seed=3)
