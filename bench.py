@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each scene's launch shapes first")
     ap.add_argument("--batch-side", type=int, default=0, help="batch: override every scene's image side")
     ap.add_argument("--batch-spp", type=int, default=0, help="batch: override every scene's spp")
-    ap.add_argument("--budget", type=float, default=2.0, help="guiding: seconds of rendering per method")
+    ap.add_argument("--budget", type=float, default=6.0, help="guiding: seconds of rendering per method")
     ap.add_argument("--ref-spp", type=int, default=16384, help="guiding: samples per pixel of the FP64 reference")
     ap.add_argument("--guided-runs", type=int, default=3,
                     help="guiding: guided runs from fresh networks (seeds 1..n); the median ratio is reported")
