@@ -581,8 +581,8 @@ def rays_main(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only, "
-              f"see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the renderer shards by sample range "
+              f"with the scene replicated per GPU and one film reduction, which is not built", file=sys.stderr)
         sys.exit(2)
     if args.workload == "guiding":
         return guiding_main(args)
