@@ -267,16 +267,15 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
             const float4 h = hits[i];
             const uint32_t slot = __float_as_uint(h.y);
             const uint32_t v = __float_as_uint(pq.w[i].w) & 0xFFFFFFu;   // vertex index (shaded at v)
+            // every surface hit gets lobes: the shade kernel itself skips emitters
+            // (their paths end) and delta materials, so no material lookup here
+            // stands between the hit and the surface reads
             if (slot != INVALID && (max_vertex == 0 || v <= max_vertex)) {
-                const uint32_t mat_id = __float_as_uint(sc.tris[(size_t)slot * TRI_F4 + 1].w);
-                const uint32_t mt = sc.mats[mat_id].type;
-                if (mt != XT_MAT_EMISSION && !mat_is_delta(mt)) {
-                    const Surf sf = make_surf(sc, slot, xyz(pq.d[i]), h.z, h.w);
-                    P = sf.p;
-                    N = sf.sn;
-                    WO = sf.wo;
-                    use = true;
-                }
+                const Surf sf = make_surf(sc, slot, xyz(pq.d[i]), h.z, h.w);
+                P = sf.p;
+                N = sf.sn;
+                WO = sf.wo;
+                use = true;
             }
         }
         if (__ballot(use) == 0) continue;
