@@ -76,7 +76,8 @@ __global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4*
                                  uint32_t* count, float* weight_sum);
 
 constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
-constexpr int GB_BLOCK = 512;   // threads per block of k_guide_backprop (32 samples per wave pass)
+constexpr int GB_BLOCK = 256;   // threads per block of k_guide_backprop (32 samples per wave pass); a
+                                // 32768-sample minibatch is 1024 waves, one 4-wave block per CU
 constexpr int GW_BLOCK = 256;   // threads per block of k_guide_wgrad (one k chunk per block)
 constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize
 

@@ -875,7 +875,8 @@ static xt_status guide_gradient_rows(xt_scene* s, const GuideSample* d_samples, 
                                                      (uint32_t)s->dev->num_cu));
     hipLaunchKernelGGL(k_guide_backprop, dim3(bp_blocks), dim3(GB_BLOCK), 0, s->stream, g.net, g.gb, d_samples, n,
                        weight_scale, weight_clip, g.T, g.t_ld);
-    uint32_t chunk = std::max(256u, (((npad + g.partial_rows - 1) / g.partial_rows) + 31u) & ~31u);
+    // 128-sample chunks: a 32768-sample minibatch fills one block per CU
+    uint32_t chunk = std::max(128u, (((npad + g.partial_rows - 1) / g.partial_rows) + 31u) & ~31u);
     const uint32_t nchunks = std::max(1u, (npad + chunk - 1) / chunk);
     hipLaunchKernelGGL(k_guide_wgrad, dim3(nchunks), dim3(GW_BLOCK), 0, s->stream, g.T, g.t_ld, n, chunk, g.partial);
     XT_HIP(hipGetLastError());
