@@ -130,11 +130,16 @@ __device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const 
 }
 
 // Quantised node (bvh_layout.h): entry distances and child codes of its
-// four children from four 16-byte loads.  The plane origin + q 2^e is a
-// float by construction, so fma(q, 2^e, origin) yields it exactly and the
-// slab test is node_slabs' on the decoded (outward-rounded) box: the same
-// rounding, the same BOX_MARGIN, the same handling of zero direction
-// components.  Planes are picked per axis by the ray's direction sign.
+// four children from four 16-byte loads.  Each plane is origin + q 2^e; the
+// node-wide offset origin - ray origin is computed once per axis, so a plane
+// costs a byte convert, an fma and a multiply.  That offset carries one
+// rounding (at most 2^-24 of its magnitude) that node_slabs' per-plane
+// difference does not, so the near and far offsets are pushed outward by
+// 2^-22 of it, signed by the ray direction: every decoded box contains the
+// box node_slabs would test, and the remaining roundings are those
+// BOX_MARGIN covers there.  A zero direction component gives +-inf (or NaN,
+// which fminf / fmaxf ignore, for an offset of exactly zero) as in
+// node_slabs.  Planes are picked per axis by the ray's direction sign.
 __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, const float4 w2, const float4 w3,
                                              const RayPre& r, float cull, float tn[4], int ch[4]) {
     const uint32_t meta = __float_as_uint(w0.w);
@@ -152,12 +157,16 @@ __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, c
     ch[2] = __float_as_int(w1.z);
     ch[3] = __float_as_int(w1.w);
     const float stx = ldexpf(1.0f, ex), sty = ldexpf(1.0f, ey), stz = ldexpf(1.0f, ez);
+    const float ox = w0.x - r.o.x, oy = w0.y - r.o.y, oz = w0.z - r.o.z;
+    const float sx = copysignf(fabsf(ox) * 0x1p-22f, r.ix), sy = copysignf(fabsf(oy) * 0x1p-22f, r.iy),
+                sz = copysignf(fabsf(oz) * 0x1p-22f, r.iz);
+    const float nox = ox - sx, fox = ox + sx, noy = oy - sy, foy = oy + sy, noz = oz - sz, foz = oz + sz;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         auto q = [&](uint32_t v) { return (float)((v >> (8 * k)) & 255u); };
-        const float t0x = (fmaf(q(nx), stx, w0.x) - r.o.x) * r.ix, t1x = (fmaf(q(fx), stx, w0.x) - r.o.x) * r.ix;
-        const float t0y = (fmaf(q(ny), sty, w0.y) - r.o.y) * r.iy, t1y = (fmaf(q(fy), sty, w0.y) - r.o.y) * r.iy;
-        const float t0z = (fmaf(q(nz), stz, w0.z) - r.o.z) * r.iz, t1z = (fmaf(q(fz), stz, w0.z) - r.o.z) * r.iz;
+        const float t0x = fmaf(q(nx), stx, nox) * r.ix, t1x = fmaf(q(fx), stx, fox) * r.ix;
+        const float t0y = fmaf(q(ny), sty, noy) * r.iy, t1y = fmaf(q(fy), sty, foy) * r.iy;
+        const float t0z = fmaf(q(nz), stz, noz) * r.iz, t1z = fmaf(q(fz), stz, foz) * r.iz;
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         const float tmax = fminf(fminf(t1x, t1y), fminf(t1z, cull)) * BOX_MARGIN;
         tn[k] = (k < nvalid && tmin <= tmax) ? tmin : INFINITY;
