@@ -109,6 +109,25 @@ int main(int argc, char** argv) {
         printf("FAIL top_nodes %u vs %u\n", b.top_nodes, in_top);
         return 1;
     }
+    // quantised empty slots: inverted boxes (bytes 255 / 0 on every axis) whose
+    // code is child 0's (the traversal does not mask them), -1 in an all-empty node
+    if constexpr (BVH_QUANT) {
+        for (uint32_t v = 0; v < b.num_nodes; ++v) {
+            uint32_t w[16];
+            std::memcpy(w, &b.nodes[(size_t)v * BVH_NODE_WORDS], sizeof w);
+            const int nvalid = (int)(w[3] >> 24);
+            for (int k = nvalid; k < 4; ++k) {
+                bool inverted = true;
+                for (int a = 0; a < 3; ++a)
+                    inverted = inverted && ((w[8 + 2 * a] >> (8 * k)) & 255u) == 255u &&
+                               ((w[9 + 2 * a] >> (8 * k)) & 255u) == 0u;
+                if (w[4 + k] != (nvalid > 0 ? w[4] : ~0u) || !inverted) {
+                    printf("FAIL empty slot %d of node %u\n", k, v);
+                    return 1;
+                }
+            }
+        }
+    }
     printf("OK %u %u\n", b.num_nodes, b.depth);
     return 0;
 }

@@ -282,6 +282,16 @@ void quantize_node(const float* f, uint32_t* q) {
     }
     q[3] = meta;
     for (int k = 0; k < W; ++k) std::memcpy(&q[4 + k], &f[6 * W + k], 4);
+    // Empty slots (inverted boxes) take child 0's code, so the traversal needs
+    // no per-slot validity test: only a ray whose slab distances are all NaN,
+    // or one whose widened plane offset exceeds the box's 255 steps, can
+    // accept one, and then visits a real subtree a second time (closest-hit
+    // and any-hit answers do not change).  The all-empty root of a scene
+    // without triangles takes leaf code -1 (slot 0, a zero-area record)
+    // instead of looping back to itself.
+    uint32_t fill = ~0u;
+    if (nvalid > 0) fill = q[4];
+    for (int k = nvalid; k < W; ++k) q[4 + k] = fill;
 }
 
 // Renumbers the float-layout nodes so the first `levels` levels come first,

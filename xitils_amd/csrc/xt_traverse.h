@@ -140,10 +140,11 @@ __device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const 
 // BOX_MARGIN covers there.  A zero direction component gives +-inf (or NaN,
 // which fminf / fmaxf ignore, for an offset of exactly zero) as in
 // node_slabs.  Planes are picked per axis by the ray's direction sign.
+// Empty slots are inverted boxes (bytes 255 / 0) that carry a real child
+// code (quantize_node), so they are not masked here.
 __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, const float4 w2, const float4 w3,
                                              const RayPre& r, float cull, float tn[4], int ch[4]) {
     const uint32_t meta = __float_as_uint(w0.w);
-    const int nvalid = (int)(meta >> 24);
     const int ex = (int)(int8_t)(meta & 255u), ey = (int)(int8_t)((meta >> 8) & 255u),
               ez = (int)(int8_t)((meta >> 16) & 255u);
     const uint32_t lx = __float_as_uint(w2.x), hx = __float_as_uint(w2.y);
@@ -169,7 +170,7 @@ __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, c
         const float t0z = fmaf(q(nz), stz, noz) * r.iz, t1z = fmaf(q(fz), stz, foz) * r.iz;
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         const float tmax = fminf(fminf(t1x, t1y), fminf(t1z, cull)) * BOX_MARGIN;
-        tn[k] = (k < nvalid && tmin <= tmax) ? tmin : INFINITY;
+        tn[k] = tmin <= tmax ? tmin : INFINITY;
     }
 }
 
