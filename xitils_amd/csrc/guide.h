@@ -80,5 +80,6 @@ constexpr int GB_BLOCK = 256;   // threads per block of k_guide_backprop (32 sam
                                 // 32768-sample minibatch is 1024 waves, one 4-wave block per CU
 constexpr int GW_BLOCK = 256;   // threads per block of k_guide_wgrad (one k chunk per block)
 constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize
+constexpr int GA_BLOCK = 1024;  // threads per block of k_guide_adam (32 row groups x 32 parameters)
 
 }  // namespace xt

@@ -677,23 +677,26 @@ __global__ __launch_bounds__(GW_BLOCK) void k_guide_wgrad(const __bf16* __restri
 }
 
 // Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the sum
-// of nparts partial rows (8 interleaved row groups per parameter, then the
-// groups in order) times grad_scale; b1t, b2t = beta^t.  32 parameters per
-// 256-thread block.
-__global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                                                    const float* __restrict__ partial, uint32_t nparts, float lr, float b1,
-                                                    float b2, float eps, float b1t, float b2t, float grad_scale) {
-    __shared__ float red[8][32];
+// of nparts partial rows (GA_GROUPS interleaved row groups per parameter,
+// then the groups in order: deterministic) times grad_scale; b1t, b2t =
+// beta^t.  32 parameters per block, one row group per wave half, so a
+// 256-row sum is 8 loads per thread with 16 waves per block in flight.
+__global__ __launch_bounds__(GA_BLOCK) void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                                                         const float* __restrict__ partial, uint32_t nparts, float lr,
+                                                         float b1, float b2, float eps, float b1t, float b2t,
+                                                         float grad_scale) {
+    constexpr int GA_GROUPS = GA_BLOCK / 32;
+    __shared__ float red[GA_GROUPS][32];
     const int pl = threadIdx.x & 31, rg = threadIdx.x >> 5;
     const int i = blockIdx.x * 32 + pl;
     float gp = 0.0f;
     if (i < G_NPARAMS)
-        for (uint32_t b = rg; b < nparts; b += 8) gp += partial[(size_t)b * G_NPARAMS + i];
+        for (uint32_t b = rg; b < nparts; b += GA_GROUPS) gp += partial[(size_t)b * G_NPARAMS + i];
     red[rg][pl] = gp;
     __syncthreads();
     if (rg != 0 || i >= G_NPARAMS) return;
     float g = 0.0f;
-    for (int r = 0; r < 8; ++r) g += red[r][pl];
+    for (int r = 0; r < GA_GROUPS; ++r) g += red[r][pl];
     g *= grad_scale;
     if (!isfinite(g)) g = 0.0f;
     const float mi = b1 * m[i] + (1.0f - b1) * g;

@@ -900,7 +900,7 @@ static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t
         g.step++;
         const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
         const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
-        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(256), 0, s->stream, g.P, g.m, g.v, g.partial,
+        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(GA_BLOCK), 0, s->stream, g.P, g.m, g.v, g.partial,
                            grid, g.cfg.learning_rate, g.cfg.beta1, g.cfg.beta2, g.cfg.adam_eps, b1t, b2t,
                            1.0f / (float)cnt);
         xt_status st = guide_prepare(s);
