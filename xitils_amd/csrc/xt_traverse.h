@@ -142,16 +142,6 @@ __device__ __forceinline__ void node_slabs(const float4* __restrict__ nd, const 
 // node_slabs.  Planes are picked per axis by the ray's direction sign.
 // Empty slots are inverted boxes (bytes 255 / 0) that carry a real child
 // code (quantize_node), so they are not masked here.
-//
-// AFFINE (the any-hit traversal) also folds 1/d into the node terms: a
-// plane's crossing is fma(q, 2^e / d, offset / d), two operations per
-// plane.  2^e / d is exact (a power-of-two scaling); offset / d rounds once
-// more, so the offsets are widened by 2^-21 instead of 2^-22, which covers
-// both roundings.  A zero direction component gives 0 * inf = NaN terms
-// that fminf / fmaxf ignore, so such an axis culls nothing: still
-// conservative, but an exactly axis-aligned any-hit query visits more nodes
-// (sampled shadow directions are never exactly axis-aligned).
-template <bool AFFINE>
 __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, const float4 w2, const float4 w3,
                                              const RayPre& r, float cull, float tn[4], int ch[4]) {
     const uint32_t meta = __float_as_uint(w0.w);
@@ -167,22 +157,17 @@ __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, c
     ch[1] = __float_as_int(w1.y);
     ch[2] = __float_as_int(w1.z);
     ch[3] = __float_as_int(w1.w);
+    const float stx = ldexpf(1.0f, ex), sty = ldexpf(1.0f, ey), stz = ldexpf(1.0f, ez);
     const float ox = w0.x - r.o.x, oy = w0.y - r.o.y, oz = w0.z - r.o.z;
-    constexpr float WIDEN = AFFINE ? 0x1p-21f : 0x1p-22f;
-    const float sx = copysignf(fabsf(ox) * WIDEN, r.ix), sy = copysignf(fabsf(oy) * WIDEN, r.iy),
-                sz = copysignf(fabsf(oz) * WIDEN, r.iz);
-    // node-side scale (1/d when AFFINE) and plane-side scale (1/d otherwise)
-    const float mx = AFFINE ? r.ix : 1.0f, my = AFFINE ? r.iy : 1.0f, mz = AFFINE ? r.iz : 1.0f;
-    const float px = AFFINE ? 1.0f : r.ix, py = AFFINE ? 1.0f : r.iy, pz = AFFINE ? 1.0f : r.iz;
-    const float stx = ldexpf(mx, ex), sty = ldexpf(my, ey), stz = ldexpf(mz, ez);
-    const float nox = (ox - sx) * mx, fox = (ox + sx) * mx, noy = (oy - sy) * my, foy = (oy + sy) * my,
-                noz = (oz - sz) * mz, foz = (oz + sz) * mz;
+    const float sx = copysignf(fabsf(ox) * 0x1p-22f, r.ix), sy = copysignf(fabsf(oy) * 0x1p-22f, r.iy),
+                sz = copysignf(fabsf(oz) * 0x1p-22f, r.iz);
+    const float nox = ox - sx, fox = ox + sx, noy = oy - sy, foy = oy + sy, noz = oz - sz, foz = oz + sz;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         auto q = [&](uint32_t v) { return (float)((v >> (8 * k)) & 255u); };
-        const float t0x = fmaf(q(nx), stx, nox) * px, t1x = fmaf(q(fx), stx, fox) * px;
-        const float t0y = fmaf(q(ny), sty, noy) * py, t1y = fmaf(q(fy), sty, foy) * py;
-        const float t0z = fmaf(q(nz), stz, noz) * pz, t1z = fmaf(q(fz), stz, foz) * pz;
+        const float t0x = fmaf(q(nx), stx, nox) * r.ix, t1x = fmaf(q(fx), stx, fox) * r.ix;
+        const float t0y = fmaf(q(ny), sty, noy) * r.iy, t1y = fmaf(q(fy), sty, foy) * r.iy;
+        const float t0z = fmaf(q(nz), stz, noz) * r.iz, t1z = fmaf(q(fz), stz, foz) * r.iz;
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         const float tmax = fminf(fminf(t1x, t1y), fminf(t1z, cull)) * BOX_MARGIN;
         tn[k] = tmin <= tmax ? tmin : INFINITY;
@@ -259,7 +244,7 @@ __device__ __forceinline__ bool visit_node(const DevScene& sc, const RayPre& r, 
             w2 = nd[2];
             w3 = nd[3];
         }
-        node_slabs_q<ANY>(w0, w1, w2, w3, r, t.cull, tn, ch);
+        node_slabs_q(w0, w1, w2, w3, r, t.cull, tn, ch);
     } else {
         node_slabs(nd, r, t.cull, tn);
 #pragma unroll
