@@ -24,7 +24,7 @@ from . import api
 from .quality import rel_mse
 
 
-def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=0.25, spp_per_call=4,
+def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=0.15, spp_per_call=4,
                 train_spp_per_call=1, seed=1, flags=0, max_depth=16, max_spp=1 << 20, keep_training_samples=True,
                 log=None):
     """Returns the error curve [(seconds, spp, relMSE)] of one progressive run.
