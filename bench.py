@@ -158,7 +158,7 @@ def guiding_main(args):
         "data": "synthetic",
         "config": {"workload": f"occluded_room_{W}x{H}_{info['num_prims']}tris_depth{max_depth}",
                    "budget_s": args.budget, "reference_spp": args.ref_spp, "guide_prob": 0.5,
-                   "train_fraction": 0.15, "learning_rate": 1e-2, "parallelism": "1 GPU"},
+                   "train_fraction": 0.15, "learning_rate": 5e-3, "parallelism": "1 GPU"},
         "time_speedup": res["time_speedup"],
         "guided_seconds_to_plain_relmse": res["guided_seconds_to_plain_relmse"],
         "plain": res["plain"],

@@ -312,7 +312,7 @@ xt_status xt_vmf_fit(xt_device* dev, uint32_t num_lobes, const float* samples, u
 
 typedef struct xt_guide_config {
     float guide_prob;            /* one-sample MIS probability of sampling the guide (0.5) */
-    float learning_rate;         /* Adam step size (1e-2) */
+    float learning_rate;         /* Adam step size (5e-3) */
     float beta1, beta2, adam_eps;/* Adam (0.9, 0.999, 1e-8) */
     uint32_t minibatch;          /* samples per Adam step (32768) */
     uint32_t max_steps_per_batch;/* Adam steps after one render batch (256) */

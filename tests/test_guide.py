@@ -169,8 +169,11 @@ def test_guide_train_step_is_adam(gscene):
     sig = np.abs(g) > 1e-3 * np.abs(g).max()
     dP = P1 - P
     # first Adam step: -lr * g / (|g| + eps) = -lr * sign(g) where |g| >> eps
-    np.testing.assert_allclose(dP[sig], -1e-2 * np.sign(g[sig]), rtol=1e-3, atol=1e-6)
-    assert np.abs(dP).max() <= 1e-2 * (1 + 1e-5)
+    # (lr: the library default the fixture's guide_init used)
+    lr = float(np.float32(api.Scene.guide_config().learning_rate))
+    assert lr > 0.0
+    np.testing.assert_allclose(dP[sig], -lr * np.sign(g[sig]), rtol=1e-3, atol=1e-6)
+    assert np.abs(dP).max() <= lr * (1 + 1e-5)
 
 
 @pytest.mark.gpu

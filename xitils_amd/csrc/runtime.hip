@@ -1512,7 +1512,7 @@ xt_status xt_render_get_stats(const xt_scene* s, xt_render_stats* out) {
 void xt_guide_config_default(xt_guide_config* c) {
     if (!c) return;
     c->guide_prob = 0.5f;
-    c->learning_rate = 1e-2f;
+    c->learning_rate = 5e-3f;
     c->beta1 = 0.9f;
     c->beta2 = 0.999f;
     c->adam_eps = 1e-8f;
