@@ -34,53 +34,70 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
-        L.orc_scene_new.restype = C.c_void_p
-        L.orc_scene_free.argtypes = [C.c_void_p]
-        L.orc_scene_add_material.argtypes = [C.c_void_p, C.c_uint32, f32p]
-        L.orc_scene_add_material.restype = C.c_int
-        L.orc_scene_add_object.argtypes = [C.c_void_p, f32p, C.c_void_p, C.c_int, i32p, C.c_int,
-                                           C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
-        L.orc_scene_add_object.restype = C.c_int
-        L.orc_scene_set_env.argtypes = [C.c_void_p, C.c_int, f32p, f32p, C.c_float, C.c_float, f32p]
-        L.orc_scene_set_env_map.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, f32p]
-        L.orc_env_radiance.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
-        L.orc_env_map_pdf.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
-        L.orc_env_map_sample.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
-        L.orc_scene_build.argtypes = [C.c_void_p]
-        L.orc_scene_num_prims.argtypes = [C.c_void_p]
-        L.orc_scene_num_prims.restype = C.c_uint32
-        L.orc_trace_closest.argtypes = [C.c_void_p, f32p, C.c_int, f32p, C.c_int, C.c_int]
-        L.orc_trace_any.argtypes = [C.c_void_p, f32p, C.c_int, u8p, C.c_int, C.c_int]
-        L.orc_render.argtypes = [C.c_void_p, f32p, i32p, f32p, C.c_void_p, C.c_void_p, u64p, C.c_int]
-        L.orc_philox.argtypes = [u32p, u32p, u32p]
-        L.orc_uniform.argtypes = [C.c_uint32] * 5 + [C.c_int]
-        L.orc_uniform.restype = C.c_float
-        L.orc_bsdf_sample.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
-        L.orc_bsdf_sample_walk.argtypes = [C.c_uint32, f32p, f32p, u32p, f32p]
-        L.orc_bsdf_eval_walk.argtypes = [C.c_uint32, f32p, f32p, f32p, u32p, f32p]
-        L.orc_nmap_terms.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
-        L.orc_bsdf_moments.argtypes = [C.c_uint32, f32p, f32p, C.c_uint32, C.c_int, C.c_int, C.c_int, f64p]
-        L.orc_shading_normals.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
-        L.orc_object_set_attributes.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_scene_add_texture.argtypes = [C.c_void_p, C.c_int, C.c_int, f32p]
-        L.orc_scene_add_texture.restype = C.c_int
-        L.orc_material_set_normal_map.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        L.orc_bsdf_eval.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
-        L.orc_ggx_D.argtypes = [C.c_float, f32p, f32p]
-        L.orc_ggx_D.restype = C.c_float
-        L.orc_ggx_G1.argtypes = [C.c_float, f32p, f32p]
-        L.orc_ggx_G1.restype = C.c_float
-        L.orc_tri_test.argtypes = [f32p, f32p, f32p]
-        L.orc_tri_test.restype = C.c_int
-        L.orc_camera_ray.argtypes = [f32p, C.c_float, C.c_float, f32p]
-        L.orc_sample_light.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, f32p]
-        L.orc_ref_bvh_info.argtypes = [C.c_void_p, C.c_int, i32p]
-        L.orc_vmf_eval.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
-        L.orc_vmf_sample.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
-        L.orc_vmf_fit.argtypes = [C.c_int, f32p, u32p, C.c_int, f32p, f32p, i32p, C.c_void_p, C.c_int]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+_slips = {}
+
+
+def slip_lib(k):
+    """The oracle built with planted slip k (xo_core.hpp, XO_SLIP): a
+    deliberately wrong checker, for the tests that show a pin catches it."""
+    if k not in _slips:
+        subprocess.run(["make", "-s", "-C", _HERE, f"slip-{k}"], check=True)
+        _slips[k] = _configure(C.CDLL(os.path.join(_HERE, "_build", f"liboracle_slip{k}.so")))
+    return _slips[k]
+
+
+def _configure(L):
+    """Declares the C signatures of an oracle library."""
+    L.orc_scene_new.restype = C.c_void_p
+    L.orc_scene_free.argtypes = [C.c_void_p]
+    L.orc_scene_add_material.argtypes = [C.c_void_p, C.c_uint32, f32p]
+    L.orc_scene_add_material.restype = C.c_int
+    L.orc_scene_add_object.argtypes = [C.c_void_p, f32p, C.c_void_p, C.c_int, i32p, C.c_int,
+                                       C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+    L.orc_scene_add_object.restype = C.c_int
+    L.orc_scene_set_env.argtypes = [C.c_void_p, C.c_int, f32p, f32p, C.c_float, C.c_float, f32p]
+    L.orc_scene_set_env_map.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, f32p]
+    L.orc_env_radiance.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+    L.orc_env_map_pdf.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+    L.orc_env_map_sample.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+    L.orc_scene_build.argtypes = [C.c_void_p]
+    L.orc_scene_num_prims.argtypes = [C.c_void_p]
+    L.orc_scene_num_prims.restype = C.c_uint32
+    L.orc_trace_closest.argtypes = [C.c_void_p, f32p, C.c_int, f32p, C.c_int, C.c_int]
+    L.orc_trace_any.argtypes = [C.c_void_p, f32p, C.c_int, u8p, C.c_int, C.c_int]
+    L.orc_render.argtypes = [C.c_void_p, f32p, i32p, f32p, C.c_void_p, C.c_void_p, u64p, C.c_int]
+    L.orc_philox.argtypes = [u32p, u32p, u32p]
+    L.orc_uniform.argtypes = [C.c_uint32] * 5 + [C.c_int]
+    L.orc_uniform.restype = C.c_float
+    L.orc_bsdf_sample.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
+    L.orc_bsdf_sample_walk.argtypes = [C.c_uint32, f32p, f32p, u32p, f32p]
+    L.orc_bsdf_eval_walk.argtypes = [C.c_uint32, f32p, f32p, f32p, u32p, f32p]
+    L.orc_nmap_terms.argtypes = [f32p, f32p, f32p, f32p, f32p, f32p]
+    L.orc_bsdf_moments.argtypes = [C.c_uint32, f32p, f32p, C.c_uint32, C.c_int, C.c_int, C.c_int, f64p]
+    L.orc_shading_normals.argtypes = [C.c_void_p, f32p, C.c_int, f32p]
+    L.orc_object_set_attributes.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_scene_add_texture.argtypes = [C.c_void_p, C.c_int, C.c_int, f32p]
+    L.orc_scene_add_texture.restype = C.c_int
+    L.orc_material_set_normal_map.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_bsdf_eval.argtypes = [C.c_uint32, f32p, f32p, f32p, f32p]
+    L.orc_ggx_D.argtypes = [C.c_float, f32p, f32p]
+    L.orc_ggx_D.restype = C.c_float
+    L.orc_ggx_G1.argtypes = [C.c_float, f32p, f32p]
+    L.orc_ggx_G1.restype = C.c_float
+    L.orc_tri_test.argtypes = [f32p, f32p, f32p]
+    L.orc_tri_test.restype = C.c_int
+    L.orc_camera_ray.argtypes = [f32p, C.c_float, C.c_float, f32p]
+    L.orc_sample_light.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, f32p]
+    L.orc_ref_bvh_info.argtypes = [C.c_void_p, C.c_int, i32p]
+    L.orc_vmf_eval.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
+    L.orc_vmf_sample.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
+    L.orc_vmf_fit.argtypes = [C.c_int, f32p, u32p, C.c_int, f32p, f32p, i32p, C.c_void_p, C.c_int]
+    L.orc_film_position.argtypes = [f32p, C.c_int, C.c_int, C.c_int, f32p]
+    return L
 
 
 def _ptr(a):
@@ -90,8 +107,9 @@ def _ptr(a):
 class OracleScene:
     """Oracle scene built from a xitils_amd.scenes.SceneDesc."""
 
-    def __init__(self, desc):
-        L = lib()
+    def __init__(self, desc, L=None):
+        L = L or lib()
+        self.L = L
         self.desc = desc
         self.h = C.c_void_p(L.orc_scene_new())
         from xitils_amd.scenes import pack_materials
@@ -133,30 +151,30 @@ class OracleScene:
 
     def __del__(self):
         try:
-            lib().orc_scene_free(self.h)
+            self.L.orc_scene_free(self.h)
         except Exception:
             pass
 
     @property
     def num_prims(self):
-        return lib().orc_scene_num_prims(self.h)
+        return self.L.orc_scene_num_prims(self.h)
 
     def trace_closest(self, rays, mode=0, nthreads=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         hits = np.zeros((rays.shape[0], 4), np.float32)
-        lib().orc_trace_closest(self.h, rays, rays.shape[0], hits, mode, nthreads)
+        self.L.orc_trace_closest(self.h, rays, rays.shape[0], hits, mode, nthreads)
         return hits
 
     def shading_normals(self, rays):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         out = np.zeros((rays.shape[0], 3), np.float32)
-        lib().orc_shading_normals(self.h, rays, rays.shape[0], out)
+        self.L.orc_shading_normals(self.h, rays, rays.shape[0], out)
         return out
 
     def trace_any(self, rays, mode=0, nthreads=0):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         occ = np.zeros(rays.shape[0], np.uint8)
-        lib().orc_trace_any(self.h, rays, rays.shape[0], occ, mode, nthreads)
+        self.L.orc_trace_any(self.h, rays, rays.shape[0], occ, mode, nthreads)
         return occ
 
     def render(self, width, height, spp, seed=1, max_depth=0, integrator=0, flags=0, spp_offset=0, stream=0,
@@ -169,36 +187,36 @@ class OracleScene:
         if film32 is None and film64 is None:
             film32 = np.zeros((height, width, 3), np.float32)
         stats = np.zeros(2, np.uint64)
-        lib().orc_render(self.h, cam, ip, fp, _ptr(film32), _ptr(film64), stats, nthreads)
+        self.L.orc_render(self.h, cam, ip, fp, _ptr(film32), _ptr(film64), stats, nthreads)
         self.last_stats = stats
         return film32 if film32 is not None else film64
 
     def env_radiance(self, dirs):
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         out = np.zeros_like(d)
-        lib().orc_env_radiance(self.h, d, d.shape[0], out)
+        self.L.orc_env_radiance(self.h, d, d.shape[0], out)
         return out
 
     def env_map_pdf(self, dirs):
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         out = np.zeros(d.shape[0], np.float32)
-        lib().orc_env_map_pdf(self.h, d, d.shape[0], out)
+        self.L.orc_env_map_pdf(self.h, d, d.shape[0], out)
         return out
 
     def env_map_sample(self, u2):
         u = np.ascontiguousarray(u2, np.float32).reshape(-1, 2)
         out = np.zeros((u.shape[0], 4), np.float32)
-        lib().orc_env_map_sample(self.h, u, u.shape[0], out)
+        self.L.orc_env_map_sample(self.h, u, u.shape[0], out)
         return out[:, :3], out[:, 3]
 
     def sample_light(self, seed, pixel, sample, slot, uniform_bary=False):
         out = np.zeros(12, np.float32)
-        lib().orc_sample_light(self.h, seed, pixel, sample, slot, int(uniform_bary), out)
+        self.L.orc_sample_light(self.h, seed, pixel, sample, slot, int(uniform_bary), out)
         return out
 
     def ref_bvh_info(self, obj=-1):
         out = np.zeros(2, np.int32)
-        lib().orc_ref_bvh_info(self.h, obj, out)
+        self.L.orc_ref_bvh_info(self.h, obj, out)
         return out
 
 
@@ -345,3 +363,11 @@ def vmf_fit(num_lobes, samples, offsets, init_u, nthreads=0, fixed_steps=None):
     fs = None if fixed_steps is None else np.ascontiguousarray(fixed_steps, np.int32)
     lib().orc_vmf_fit(num_lobes, s, off, nf, iu, enc, steps, None if fs is None else fs.ctypes.data, nthreads)
     return enc, steps
+
+
+def film_position(xyuv, width, height, L=None):
+    """RenderTargetTile::GenerateFilmPosition: rows (x, y, ux, uy) -> (fx, fy)."""
+    a = np.ascontiguousarray(xyuv, np.float32).reshape(-1, 4)
+    out = np.zeros((a.shape[0], 2), np.float32)
+    (L or lib()).orc_film_position(a, a.shape[0], width, height, out)
+    return out

@@ -224,6 +224,13 @@ void orc_render(void* s, const float* cam, const int32_t* ip, const float* fp, f
 
 // ---- unit-level entry points for the pin tests --------------------------
 
+// RenderTargetTile::GenerateFilmPosition for n (x, y, ux, uy) rows -> (fx, fy).
+void orc_film_position(const float* xyuv, int n, int W, int H, float* out) {
+    for (int i = 0; i < n; ++i)
+        filmPosition((int)xyuv[4 * i], (int)xyuv[4 * i + 1], W, H, xyuv[4 * i + 2], xyuv[4 * i + 3], &out[2 * i],
+                     &out[2 * i + 1]);
+}
+
 void orc_philox(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4) { philox4x32_10(ctr4, key2, out4); }
 
 float orc_uniform(uint32_t seed, uint32_t stream, uint32_t pixel, uint32_t sample, uint32_t slot, int dim) {

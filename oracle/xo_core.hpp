@@ -28,7 +28,22 @@
 //    TriangleIndexed.h:184-192 normalised-uniforms scheme is the default).
 //  * GGX-over-Lambert material (new, needed by the heightfield workload) uses
 //    the one-sample lobe mixture of GlossyWithHighLight (Material.h:134-175).
+//
+// Planted slips: building with -DXO_SLIP=k (k > 0) puts one deliberate
+// mistake into the restatement, so the pin tests can show that they fail
+// under it (tests/test_oracle_slips.py).  The oracle itself is built with
+// XO_SLIP = 0, which changes nothing.
+//   1  film position: y not flipped               (RenderTarget.h:77)
+//   2  film position: jitter not centred on the pixel corner (RenderTarget.h:78-79)
+//   3  roulette: survivor weight not divided by the survival probability (PathTracer.h:153)
+//   4  light pdf: no division by the number of light objects (Scene.h:46, 53)
+//   5  light pdf: no division by the mesh's triangle count (TriangleMesh.h:225, 234)
+//   6  light pdf: no division by the object's area scaling (Object.h:95, 110)
 #pragma once
+
+#ifndef XO_SLIP
+#define XO_SLIP 0
+#endif
 
 #include <algorithm>
 #include <array>
@@ -1127,9 +1142,9 @@ struct Scene {
         const Object& o = objects[oi];
         if (!materials[o.material].emissive()) return 0.0f;   // not in the light list
         float pdf = 1.0f / o.mesh.triArea(t);
-        pdf = pdf / (float)o.mesh.ntri();
-        pdf = pdf / o.areaScaling;
-        return pdf / (float)lights.size();
+        if (XO_SLIP != 5) pdf = pdf / (float)o.mesh.ntri();
+        if (XO_SLIP != 6) pdf = pdf / o.areaScaling;
+        return XO_SLIP == 4 ? pdf : pdf / (float)lights.size();
     }
 
     // Scene::sampleSurface (Scene.h:40-47) -> Object::sampleSurface
@@ -1199,9 +1214,10 @@ inline void filmPosition(int x, int y, int W, int H, float ux, float uy, float* 
     float px = (float)x, py = (float)y;
     px /= (float)W;
     py /= (float)H;
-    py = 1.0f - py;
-    px += (ux - 0.5f) / (float)W;
-    py += (uy - 0.5f) / (float)H;
+    if (XO_SLIP != 1) py = 1.0f - py;
+    const float half = XO_SLIP == 2 ? 0.0f : 0.5f;
+    px += (ux - half) / (float)W;
+    py += (uy - half) / (float)H;
     *fx = px; *fy = py;
 }
 
@@ -1271,7 +1287,7 @@ inline V3 evalStandard(const Scene& sc, const PathRng& rng, const Ray& camRay, c
         if (rp.maxDepth > 0 && (int)slot > rp.maxDepth) break;   // added bounce cap
         if (pathLength > rp.rrStart) {
             if (rng.uni(slot, DIM_RR) >= rp.rrProb) break;
-            weight /= rp.rrProb;
+            if (XO_SLIP != 3) weight /= rp.rrProb;
         }
         bool nextSampled = true;
         float pdfB;
@@ -1385,7 +1401,7 @@ inline V3 evalNaive(const Scene& sc, const PathRng& rng, const Ray& camRay, cons
         uint32_t slot = (uint32_t)pathLength;
         if (pathLength > rp.rrStart) {
             if (rng.uni(slot, DIM_RR) >= rp.rrProb) break;
-            weight /= rp.rrProb;
+            if (XO_SLIP != 3) weight /= rp.rrProb;
         }
         st->extRays++;
         if (sc.intersect(cur, &is)) {

@@ -1,0 +1,185 @@
+"""The benchmark workloads and the transform / light paths against the oracle
+on the GPU, through the C ABI.
+
+* The Cornell box at its stated size (256 x 256, 16 spp, depth 8, seed 1).
+* Rooms of the 64-scene batch workload rendered together by xt_render_batch
+  (graph-replayed bounce loops), each film compared with the oracle's.
+* The occluded room (500 k triangles, lit through a slit) plain against the
+  oracle, and guided against plain in the mean.
+* Instanced objects (rotation about tilted axes, uniform scale, translation;
+  Object.h:44-111) and several emitters, one instanced and scaled (Scene.h:
+  40-55, Object.h:91-111).
+* The pass-through stack whose every sample is 0 or 0.9^-m (roulette,
+  PathTracer.h:149-155).
+
+Tolerance: relMSE <= 1e-4 between GPU and oracle films at the same seed and
+spp (mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)); both draw the
+same counter-based random numbers, so the films differ only where a
+transcendental's last-ulp rounding (device libm vs glibc) flips a path."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from helpers import rel_mse, camera_rays, scene_rays, hit_prims
+from pin_scenes import instanced_room, passthrough_stack, two_lights
+from xitils_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    print("[gpu] configs: opening device 0", flush=True)
+    return api.Device(0)
+
+
+def _parity(fg, fo, spp, tag):
+    assert np.isfinite(fg).all()
+    err = rel_mse(fg / spp, fo / spp)
+    exact = float(np.mean(np.all(fg == fo, axis=2)))
+    print(f"[gpu]   {tag}: relMSE={err:.3e} bit-exact pixels={exact:.4f} mean gpu={fg.mean() / spp:.5f} "
+          f"oracle={fo.mean() / spp:.5f}", flush=True)
+    return err, exact
+
+
+def test_cornell_stated_size(dev):
+    """The Cornell configuration as stated: 256 x 256, 16 spp, depth 8, seed 1."""
+    desc = scenes.cornell(256, 256)
+    print("[gpu] cornell 256x256 16 spp depth 8 seed 1", flush=True)
+    g = api.Scene(dev, desc)
+    fg = g.render(256, 256, 16, seed=1, max_depth=8)
+    fo = po.OracleScene(desc).render(256, 256, 16, seed=1, max_depth=8)
+    err, exact = _parity(fg, fo, 16, "cornell")
+    assert err <= 1e-4 and exact > 0.5
+    assert g.stats()["camera_rays"] == 256 * 256 * 16
+
+
+BATCH_ROOMS = [0, 5, 11, 24, 36, 50]   # 9.9 k .. 262 k triangles, depth 4 .. 29
+
+
+def test_batch_rooms_against_oracle(dev):
+    """Rooms of the batch workload (seeded spheres and boxes, one area light)
+    rendered in one xt_render_batch call at 96 x 96 and 4 spp with their
+    drawn depths; every film against the oracle's."""
+    jobs, descs = [], []
+    for k in BATCH_ROOMS:
+        desc, _, _, _, depth = scenes.batch_scene(k)
+        print(f"[gpu] batch room {k}: {desc.num_triangles} triangles, depth {depth}", flush=True)
+        descs.append((desc, depth))
+        jobs.append((api.Scene(dev, desc), 96, 96, 4, None, dict(seed=100 + k, max_depth=depth)))
+    print("[gpu] xt_render_batch of the rooms", flush=True)
+    films, ms = api.render_batch(jobs)
+    for (desc, depth), fg, k in zip(descs, films, BATCH_ROOMS):
+        fo = po.OracleScene(desc).render(96, 96, 4, seed=100 + k, max_depth=depth)
+        err, exact = _parity(fg, fo, 4, desc.name)
+        assert err <= 1e-4 and exact > 0.5
+
+
+@pytest.fixture(scope="module")
+def room(dev):
+    desc = scenes.occluded_room(160, 90)
+    print(f"[gpu] occluded room: {desc.num_triangles} triangles", flush=True)
+    return desc, api.Scene(dev, desc)
+
+
+def test_occluded_room_plain_against_oracle(room):
+    desc, g = room
+    print("[gpu] occluded room 160x90 8 spp depth 16", flush=True)
+    fg = g.render(160, 90, 8, seed=4, max_depth=16)
+    fo = po.OracleScene(desc).render(160, 90, 8, seed=4, max_depth=16)
+    err, exact = _parity(fg, fo, 8, "occluded room")
+    assert err <= 1e-4 and exact > 0.5
+
+
+def test_occluded_room_guided_mean_matches_plain(room):
+    """Guided sampling (one-sample MIS of the trained network and the BSDF)
+    is unbiased: after online training its image mean matches plain path
+    tracing's within the Monte Carlo error of the two means."""
+    desc, g = room
+    W, H, spp = 160, 90, 256
+    g.guide_init(seed=1)
+    print("[gpu] occluded room: training the guide (8 passes)", flush=True)
+    for k in range(8):
+        g.render(W, H, 4, seed=200 + k, max_depth=16, flags=api.XT_FLAG_GUIDE | api.XT_FLAG_GUIDE_TRAIN)
+    assert g.stats()["guide_steps"] > 0
+    print(f"[gpu] occluded room: plain and guided {spp} spp", flush=True)
+    plain = g.render(W, H, spp, seed=31, max_depth=16) / spp
+    guided = g.render(W, H, spp, seed=32, max_depth=16, flags=api.XT_FLAG_GUIDE) / spp
+    assert np.isfinite(guided).all()
+    # standard error of each image mean from the means of 10 x 10 pixel blocks
+    def mean_and_se(img):
+        b = img.reshape(H // 10, 10, W // 10, 10, 3).mean(axis=(1, 3)).reshape(-1, 3)
+        return b.mean(0), b.std(0, ddof=1) / math.sqrt(b.shape[0])
+    mp, sp = mean_and_se(plain.astype(np.float64))
+    mg, sg = mean_and_se(guided.astype(np.float64))
+    z = np.abs(mp - mg) / np.sqrt(sp ** 2 + sg ** 2)
+    print(f"[gpu]   mean plain={mp} guided={mg} z={z}", flush=True)
+    assert np.all(z < 4.5)
+    assert np.allclose(mg, mp, rtol=0.05)
+
+
+def test_instanced_room_against_oracle(dev):
+    """Rotated, translated, uniformly scaled instances and two emitters (one
+    an instance scaled by 1.5): closest-hit ids against the oracle's
+    object-space traversal, then the render."""
+    desc = instanced_room(64, 64)
+    print("[gpu] instanced room: ray queries", flush=True)
+    g = api.Scene(dev, desc)
+    o = po.OracleScene(desc)
+    rng = np.random.default_rng(17)
+    rays = np.concatenate([scene_rays(rng, desc, 20000, 0.02, 0.98), camera_rays(desc, 64, 64)])
+    hg, ho = g.trace_closest(rays), o.trace_closest(rays, mode=0)
+    pg, pr = hit_prims(hg), hit_prims(ho)
+    same = pg == pr
+    print(f"[gpu]   ids equal {same.mean():.5f}", flush=True)
+    # instances are intersected in object space by the reference (Object.h:44-48)
+    # and in world space here: only rays grazing an edge may pick the neighbour
+    assert same.mean() >= 0.999
+    both = same & (pg != 0xFFFFFFFF)
+    assert np.all(np.abs(hg[both, 0] - ho[both, 0]) <= 1e-4 * np.maximum(1.0, ho[both, 0]))
+    print("[gpu] instanced room: render 64x64 8 spp depth 8", flush=True)
+    fg = g.render(64, 64, 8, seed=3, max_depth=8)
+    fo = o.render(64, 64, 8, seed=3, max_depth=8)
+    err, _ = _parity(fg, fo, 8, "instanced room")
+    assert err <= 1e-4
+    fg = g.render(64, 64, 8, seed=4, max_depth=8, flags=scenes.FLAG_UNIFORM_BARY)
+    fo = o.render(64, 64, 8, seed=4, max_depth=8, flags=scenes.FLAG_UNIFORM_BARY)
+    err, _ = _parity(fg, fo, 8, "instanced room, uniform barycentrics")
+    assert err <= 1e-4
+
+
+def test_two_lights_closed_form_on_gpu(dev):
+    """Direct light from two emitters (2 and 6 triangles, one instanced under
+    scale 2): the GPU film against the oracle's at the same seed, and its
+    mean against Lambert's polygon irradiance."""
+    desc, q, want = two_lights()
+    g = api.Scene(dev, desc)
+    spp = 40000
+    print("[gpu] two lights: 1 pixel, 40000 spp", flush=True)
+    fg = g.render(1, 1, spp, seed=9, max_depth=1, flags=scenes.FLAG_UNIFORM_BARY)
+    fo = po.OracleScene(desc).render(1, 1, spp, seed=9, max_depth=1, flags=scenes.FLAG_UNIFORM_BARY)
+    got = fg[0, 0] / spp
+    print(f"[gpu]   gpu={got} oracle={fo[0, 0] / spp} closed form={want}", flush=True)
+    assert np.allclose(got, fo[0, 0] / spp, rtol=1e-4)
+    assert np.allclose(got, want, rtol=0.02)
+
+
+def test_roulette_passthrough_on_gpu(dev):
+    """Every sample of the pass-through stack is 0 or 0.9^-m (m = 8 roulette
+    tests) on the GPU as in the oracle, sample for sample."""
+    n, m = 12, 8
+    desc = passthrough_stack(n, 64, 64)
+    g = api.Scene(dev, desc)
+    o = po.OracleScene(desc)
+    for integ in (0, 1):
+        print(f"[gpu] pass-through stack, integrator {integ}", flush=True)
+        gv = np.concatenate([g.render(64, 64, 1, seed=21, integrator=integ, spp_offset=k)[..., 0].ravel()
+                             for k in range(8)])
+        ov = np.concatenate([o.render(64, 64, 1, seed=21, integrator=integ, spp_offset=k)[..., 0].ravel()
+                             for k in range(8)])
+        assert np.array_equal(gv, ov)
+        alive = gv != 0
+        assert np.all(np.abs(gv[alive] - 0.9 ** -m) <= 1e-5 * 0.9 ** -m)
+        assert abs(alive.mean() - 0.9 ** m) < 0.02

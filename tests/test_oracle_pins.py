@@ -522,3 +522,114 @@ def test_pinhole_camera_closed_form():
     assert abs(d[0] / d[2] - math.tan(fov / 2)) < 1e-6
     assert abs(d[1] / d[2] - math.tan(fov / 2 * aspect)) < 1e-6
     assert abs(np.linalg.norm(d) - 1) < 1e-6
+
+
+# ------------------------------------------------------------------ film position (R26)
+
+def check_film_position(L=None):
+    """RenderTargetTile::GenerateFilmPosition (RenderTarget.h:68-80): the
+    jitter span of pixel x is one pixel wide and centred on its corner x / W
+    (quirk Q8), neighbouring spans abut so the spans tile the film, and y is
+    flipped (row 0 at the top, fy near 1)."""
+    W, H = 7, 5
+    ulast = float(np.float32(1 - 2.0 ** -24))       # the largest uniform the RNG returns
+    xs, ys = np.meshgrid(np.arange(W), np.arange(H), indexing="ij")
+    xs, ys = xs.ravel(), ys.ravel()
+    def fp(ux, uy):
+        rows = np.stack([xs, ys, np.full(xs.shape, ux), np.full(xs.shape, uy)], 1)
+        return po.film_position(rows, W, H, L=L).astype(np.float64)
+    mid, lo, hi = fp(0.5, 0.5), fp(0.0, 0.0), fp(ulast, ulast)
+    # centre of the span: the pixel corner, y flipped
+    assert np.allclose(mid[:, 0], xs / W, atol=1e-7)
+    assert np.allclose(mid[:, 1], 1 - ys / H, atol=1e-7)
+    # span: one pixel, centred on the corner
+    assert np.allclose(lo[:, 0], xs / W - 0.5 / W, atol=1e-6)
+    assert np.allclose(hi[:, 0] - lo[:, 0], 1.0 / W, atol=1e-6)
+    assert np.allclose(lo[:, 1], 1 - ys / H - 0.5 / H, atol=1e-6)
+    assert np.allclose(hi[:, 1] - lo[:, 1], 1.0 / H, atol=1e-6)
+    # the spans of a row tile [-1/2W, 1 - 1/2W): pixel x + 1 starts where x ends
+    row = ys == 0
+    order = np.argsort(xs[row])
+    assert np.allclose(lo[row][order][1:, 0], hi[row][order][:-1, 0], atol=1e-6)
+    # row 0 is the top of the image: camera rays of pixel (0, 0) point up-left
+    cam = scenes.cornell().camera.packed()
+    r = po.camera_ray(cam, float(mid[0, 0]), float(mid[0, 1]))
+    assert r[4] < 0 and r[5] > 0
+
+
+def test_film_position():
+    check_film_position()
+
+
+# ------------------------------------------------------------------ roulette weighting (R10, R11)
+
+def check_roulette_passthrough(L=None, integrator=0):
+    """Every path through a stack of n pass-through planes (weight 1 each)
+    meets one roulette test per vertex beyond pathLength 5 (PathTracer.h:
+    149-155): m = n - 4 tests for n >= 5, in both estimators.  A surviving
+    path carries exactly 1 / 0.9^m of sky radiance 1, a killed one 0, so each
+    sample is 0 or 0.9^-m, the surviving fraction is 0.9^m and the mean is 1
+    (the roulette is unbiased).  With a bounce cap below n nothing reaches
+    the sky."""
+    from pin_scenes import passthrough_stack
+    n = 12
+    m = n - 4
+    desc = passthrough_stack(n, 64, 64)
+    o = po.OracleScene(desc, L=L)
+    vals = np.concatenate([o.render(64, 64, 1, seed=21, integrator=integrator, spp_offset=k)[..., 0].ravel()
+                           for k in range(16)])
+    v_m = 0.9 ** -m
+    alive = vals != 0
+    assert np.all(np.abs(vals[alive] - v_m) <= 1e-5 * v_m)
+    p = 0.9 ** m
+    sigma = math.sqrt(p * (1 - p) / vals.size)
+    assert abs(alive.mean() - p) < 5 * sigma
+    assert abs(vals.mean() - 1.0) < 5 * sigma * v_m
+    # bounce cap: n planes need max_depth >= n
+    assert o.render(16, 16, 4, seed=2, integrator=integrator, max_depth=n - 1).max() == 0.0
+    assert o.render(16, 16, 4, seed=2, integrator=integrator, max_depth=n).max() > 0.0
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_roulette_passthrough_stack(integrator):
+    check_roulette_passthrough(integrator=integrator)
+
+
+# ------------------------------------------------------------------ several lights, instanced emitter (R19-R22)
+
+def check_two_lights(L=None):
+    """Direct light at a floor point from two emitters with unequal triangle
+    counts (2 and 6, of unequal areas) and areas, one an instance under a
+    uniform scale 2, a rotation and a translation: the NEE + BSDF MIS
+    estimate (uniform area barycentrics, so unbiased) converges to
+    a / pi * sum_k Le_k E_k with E_k Lambert's polygon irradiance.  This pins
+    the light-object choice (1 / lights.size(), Scene.h:40-55), the triangle
+    choice (1 / ntri, TriangleMesh.h:216-235) and the instance's area scaling
+    (Object.h:91-111)."""
+    from pin_scenes import two_lights
+    desc, q, want = two_lights()
+    o = po.OracleScene(desc, L=L)
+    spp = 40000
+    f = o.render(1, 1, spp, seed=9, max_depth=1, flags=scenes.FLAG_UNIFORM_BARY)
+    got = f[0, 0] / spp
+    assert np.allclose(got, want, rtol=0.02), (got, want)
+    # the sampled light pdf of the instanced strip is 1 / (world area * ntri * nlights)
+    areas = []
+    for k in range(200):
+        out = o.sample_light(5, k, 0, 1, uniform_bary=True)
+        p, pdf = out[:3], out[9]
+        if abs(p[1] - 1.0) > 1e-4:      # a point of B
+            areas.append(1.0 / (pdf * 6 * 2))
+    assert len(areas) > 50
+    tri_world = []
+    from pin_scenes import world_points
+    ob = desc.objects[2]
+    for t in ob.indices:
+        w = world_points(ob, ob.positions[t])
+        tri_world.append(np.linalg.norm(np.cross(w[1] - w[0], w[2] - w[0])) / 2)
+    for a in areas:
+        assert min(abs(a - t) for t in tri_world) <= 1e-4 * a
+
+
+def test_two_lights_instanced_emitter():
+    check_two_lights()

@@ -216,7 +216,10 @@ def test_gpu_vmf_eval_and_sample_match_oracle(dev):
         dg = api.vmf_sample(dev, encs, u, idx)
         do = np.concatenate([po.vmf_sample(encs[i], u[k:k + 1]) for k, i in enumerate(idx)])
         err = np.abs(dg - do).max()
-        print(f"[gpu]   max |pdf rel err| {np.max(np.abs(pg / po_ - 1)):.2e}, max |dir err| {err:.2e}", flush=True)
+        pos = po_ > 0   # a lobe far from a direction can underflow the pdf to 0
+        rel = np.max(np.abs(pg[pos] / po_[pos] - 1)) if pos.any() else 0.0
+        print(f"[gpu]   max |pdf rel err| {rel:.2e} over {pos.sum()} positive pdfs, max |dir err| {err:.2e}",
+              flush=True)
         assert err < 2e-5
     # out-of-range distribution index -> NaN, no access
     p = api.vmf_eval(dev, encs, w[:4], np.array([0, 7, 100, 1], np.uint32))

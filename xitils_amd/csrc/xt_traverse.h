@@ -179,13 +179,15 @@ __device__ __forceinline__ void node_slabs_q(const float4 w0, const float4 w1, c
 // node visits of each ray are LDS reads instead of global gathers.
 constexpr int TOP_F4 = BVH_QUANT && BVH_TOP_MAX_NODES > 0 ? BVH_TOP_MAX_NODES * NODE_F4 : 1;
 
-// Block-wide copy into LDS; every thread of the block must call it.
+// Block-wide copy into LDS; every thread of the block must call it.  A build
+// without staged levels (XT_BVH_TOP_LEVELS = 0, the default) copies nothing
+// and needs no barrier.
 __device__ __forceinline__ void stage_top_nodes(float4* top, const DevScene& sc) {
     if constexpr (BVH_QUANT && BVH_TOP_MAX_NODES > 0) {
         const uint32_t n = min(sc.top_nodes, (uint32_t)BVH_TOP_MAX_NODES) * NODE_F4;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) top[i] = sc.nodes[i];
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Fetch counters of the counting build (algorithmic traffic accounting).
