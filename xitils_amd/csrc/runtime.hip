@@ -1227,28 +1227,20 @@ static uint32_t graph_capacity(uint64_t npix, uint32_t spp) {
     return (uint32_t)std::min<uint64_t>(std::min<uint32_t>(path_capacity(), 1u << 22), npix * spp);
 }
 
-// The bounce loop of one batch, captured from s->stream into a graph.
-static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* d_film, int f64,
-                               RenderGraph* out) {
-    const uint64_t npix = (uint64_t)p->width * p->height;
-    const uint32_t cap = graph_capacity(npix, p->spp);
-    xt_status st = ensure_buffers(s, cap);
-    if (st != XT_OK) return st;
-    RenderBuffers& rb = s->rb;
-    const bool guide = (p->flags & XT_FLAG_GUIDE) != 0;
-    if (guide && !rb.lobes) {
-        drop_graphs(s);
-        XT_HIP(dalloc(&rb.lobes, (size_t)G_LOBE_FIELDS * rb.capacity));
-    }
-    if (!s->d_cursor) XT_HIP(dalloc(&s->d_cursor, 1));
-    for (int k = 0; k < 2; ++k) rb.q[k].rec = nullptr;
-    BatchPlan plan;
-    plan.npix = (uint32_t)npix;
-    plan.S = npix <= cap ? (uint32_t)std::min<uint64_t>(cap / npix, p->spp) : 1u;
-    plan.pix_chunk = (uint32_t)std::min<uint64_t>(npix, cap);
-    plan.spp = p->spp;
-    plan.spp_offset = p->spp_offset;
-    const uint32_t maxpaths = plan.pix_chunk * plan.S;
+// Path state of one count-driven batch (the graph renders' buffers).
+struct BounceBufs {
+    PathQueue q[2];
+    ShadowQueue sq;
+    float4* hits;
+    float4* radiance;
+    float4* pending;
+    uint32_t* counts;        // [0], [1]: path queues, [2]: shadow queue
+    uint32_t* ovf;           // traversal spill columns for tblocks blocks
+    uint32_t capacity;       // paths
+};
+
+// WaveParams of a count-driven render: the batch fields come from `cursor`.
+static WaveParams cursor_wave_params(const xt_camera* cam, const xt_render_params* p, const BatchCursor* cursor) {
     WaveParams wp{};
     wp.width = p->width;
     wp.height = p->height;
@@ -1264,42 +1256,99 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
     wp.tan_x = cam->tan_x;
     wp.tan_y = cam->tan_y;
     std::memcpy(wp.cam, cam->cam_to_world, sizeof(wp.cam));
-    wp.cursor = s->d_cursor;
+    wp.cursor = cursor;
+    return wp;
+}
+
+// Batch schedule of a count-driven render with `cap` paths per batch.
+static BatchPlan cursor_plan(const xt_render_params* p, uint32_t cap, uint32_t* nbatches) {
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    BatchPlan plan;
+    plan.npix = (uint32_t)npix;
+    plan.S = npix <= cap ? (uint32_t)std::min<uint64_t>(cap / npix, p->spp) : 1u;
+    plan.pix_chunk = (uint32_t)std::min<uint64_t>(npix, cap);
+    plan.spp = p->spp;
+    plan.spp_offset = p->spp_offset;
+    *nbatches = ((p->spp + plan.S - 1) / plan.S) * (uint32_t)((npix + plan.pix_chunk - 1) / plan.pix_chunk);
+    return plan;
+}
+
+// Guide inputs of a count-driven batch (lobes_mk == nullptr: unguided).
+struct BounceGuide {
+    uint2* lobes_mk = nullptr;
+    float2* lobes2 = nullptr;
+    uint32_t stride = 0;
+};
+
+// Enqueues the bounce loop of the cursor's batch on s->stream: generate,
+// max_depth + 1 rounds of extend / (guide) / shade / shadow, accumulate,
+// advance the cursor.  Every kernel reads its queue length from device memory.
+static void enqueue_cursor_batch(xt_scene* s, const xt_render_params* p, const WaveParams& wp, const BounceBufs& b,
+                                 const BatchPlan& plan, BatchCursor* cursor, uint32_t tblocks, const BounceGuide& gd,
+                                 void* d_film, int f64) {
+    const uint32_t maxpaths = plan.pix_chunk * plan.S;
     const uint32_t gblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((maxpaths + 255) / 256, (uint64_t)s->dev->num_cu * 8));
     const uint32_t ablocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((plan.pix_chunk + 255) / 256, (uint64_t)s->dev->num_cu * 8));
     const uint32_t sblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((maxpaths + SHADE_BLOCK - 1) / SHADE_BLOCK, s->shade_grid));
-    const uint32_t tblocks = trace_blocks(s, maxpaths);
+    const bool guide = gd.lobes_mk != nullptr;
     GuideState& g = s->guide;
-    uint2* l4 = guide ? (uint2*)rb.lobes : nullptr;
-    float2* l2 = guide ? (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity) : nullptr;
     const uint32_t ib = std::max(1u, std::min((maxpaths + GI_BLOCK - 1) / GI_BLOCK, (uint32_t)s->dev->num_cu * 4));
-    XT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, s->stream, wp, rb.q[0], rb.radiance);
-    hipLaunchKernelGGL(k_count_from_cursor, dim3(1), dim3(1), 0, s->stream, rb.counts + 0, s->d_cursor);
+    hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(256), 0, s->stream, wp, b.q[0], b.radiance);
+    hipLaunchKernelGGL(k_count_from_cursor, dim3(1), dim3(1), 0, s->stream, b.counts + 0, cursor);
     int cur = 0;
     for (int it = 0; it <= p->max_depth; ++it) {
         hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
-                           rb.q[cur].o, rb.q[cur].d, 1, rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr,
-                           rb.counts + (1 - cur), rb.counts + 2);
+                           b.q[cur].o, b.q[cur].d, 1, b.counts + cur, 0u, b.hits, nullptr, 0, 0, b.ovf, nullptr,
+                           b.counts + (1 - cur), b.counts + 2);
         GuideRender gr{};
         if (guide) {
-            hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, rb.q[cur],
-                               rb.counts + cur, rb.hits, l4, l2, rb.capacity, g.cfg.max_guided_vertex);
-            gr.lobes_mk = l4;
-            gr.lobes2 = l2;
-            gr.stride = rb.capacity;
+            hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds, b.q[cur],
+                               b.counts + cur, b.hits, gd.lobes_mk, gd.lobes2, gd.stride, g.cfg.max_guided_vertex);
+            gr.lobes_mk = gd.lobes_mk;
+            gr.lobes2 = gd.lobes2;
+            gr.stride = gd.stride;
             gr.prob = g.cfg.guide_prob;
             gr.max_vertex = g.cfg.max_guided_vertex;
         }
         hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
-                           rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur], rb.counts + (1 - cur), rb.sq,
-                           rb.counts + 2, rb.radiance, rb.pending, gr);
-        hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.sq, rb.counts + 2,
-                           rb.radiance, rb.pending, s->d_ovf, nullptr, rb.counts, nullptr);
+                           b.q[cur], b.counts + cur, b.hits, b.q[1 - cur], b.counts + (1 - cur), b.sq,
+                           b.counts + 2, b.radiance, b.pending, gr);
+        hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, b.sq, b.counts + 2,
+                           b.radiance, b.pending, b.ovf, nullptr, b.counts, nullptr);
         cur = 1 - cur;
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(ablocks), dim3(256), 0, s->stream, wp, rb.radiance, d_film, f64);
-    hipLaunchKernelGGL(k_advance_cursor, dim3(1), dim3(1), 0, s->stream, s->d_cursor, plan);
+    hipLaunchKernelGGL(k_accumulate, dim3(ablocks), dim3(256), 0, s->stream, wp, b.radiance, d_film, f64);
+    hipLaunchKernelGGL(k_advance_cursor, dim3(1), dim3(1), 0, s->stream, cursor, plan);
+}
+
+// The bounce loop of one batch, captured from s->stream into a graph.
+static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* d_film, int f64,
+                               RenderGraph* out) {
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    const uint32_t cap = graph_capacity(npix, p->spp);
+    xt_status st = ensure_buffers(s, cap);
+    if (st != XT_OK) return st;
+    RenderBuffers& rb = s->rb;
+    const bool guide = (p->flags & XT_FLAG_GUIDE) != 0;
+    if (guide && !rb.lobes) {
+        drop_graphs(s);
+        XT_HIP(dalloc(&rb.lobes, (size_t)G_LOBE_FIELDS * rb.capacity));
+    }
+    if (!s->d_cursor) XT_HIP(dalloc(&s->d_cursor, 1));
+    for (int k = 0; k < 2; ++k) rb.q[k].rec = nullptr;
+    uint32_t nbatches = 0;
+    const BatchPlan plan = cursor_plan(p, cap, &nbatches);
+    const WaveParams wp = cursor_wave_params(cam, p, s->d_cursor);
+    BounceBufs b{{rb.q[0], rb.q[1]}, rb.sq, rb.hits, rb.radiance, rb.pending, rb.counts, s->d_ovf, rb.capacity};
+    BounceGuide gd;
+    if (guide) {
+        gd.lobes_mk = (uint2*)rb.lobes;
+        gd.lobes2 = (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity);
+        gd.stride = rb.capacity;
+    }
+    const uint32_t tblocks = trace_blocks(s, plan.pix_chunk * plan.S);
+    XT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_cursor_batch(s, p, wp, b, plan, s->d_cursor, tblocks, gd, d_film, f64);
     hipError_t launch_err = hipGetLastError();
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
@@ -1314,7 +1363,7 @@ static xt_status capture_graph(xt_scene* s, const xt_camera* cam, const xt_rende
     if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(e));
     out->exec = exec;
     out->plan = plan;
-    out->nbatches = ((p->spp + plan.S - 1) / plan.S) * (uint32_t)((npix + plan.pix_chunk - 1) / plan.pix_chunk);
+    out->nbatches = nbatches;
     return XT_OK;
 }
 

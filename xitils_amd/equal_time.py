@@ -23,8 +23,12 @@ import numpy as np
 from . import api
 from .quality import rel_mse
 
+# Share of the budget spent training online (one sample per pixel per call,
+# Adam steps after each call); chosen on the occluded room (see compare).
+TRAIN_FRACTION = 0.15
 
-def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=0.15, spp_per_call=4,
+
+def progressive(scene, width, height, budget_s, ref, *, guided=False, train_fraction=TRAIN_FRACTION, spp_per_call=4,
                 train_spp_per_call=1, seed=1, flags=0, max_depth=16, max_spp=1 << 20, keep_training_samples=True,
                 log=None):
     """Returns the error curve [(seconds, spp, relMSE)] of one progressive run.
@@ -86,34 +90,49 @@ def time_to_reach(curve, target):
     return None
 
 
-def compare(scene, width, height, budget_s, ref, *, log=None, guide_seeds=None, **kw):
+def compare(scene, width, height, budget_s, ref, *, log=None, guide_seeds=None, render_seeds=None,
+            guide_config=None, **kw):
     """Runs plain, then guided for the same budget; returns a summary dict.
-    With guide_seeds, the guided run is repeated from a fresh network per
-    seed (scene.guide_init(seed=...)) and the run with the median ratio is
-    reported, all runs listed (training is stochastic: single runs spread by
-    about 10%).  Without, one guided run uses the scene's current network."""
-    plain, _ = progressive(scene, width, height, budget_s, ref, guided=False, log=log, **kw)
-    e_plain = plain[-1][2]
+    Plain runs once per render seed (render_seeds, default [kw's seed or 1]);
+    their mean final relMSE is the plain error.  With guide_seeds, the guided
+    run is repeated from a fresh network per seed (scene.guide_init(seed=...,
+    **guide_config)), each with a render seed of its own (100 + the plain
+    seeds, cycled), and the run with the median ratio is reported, all runs
+    listed (training is stochastic: single runs spread by about 10%).
+    Without, one guided run uses the scene's current network."""
+    seeds = list(render_seeds) if render_seeds else [kw.pop("seed", 1)]
+    kw.pop("seed", None)
+    plains = []
+    for rs in seeds:
+        curve, _ = progressive(scene, width, height, budget_s, ref, guided=False, log=log, seed=rs, **kw)
+        plains.append({"seed": rs, "curve": curve})
+    e_plain = float(np.mean([p["curve"][-1][2] for p in plains]))
+    t_plain = float(np.mean([p["curve"][-1][0] for p in plains]))
     runs = []
-    for seed in (guide_seeds or [None]):
-        if seed is not None:
-            scene.guide_init(seed=seed)
-        guided, gstats = progressive(scene, width, height, budget_s, ref, guided=True, log=log, **kw)
+    for i, gseed in enumerate(guide_seeds or [None]):
+        if gseed is not None:
+            scene.guide_init(seed=gseed, **(guide_config or {}))
+        rs = 100 + seeds[i % len(seeds)]
+        guided, gstats = progressive(scene, width, height, budget_s, ref, guided=True, log=log, seed=rs, **kw)
         t_reach = time_to_reach(guided, e_plain)
-        runs.append({"seed": seed, "curve": guided, "stats": gstats, "relmse": guided[-1][2],
+        runs.append({"seed": gseed, "render_seed": rs, "curve": guided, "stats": gstats, "relmse": guided[-1][2],
                      "ratio": e_plain / guided[-1][2], "t_reach": t_reach})
     med = sorted(runs, key=lambda r: r["ratio"])[len(runs) // 2]
     guided, gstats, t_reach = med["curve"], med["stats"], med["t_reach"]
+    plain = plains[0]["curve"]
     return {
         "budget_s": budget_s,
-        "plain": {"spp": plain[-1][1], "seconds": round(plain[-1][0], 4), "relmse": e_plain},
+        "plain": {"spp": int(np.mean([p["curve"][-1][1] for p in plains])), "seconds": round(t_plain, 4),
+                  "relmse": e_plain},
+        "plain_runs": [{"seed": p["seed"], "spp": p["curve"][-1][1], "relmse": p["curve"][-1][2]} for p in plains],
         "guided": {"spp": guided[-1][1], "seconds": round(guided[-1][0], 4), "relmse": guided[-1][2],
                    "seed": med["seed"], **gstats},
         "relmse_ratio_equal_time": med["ratio"],
         "guided_seconds_to_plain_relmse": t_reach,
-        "time_speedup": (plain[-1][0] / t_reach) if t_reach else None,
-        "runs": [{"seed": r["seed"], "spp": r["curve"][-1][1], "relmse": r["relmse"], "ratio": round(r["ratio"], 4),
-                  "time_speedup": (plain[-1][0] / r["t_reach"]) if r["t_reach"] else None} for r in runs],
+        "time_speedup": (t_plain / t_reach) if t_reach else None,
+        "runs": [{"seed": r["seed"], "render_seed": r["render_seed"], "spp": r["curve"][-1][1], "relmse": r["relmse"],
+                  "ratio": round(r["ratio"], 4), "time_speedup": (t_plain / r["t_reach"]) if r["t_reach"] else None}
+                 for r in runs],
         "curves": {"plain": [(round(a, 4), b, c) for a, b, c in plain[:: max(1, len(plain) // 40)]],
                    "guided": [(round(a, 4), b, c) for a, b, c in guided[:: max(1, len(guided) // 40)]]},
     }
