@@ -49,8 +49,10 @@ def parse():
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["heightfield", "guiding", "batch", "nmap", "rays"], default="heightfield",
-                    help="guiding: equal-time relMSE of guided vs plain path tracing on the occluded room; "
+    ap.add_argument("--workload", choices=["heightfield", "cornell", "guiding", "batch", "nmap", "rays"],
+                    default="heightfield",
+                    help="cornell: the 256x256, 16 spp, depth 8, seed 1 Cornell box; "
+                         "guiding: equal-time relMSE of guided vs plain path tracing on the occluded room; "
                          "batch: 64 small scenes rendered one by one and as one graph-replayed batch; "
                          "nmap: the displaced plane with microfacet-based normal mapping, 16 camera variants")
     ap.add_argument("--plane-grid", type=int, default=4096, help="nmap: vertices per side of the displaced plane")
@@ -66,7 +68,9 @@ def parse():
     ap.add_argument("--batch-side", type=int, default=0, help="batch: override every scene's image side")
     ap.add_argument("--batch-spp", type=int, default=0, help="batch: override every scene's spp")
     ap.add_argument("--budget", type=float, default=6.0, help="guiding: seconds of rendering per method")
-    ap.add_argument("--ref-spp", type=int, default=16384, help="guiding: samples per pixel of the FP64 reference")
+    ap.add_argument("--ref-spp", type=int, default=16384,
+                    help="samples per pixel of the FP64 reference each image's relMSE is stated against")
+    ap.add_argument("--no-ref", action="store_true", help="skip the converged reference (no relMSE in the line)")
     ap.add_argument("--guided-runs", type=int, default=3,
                     help="guiding: guided runs from fresh networks (seeds 5..4+n); the median ratio is reported")
     ap.add_argument("--room-tris", type=int, default=500_000)
@@ -265,6 +269,103 @@ def guiding_main(args):
                         "frac": round(tf / 2500.0, 4)}
     out["mlp"] = mlp
     out["curves"] = head["curves"]
+    print(json.dumps(out), flush=True)
+
+
+def image_quality(scene, W, H, spp, ref_spp, **kw):
+    """relMSE of one `spp`-sample image (the bench's seed) against a
+    `ref_spp`-sample reference accumulated in FP64 on the GPU with another
+    seed; both outside the timed region."""
+    from xitils_amd import quality
+    img = scene.render(W, H, spp, **kw) / spp
+    t0 = time.perf_counter()
+    kw = {k: v for k, v in kw.items() if k != "seed"}
+    ref = quality.reference_image(scene, W, H, spp=ref_spp, **kw)
+    return {"relmse": quality.rel_mse(img, ref), "image_spp": spp, "reference_spp": ref_spp,
+            "reference": "same renderer, FP64 film, seed 0x5EED", "reference_seconds": round(time.perf_counter() - t0, 2)}
+
+
+def cornell_main(args):
+    """The Cornell box as configured: unit room of 5 Lambert walls (10
+    triangles; 0.73 grey, red (0.63, 0.065, 0.05), green (0.14, 0.45, 0.09)),
+    a 0.3 short box and a 0.6 tall box (12 triangles each, rotated -18/+18
+    degrees), a 0.25 x 0.2 ceiling light of radiance (17, 12, 4); 256 x 256,
+    16 spp, max depth 8, seed 1, StandardPathTracer.  One step renders the
+    image once (1,048,576 paths) through xt_render into a device film."""
+    import torch
+    from xitils_amd import api, scenes
+    W = H = 256
+    spp, depth = 16, 8
+    desc = scenes.cornell(W, H)
+    dev = api.Device(0)
+    scene = api.Scene(dev, desc)
+    info = scene.info()
+    film = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    kw = dict(seed=1, max_depth=depth)
+    scene.set_counting(True)
+    scene.render_device(film.data_ptr(), W, H, spp, **kw)
+    cst = scene.stats()
+    scene.set_counting(False)
+    steps = max(args.steps, 50)
+    for _ in range(max(args.warmup, 5)):
+        scene.render_device(film.data_ptr(), W, H, spp, **kw)
+    agg = {"ms_extend": 0.0, "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "extension_rays": 0,
+           "shadow_rays": 0, "extend_launches": 0}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scene.render_device(film.data_ptr(), W, H, spp, **kw)
+        st = scene.stats()
+        for key in agg:
+            agg[key] += st[key]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    paths = W * H * spp * steps
+    n_ext = cst["extension_rays"]
+    node_bytes = info["bytes_nodes"] // max(1, info["bvh_nodes"])
+    bytes_ext = n_ext * EXT_RAY_IO_BYTES + cst["ext_node_fetches"] * node_bytes + cst["ext_tri_fetches"] * TRI_BYTES
+    launches = agg["extend_launches"] / steps
+    out = {
+        "metric": "Mpaths/s (Cornell box 256x256, 16 spp)",
+        "value": round(paths / wall / 1e6, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 5),
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cornell_256x256_16spp_depth8_seed1", "triangles": info["num_prims"], "width": W,
+                   "height": H, "spp": spp, "max_depth": depth, "integrator": "StandardPathTracer (NEE+BSDF MIS)",
+                   "parallelism": "1 GPU"},
+        "mrays_per_s": round((agg["extension_rays"] + agg["shadow_rays"]) / wall / 1e6, 2),
+        "stage_ms_per_step": {k: round(agg[k] / steps, 3) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")},
+        "roofline": kernel_roofline("k_trace<closest> (extend)", bytes_ext / launches,
+                                    agg["ms_extend"] / max(1, agg["extend_launches"]), {},
+                                    {"note": "36 triangles: the tree and triangles sit in L1/LDS-sized caches; "
+                                             "launch and latency bound, Mrays/s is the figure of merit"}),
+    }
+    if not args.no_ref:
+        out["quality"] = image_quality(scene, W, H, spp, args.ref_spp, **kw)
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = pyoracle.OracleScene(desc)
+        reps, t0 = 0, time.perf_counter()
+        while reps == 0 or time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            o.render(W, H, spp, nthreads=threads, **kw)
+            reps += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(W * H * spp * reps / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the full 256x256 x 16 spp image rendered {reps} times by the oracle C++ "
+                                         f"restatement (OpenMP over 16x16 tiles), {dt:.1f} s wall"}
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 
@@ -654,6 +755,8 @@ def main():
         return guiding_main(args)
     if args.workload == "batch":
         return batch_main(args)
+    if args.workload == "cornell":
+        return cornell_main(args)
     if args.workload == "nmap":
         return nmap_main(args)
     if args.workload == "rays":
@@ -748,6 +851,10 @@ def main():
                   "bvh_nodes": info["bvh_nodes"], "bvh_depth": info["bvh_depth"]},
         "roofline": roofline,
     }
+    if not args.no_ref:
+        log(f"[bench] relMSE of one {spp}-spp image against a {args.ref_spp}-spp FP64 reference ...")
+        out["quality"] = image_quality(scene, W, H, spp, args.ref_spp, **kw)
+        log(f"[bench] quality {out['quality']}")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(desc, args, max_depth, flags)

@@ -153,7 +153,8 @@ enum {
     XT_FLAG_UNIFORM_BARY = 1u,   /* area-uniform light points (the reference's are not, TriangleIndexed.h:184-192) */
     XT_FLAG_ENV_NEE = 2u,        /* sample sun + sky as a light, MIS with BSDF escapes */
     XT_FLAG_GUIDE = 4u,          /* standard integrator: sample with the scene's guiding network */
-    XT_FLAG_GUIDE_TRAIN = 8u     /* with XT_FLAG_GUIDE: record training samples, take Adam steps per batch */
+    XT_FLAG_GUIDE_TRAIN = 8u,    /* with XT_FLAG_GUIDE: record training samples, take Adam steps per batch */
+    XT_FLAG_SORT_MATERIALS = 16u /* shade each block's queue entries grouped by material type (same film) */
 };
 
 typedef struct xt_render_params {

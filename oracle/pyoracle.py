@@ -97,6 +97,7 @@ def _configure(L):
     L.orc_vmf_sample.argtypes = [C.c_int, f32p, f32p, C.c_int, f32p]
     L.orc_vmf_fit.argtypes = [C.c_int, f32p, u32p, C.c_int, f32p, f32p, i32p, C.c_void_p, C.c_int]
     L.orc_film_position.argtypes = [f32p, C.c_int, C.c_int, C.c_int, f32p]
+    L.orc_libm.argtypes = [C.c_int, f32p, C.c_int, f32p]
     return L
 
 
@@ -370,4 +371,17 @@ def film_position(xyuv, width, height, L=None):
     a = np.ascontiguousarray(xyuv, np.float32).reshape(-1, 4)
     out = np.zeros((a.shape[0], 2), np.float32)
     (L or lib()).orc_film_position(a, a.shape[0], width, height, out)
+    return out
+
+
+LIBM_KINDS = {"sin": 0, "cos": 1, "pow5": 2, "atan2": 3, "acos": 4}
+
+
+def libm(kind, x, L=None):
+    """The oracle's deterministic transcendentals (xo_core.hpp); atan2 takes
+    (n, 2) rows (y, x)."""
+    a = np.ascontiguousarray(x, np.float32)
+    n = a.shape[0]
+    out = np.zeros(n, np.float32)
+    (L or lib()).orc_libm(LIBM_KINDS[kind], a.reshape(-1), n, out)
     return out

@@ -224,6 +224,21 @@ void orc_render(void* s, const float* cam, const int32_t* ip, const float* fp, f
 
 // ---- unit-level entry points for the pin tests --------------------------
 
+// The oracle's transcendentals over n arguments: kind 0 sin, 1 cos, 2 x^5,
+// 3 atan2(x[2i], x[2i+1]) (n pairs), 4 acos.
+void orc_libm(int kind, const float* x, int n, float* out) {
+    for (int i = 0; i < n; ++i) {
+        float s, c;
+        switch (kind) {
+            case 0: sinCos(x[i], &s, &c); out[i] = s; break;
+            case 1: sinCos(x[i], &s, &c); out[i] = c; break;
+            case 2: out[i] = pow5(x[i]); break;
+            case 3: out[i] = atan2F(x[2 * i], x[2 * i + 1]); break;
+            default: out[i] = acosF(x[i]); break;
+        }
+    }
+}
+
 // RenderTargetTile::GenerateFilmPosition for n (x, y, ux, uy) rows -> (fx, fy).
 void orc_film_position(const float* xyuv, int n, int W, int H, float* out) {
     for (int i = 0; i < n; ++i)

@@ -111,6 +111,38 @@ inline V3 clampPositive(const V3& v) { return V3(clampPositive(v.x), clampPositi
 inline float safeSqrt(float x) { return sqrtf(clampPositive(x)); }        // Utils.h:89
 inline float heaviside(float x) { return x > 0 ? 1.0f : 0.0f; }          // Utils.h:45
 
+// Transcendentals.  The reference calls the platform libm (std::sin, std::cos,
+// std::pow, std::atan2, std::acos), whose last bits vary between libraries;
+// here they are evaluated deterministically in IEEE arithmetic, the way the
+// HIP library evaluates them, so both sides sample the same directions.
+// sin/cos (|x| <= 256): Cody-Waite reduction x = k pi/2 + r, pi/2 split into
+// three floats (the first two with 16 significant bits, so k * part is exact
+// for |k| < 2^8), Taylor series of sin to r^9 and cos to r^10 on |r| <= pi/4
+// (truncation < 2.3e-9 relative), quadrant k mod 4.  Pinned against float64
+// sin/cos within 2 ulp (tests/test_libm.py).
+inline void sinCos(float x, float* s, float* c) {
+    const float k = rintf(x * 0x1.45f306p-1f);                        // x * 2/pi
+    float r = x - k * 0x1.921ep+0f;
+    r = r - k * 0x1.b544p-16f;
+    r = r - k * 0x1.0b4612p-34f;
+    const float r2 = r * r;
+    const float ps = ((0x1.71de3ap-19f * r2 + -0x1.a01a02p-13f) * r2 + 0x1.111112p-7f) * r2 + -0x1.555556p-3f;
+    const float sp = r + (r * r2) * ps;
+    const float pc = (((-0x1.27e4fcp-22f * r2 + 0x1.a01a02p-16f) * r2 + -0x1.6c16c2p-10f) * r2 + 0x1.555556p-5f) * r2 +
+                     -0.5f;
+    const float cp = 1.0f + r2 * pc;
+    const float q = k - 4.0f * floorf(k * 0.25f);
+    if (q == 1.0f) { *s = cp; *c = -sp; }
+    else if (q == 2.0f) { *s = -sp; *c = -cp; }
+    else if (q == 3.0f) { *s = -cp; *c = sp; }
+    else { *s = sp; *c = cp; }
+}
+// std::pow(float, int) promotes to double: x^5 by double products, one float rounding.
+inline float pow5(float x) { const double d = x; return (float)(d * d * d * d * d); }
+// atan2 and acos in double, rounded to float.
+inline float atan2F(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline float acosF(float x) { return (float)std::acos((double)x); }
+
 // Vector.h:781-795
 struct Basis {
     V3 e1, e2, e3;
@@ -282,8 +314,10 @@ struct GGX {
         V3 T2 = cross(Vh, T1);
         float r = sqrtf(U1);
         float phi = 2.0f * Pi * U2;
-        float t1 = r * cosf(phi);
-        float t2 = r * sinf(phi);
+        float sphi, cphi;
+        sinCos(phi, &sphi, &cphi);
+        float t1 = r * cphi;
+        float t2 = r * sphi;
         float s = 0.5f * (1.0f + Vh.x);
         t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
         V3 Nh = t1 * T1 + t2 * T2 + safeSqrt(1.0f - t1 * t1 - t2 * t2) * Vh;
@@ -296,7 +330,7 @@ struct GGX {
     }
 };
 inline V3 schlick(const V3& f0, float c) {                                // Material.h:288-292
-    return f0 + (V3(1.0f) - f0) * powf(1 - c, 5.0f);
+    return f0 + (V3(1.0f) - f0) * pow5(1 - c);
 }
 inline bool finite3(const V3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
 
@@ -383,7 +417,9 @@ inline V3 sampleCosHemisphere(const V3& v, float u, float uphi) {
     Basis basis(v);
     float r = sqrtf(u);
     float phi = uphi * (2.0f * Pi);
-    return basis.toGlobal(safeSqrt(1.0f - u), r * cosf(phi), r * sinf(phi));
+    float sphi, cphi;
+    sinCos(phi, &sphi, &cphi);
+    return basis.toGlobal(safeSqrt(1.0f - u), r * cphi, r * sphi);
 }
 
 // Material.h:265-271
@@ -497,7 +533,7 @@ inline int nmapFrame(const Isect& is, V3* wg, V3* wp, V3* wt) {
 inline float nmapAp(const V3& w, const V3& wp, const V3& wg) { return clampPositive(dot(w, wp)) / fabsf(dot(wp, wg)); }
 inline float nmapAt(const V3& w, const V3& wp, const V3& wt, const V3& wg) {
     float d = dot(wp, wg);
-    return clampPositive(dot(w, wt)) * safeSqrt(1 - powf(d, 2)) / fabsf(d);
+    return clampPositive(dot(w, wt)) * safeSqrt(1 - d * d) / fabsf(d);   // pow(d, 2): the exact square, rounded once
 }
 inline float nmapLambdaP(const V3& w, const V3& wp, const V3& wt, const V3& wg) {
     return nmapAp(w, wp, wg) / (nmapAp(w, wp, wg) + nmapAt(w, wp, wt, wg));
@@ -851,15 +887,15 @@ struct Env {
     }
     // lat-long coordinates of a unit direction
     static void latlongUV(const V3& d, float* u, float* v) {
-        *u = (atan2f(d.z, d.x) + Pi) / (2.0f * Pi);
-        *v = acosf(std::min(std::max(d.y, -1.0f), 1.0f)) / Pi;
+        *u = (atan2F(d.z, d.x) + Pi) / (2.0f * Pi);
+        *v = acosF(std::min(std::max(d.y, -1.0f), 1.0f)) / Pi;
     }
     V3 skyRadiance(const V3& wi) const {
         if (mapKind == 1) {                         // SkySphere.h:20-33
             V3 d = normalize(-wi);
             float sq = sqrtf(d.x * d.x + d.y * d.y);
             if (sq == 0.0f) return V3(0.0f);
-            float r = (float)((1.0 / M_PI) * (double)acosf(d.z) / (double)sq);
+            float r = (float)((1.0 / M_PI) * (double)acosF(d.z) / (double)sq);
             float u = (d.x * r + 1.0f) / 2.0f;
             float v = (d.y * r + 1.0f) / 2.0f;
             return bilinear(u, v);
@@ -948,9 +984,11 @@ struct Env {
         float fx = dx > 0.0f ? (u2 - c[x]) / dx : 0.5f;
         float u = (x + fx) / (float)W, v = (y + fy) / (float)H;
         float phi = 2.0f * Pi * u - Pi, theta = Pi * v;
-        float st = sinf(theta);
+        float st, ct, sp, cp;
+        sinCos(theta, &st, &ct);
+        sinCos(phi, &sp, &cp);
         *pdf = st > 0.0f ? pdfPix[(size_t)y * W + x] / st : 0.0f;
-        return V3(st * cosf(phi), cosf(theta), st * sinf(phi));
+        return V3(st * cp, ct, st * sp);
     }
     float mapPdf(const V3& dUnit) const {
         float u, v;
@@ -1256,7 +1294,9 @@ struct EnvNee {
             float sinT = safeSqrt(uo * (2.0f - uo));
             float phi = 2.0f * Pi * u2;
             Basis b(env.sunDir);
-            return b.toGlobal(cosT, sinT * cosf(phi), sinT * sinf(phi));
+            float sphi, cphi;
+            sinCos(phi, &sphi, &cphi);
+            return b.toGlobal(cosT, sinT * cphi, sinT * sphi);
         }
         if (env.mapKind == 2) {
             float pdfMap;

@@ -162,7 +162,7 @@ def test_sky_renders_match_oracle(dev, kind, flags):
     err = rel_mse(fg / 8, fo / 8)
     exact = np.mean(np.all(fg == fo, axis=2))
     print(f"[gpu]   relMSE {err:.2e}, bit-exact pixels {exact:.3f}", flush=True)
-    assert np.isfinite(fg).all() and err <= 1e-4
+    assert np.isfinite(fg).all() and err <= 1e-4 and exact >= 0.999
 
 
 @pytest.mark.gpu

@@ -13,9 +13,13 @@ on the GPU, through the C ABI.
   PathTracer.h:149-155).
 
 Tolerance: relMSE <= 1e-4 between GPU and oracle films at the same seed and
-spp (mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)); both draw the
-same counter-based random numbers, so the films differ only where a
-transcendental's last-ulp rounding (device libm vs glibc) flips a path."""
+spp (mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)) and at least
+99.9% of the pixels bit-identical: both draw the same counter-based random
+numbers, share the operation order and evaluate the transcendentals the same
+way, so the films are equal bit for bit.  Instanced scenes are the exception:
+the oracle intersects an instance in object space (Object.h:44-48), the
+library in world space, so hit points differ in the last bits and only the
+relMSE bound applies."""
 import math
 
 import numpy as np
@@ -52,7 +56,7 @@ def test_cornell_stated_size(dev):
     fg = g.render(256, 256, 16, seed=1, max_depth=8)
     fo = po.OracleScene(desc).render(256, 256, 16, seed=1, max_depth=8)
     err, exact = _parity(fg, fo, 16, "cornell")
-    assert err <= 1e-4 and exact > 0.5
+    assert err <= 1e-4 and exact >= 0.999
     assert g.stats()["camera_rays"] == 256 * 256 * 16
 
 
@@ -74,7 +78,7 @@ def test_batch_rooms_against_oracle(dev):
     for (desc, depth), fg, k in zip(descs, films, BATCH_ROOMS):
         fo = po.OracleScene(desc).render(96, 96, 4, seed=100 + k, max_depth=depth)
         err, exact = _parity(fg, fo, 4, desc.name)
-        assert err <= 1e-4 and exact > 0.5
+        assert err <= 1e-4 and exact >= 0.999
 
 
 @pytest.fixture(scope="module")
@@ -90,7 +94,7 @@ def test_occluded_room_plain_against_oracle(room):
     fg = g.render(160, 90, 8, seed=4, max_depth=16)
     fo = po.OracleScene(desc).render(160, 90, 8, seed=4, max_depth=16)
     err, exact = _parity(fg, fo, 8, "occluded room")
-    assert err <= 1e-4 and exact > 0.5
+    assert err <= 1e-4 and exact >= 0.999
 
 
 def test_occluded_room_guided_mean_matches_plain(room):
@@ -118,6 +122,30 @@ def test_occluded_room_guided_mean_matches_plain(room):
     print(f"[gpu]   mean plain={mp} guided={mg} z={z}", flush=True)
     assert np.all(z < 4.5)
     assert np.allclose(mg, mp, rtol=0.05)
+
+
+def test_material_sorted_shading_gives_the_same_film(room, dev):
+    """XT_FLAG_SORT_MATERIALS only reorders the shade kernel's work within a
+    block (paths own their random numbers and radiance slots), so films are
+    equal bit for bit: the occluded room (Lambert / GGX), the material zoo
+    (every kind) plain and naive, a guided render, and a graph-replayed batch."""
+    desc, g = room
+    srt = scenes.FLAG_SORT_MATERIALS
+    for sc, w, h, spp, kw in [(g, 160, 90, 4, dict(seed=6, max_depth=16)),
+                              (api.Scene(dev, scenes.material_zoo(64, 64)), 64, 64, 8, dict(seed=7, max_depth=8)),
+                              (api.Scene(dev, scenes.material_zoo(48, 48)), 48, 48, 8,
+                               dict(seed=8, max_depth=8, integrator=1))]:
+        print(f"[gpu] material-sorted shading: {sc.desc.name} {kw}", flush=True)
+        flags = kw.pop("flags", 0)
+        a = sc.render(w, h, spp, flags=flags, **kw)
+        b = sc.render(w, h, spp, flags=flags | srt, **kw)
+        assert np.array_equal(a, b)
+        films, _ = api.render_batch([(sc, w, h, spp, None, dict(flags=flags | srt, **kw))])
+        assert np.array_equal(a, films[0])
+    g.guide_init(seed=3)
+    a = g.render(160, 90, 2, seed=9, max_depth=16, flags=api.XT_FLAG_GUIDE)
+    b = g.render(160, 90, 2, seed=9, max_depth=16, flags=api.XT_FLAG_GUIDE | srt)
+    assert np.array_equal(a, b)
 
 
 def test_instanced_room_against_oracle(dev):

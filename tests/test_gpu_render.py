@@ -1,10 +1,12 @@
 """GPU render parity against the oracle at the same seed and spp.
 
 Tolerance: relMSE <= 1e-4 (mean over pixels and channels of
-(x - r)^2 / (r^2 + 0.01)).  Both sides draw the same counter-based random
-numbers and share the operation order, so most pixels match bit for bit; the
-remainder differ by transcendental last-ulp rounding (ocml vs glibc sinf,
-cosf, powf) that can flip a path."""
+(x - r)^2 / (r^2 + 0.01)), and at least 99.9% of the pixels bit-identical.
+Both sides draw the same counter-based random numbers, share the operation
+order and evaluate sin, cos, x^5, atan2 and acos the same deterministic way
+(xt_libm.h / the oracle's restatement), so the films are equal bit for bit;
+the 0.1% allowance covers an atan2 or acos whose double results fall on
+either side of a float rounding boundary."""
 import math
 
 import numpy as np
@@ -33,13 +35,12 @@ def _compare(dev, desc, w, h, spp, **kw):
     err = rel_mse(fg / spp, fo / spp)
     exact = np.mean(np.all(fg == fo, axis=2))
     print(f"[gpu]   relMSE={err:.3e} bit-exact pixels={exact:.4f}", flush=True)
-    assert err <= 1e-4
+    assert err <= 1e-4 and exact >= 0.999
     return fg, fo, exact, g
 
 
 def test_cornell_standard(dev):
     _, _, exact, g = _compare(dev, scenes.cornell(64, 64), 64, 64, 8, seed=1, max_depth=8)
-    assert exact > 0.5   # the rest differ by last-ulp sinf/cosf/powf rounding
     st = g.stats()
     assert st["camera_rays"] == 64 * 64 * 8 and st["shadow_rays"] > 0
 
@@ -105,3 +106,4 @@ def test_many_batches_large_image(dev):
     o = po.OracleScene(desc)
     sub = o.render(2048, 2048, 2, seed=1, max_depth=4)
     assert rel_mse(f / 2, sub / 2) <= 1e-4
+    assert np.mean(np.all(f == sub, axis=2)) >= 0.999

@@ -31,6 +31,7 @@ XT_FLAG_UNIFORM_BARY = 1
 XT_FLAG_ENV_NEE = 2
 XT_FLAG_GUIDE = 4
 XT_FLAG_GUIDE_TRAIN = 8
+XT_FLAG_SORT_MATERIALS = 16
 GUIDE_NUM_PARAMS = 17648
 
 

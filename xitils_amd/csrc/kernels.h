@@ -130,7 +130,7 @@ __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, i
                         uint32_t* clear_a, uint32_t* clear_b);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
-template <bool GUIDE, bool NMAP>
+template <bool GUIDE, bool NMAP, bool SORT>
 __global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_t* count_in, const float4* hits,
                         PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
                         float4* radiance, float4* pending, GuideRender gr);

@@ -187,327 +187,452 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, fl
     }
 }
 
-template <bool GUIDE, bool NMAP>
+// Material-sorted shading (XT_FLAG_SORT_MATERIALS): the block's queue
+// entries [base, base + SHADE_BLOCK) are partitioned by shading key (miss,
+// then 1 + material type of the hit) before they are shaded, stably, so each
+// wave of the block mostly shades one kind of BSDF instead of every kind the
+// queue mixes.  Returns this thread's entry offset within the block; every
+// thread of the block must call it (the callers' loops are block-uniform).
+constexpr int SHADE_KEYS = 1 + XT_MAT_COUNT_ + 1;   // miss, material types, entries past the queue end
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_sort_by_material(const DevScene& sc, const float4* __restrict__ hits,
+                                                           uint32_t base, uint32_t n,
+                                                           uint32_t* sh /* (BLOCK / 64 + 1) * SHADE_KEYS + BLOCK */) {
+    constexpr int NW = BLOCK / 64;
+    const uint32_t i = base + threadIdx.x;
+    uint32_t key = SHADE_KEYS - 1;
+    if (i < n) {
+        const uint32_t slot = __float_as_uint(hits[i].y);
+        key = 0;
+        if (slot != INVALID) {
+            const uint32_t m = __float_as_uint(sc.tris[(size_t)slot * TRI_F4 + 1].w);
+            key = 1 + min(sc.mats[m].type, (uint32_t)XT_MAT_COUNT_);
+        }
+    }
+    const uint32_t wave = threadIdx.x / 64, lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < SHADE_KEYS; ++k) {
+        const uint64_t m = __ballot(key == (uint32_t)k);
+        if (key == (uint32_t)k) rank = (uint32_t)__popcll(m & below);
+        if (lane == 0) sh[wave * SHADE_KEYS + k] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // key-major, then wave order: every wave's share of a key after the
+        // earlier waves' share of it, and after every entry of a smaller key
+        uint32_t total = 0;
+        for (int k = 0; k < SHADE_KEYS; ++k)
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t c = sh[w * SHADE_KEYS + k];
+                sh[w * SHADE_KEYS + k] = total;
+                total += c;
+            }
+    }
+    __syncthreads();
+    uint32_t* order = sh + (NW + 1) * SHADE_KEYS;
+    order[sh[wave * SHADE_KEYS + key] + rank] = threadIdx.x;
+    __syncthreads();
+    const uint32_t mine = order[threadIdx.x];
+    __syncthreads();   // sh is reused by the next iteration
+    return mine;
+}
+
+// The result of shading one queue entry: the extension ray and the light
+// sample it pushes, if any.
+struct ShadeOut {
+    bool push_ext = false, push_sh = false, guided = false;
+    f3 ext_o{0.0f, 0.0f, 0.0f}, ext_d{0.0f, 0.0f, 0.0f}, ext_w{0.0f, 0.0f, 0.0f}, ext_e{0.0f, 0.0f, 0.0f};
+    float ext_pdf = 0.0f, ext_pe = 0.0f;
+    uint32_t ext_v = 0;
+    Surf s;
+    f3 sh_o{0.0f, 0.0f, 0.0f}, sh_d{0.0f, 0.0f, 0.0f}, sh_c{0.0f, 0.0f, 0.0f};
+    float sh_tmax = 0.0f;
+    uint32_t id = 0;
+};
+
+// Path id -> (pixel, sample) of a single render's batch (as k_generate).
+struct CursorMap {
+    const WaveParams& wp;
+    __device__ __forceinline__ uint2 operator()(uint32_t id) const {
+        const BatchCursor bc = batch_of(wp);
+        const uint32_t s_local = id / bc.num_pixels;
+        return make_uint2(bc.first_pixel + (id - s_local * bc.num_pixels), bc.sample0 + s_local);
+    }
+};
+
+// One StandardPathTracer / NaivePathTracer loop step (PathTracer.h:147-246 /
+// 65-101) for queue entry i: emission at the new vertex, roulette, BSDF
+// sample -> extension ray, light sample -> shadow ray.
+// Per-render constants of the shade step.
+struct ShadeConst {
+    bool standard, can_sample, uniform_bary;
+    EnvNee envn;
+    float p_area;
+    uint2 key;
+};
+__device__ __forceinline__ ShadeConst shade_const(const WaveParams& wp, const DevScene& sc) {
+    ShadeConst c;
+    c.standard = wp.integrator == XT_INTEGRATOR_STANDARD;
+    c.can_sample = sc.num_lights > 0;
+    c.envn = env_nee_setup(sc.env, c.standard && (wp.flags & XT_FLAG_ENV_NEE) != 0, c.can_sample);
+    c.p_area = c.can_sample ? 1.0f - c.envn.p_select : 0.0f;
+    c.uniform_bary = (wp.flags & XT_FLAG_UNIFORM_BARY) != 0;
+    c.key = make_uint2(wp.seed, wp.stream);
+    return c;
+}
+
+template <bool GUIDE, bool NMAP, class PathMap>
+__device__ __forceinline__ void shade_entry(const WaveParams& wp, const DevScene& sc, const ShadeConst& sc0,
+                                            const PathQueue& qin, const float4* __restrict__ hits,
+                                            float4* __restrict__ radiance, float4* __restrict__ pending,
+                                            const GuideRender& gr, uint32_t i, const PathMap& pmap, ShadeOut& out) {
+    const bool standard = sc0.standard;
+    const bool can_sample = sc0.can_sample;
+    const EnvNee envn = sc0.envn;
+    const float p_area = sc0.p_area;
+    const bool uniform_bary = sc0.uniform_bary;
+    const uint2 key = sc0.key;
+    bool& push_ext = out.push_ext;
+    bool& push_sh = out.push_sh;
+    bool& guided = out.guided;
+    f3& ext_o = out.ext_o;
+    f3& ext_d = out.ext_d;
+    f3& ext_w = out.ext_w;
+    f3& ext_e = out.ext_e;
+    float& ext_pdf = out.ext_pdf;
+    float& ext_pe = out.ext_pe;
+    uint32_t& ext_v = out.ext_v;
+    Surf& s = out.s;
+    f3& sh_o = out.sh_o;
+    f3& sh_d = out.sh_d;
+    f3& sh_c = out.sh_c;
+    float& sh_tmax = out.sh_tmax;
+    uint32_t& id = out.id;
+        const float4 o4 = qin.o[i], d4 = qin.d[i], w4 = qin.w[i], e4 = qin.e[i], h = hits[i];
+        id = __float_as_uint(d4.w);
+        const uint32_t vf = __float_as_uint(w4.w);
+        const uint32_t v = vf & 0xFFFFFFu, qflags = vf >> 24;
+        const float pdf_prev = o4.w, pe_prev = e4.w;
+        const f3 d = xyz(d4), meval = xyz(e4);
+        f3 weight = xyz(w4);
+        f3 rad = xyz(radiance[id]);
+        const f3 pend = (qflags & QF_PENDING) ? xyz(pending[id]) : splat(0.0f);
+        if (GUIDE && gr.recs) {
+            // the vertex that sampled this ray: radiance up to and including its light sample
+            const uint32_t rin = qin.rec[i];
+            if (rin != INVALID) gr.recs[rin].f = f4(rad + pend, 0.0f);
+        }
+        const uint32_t slot = __float_as_uint(h.y);
+        const bool hit = slot != INVALID;
+        xt_material mat;
+        bool emissive = false;
+        if (hit) {
+            s = make_surf<NMAP>(sc, slot, d, h.z, h.w);
+            mat = sc.mats[s.mat];
+            emissive = mat.type == XT_MAT_EMISSION;
+        }
+        // ---- radiance arriving along the ray that reached this vertex
+        if (v == 1 && standard) {
+            if (!hit) {
+                if (sc.env.has_sky) rad += weight * env_radiance(sc.env, d);          // PathTracer.h:133-136
+            } else if (emissive) {
+                rad += weight * emitted(mat, -d, s.n);                                // PathTracer.h:143-145
+            }
+        } else if (standard) {
+            if (!hit) {
+                if (sc.env.has_sky) {                                                  // PathTracer.h:192-195
+                    f3 Le = env_radiance(sc.env, d);
+                    if (envn.on && pdf_prev >= 0.0f) {
+                        // power heuristic; a direction the sky sampler cannot produce
+                        // (pe = 0) or an unbounded BSDF pdf keeps the full weight
+                        const float mis = (pe_prev > 0.0f && pdf_prev < INFINITY)
+                                              ? (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe_prev * pe_prev)
+                                              : 1.0f;
+                        rad += weight * mis * meval * Le;
+                    } else {
+                        rad += weight * meval * Le;
+                    }
+                }
+            } else if (emissive) {                                                     // PathTracer.h:171-189
+                float mis;
+                if (pdf_prev >= 0.0f && can_sample) {
+                    float pl = sc.light_pdf[slot] * p_area;
+                    float cosL = fabsf(dot(d, s.sn));
+                    float distSq = h.x * h.x;
+                    pl *= distSq / cosL;
+                    mis = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pl * pl);
+                } else {
+                    mis = 1.0f;
+                }
+                if (mis > 0.0f) rad += weight * mis * meval * emitted(mat, -d, s.n);
+            }
+        } else {                                                                       // PathTracer.h:75-98
+            if (!hit) {
+                if (sc.env.has_sky) rad += clamp_pos(weight * env_radiance(sc.env, d));
+            } else if (emissive) {
+                rad += weight * emitted(mat, -d, s.n);
+            }
+        }
+        if (qflags & QF_PENDING) rad += pend;  // light sample of the previous vertex
+        radiance[id] = f4(rad, 0.0f);
+
+        bool live = hit && !emissive;
+        Rng rng{key, 0u, 0u};
+        if (live) {
+            const uint2 ps = pmap(id);   // path id -> (pixel, sample)
+            rng.pixel = ps.x;
+            rng.sample = ps.y;
+        }
+        if (live && standard) {
+            if (v > 1) {                                       // PathTracer.h:239-245
+                weight = weight * meval;
+                if (is_zero(weight)) live = false;
+            }
+            if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
+            uint4 blk0 = make_uint4(0, 0, 0, 0);
+            if (live) blk0 = rng.block(v, 0);
+            if (live && (int)(v + 1) > wp.rr_start) {          // PathTracer.h:149-155
+                if (Rng::to_unit(blk0.x) >= wp.rr_prob) live = false;
+                else weight = div(weight, wp.rr_prob);
+            }
+            if (live) {
+                const Frame fs = make_frame(s.sn);   // BasisVectors(shading.n), shared by both samplers
+                f3 wi = splat(0.0f);
+                float pdfB = 0.0f;
+                f3 me;
+                // Guided vertex: one-sample MIS of the learned mixture (probability
+                // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
+                // pdf wherever the BSDF pdf is positive, and zero elsewhere.
+                guided = GUIDE && gr.lobes_mk != nullptr && !mat_is_delta(mat.type) &&
+                         (gr.max_vertex == 0 || v <= gr.max_vertex);
+                const WalkKey wk{key, rng.pixel, rng.sample, v};
+                auto l4 = [&](int j) { return unpack_lobe(gr.lobes_mk[(size_t)j * gr.stride + i]); };
+                auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
+                // (a NaN BSDF pdf -- GGX D at a grazing half vector -- stays NaN, so
+                // the light-sample MIS below drops that sample as the plain path does)
+                auto strategy_pdf = [&](f3 dir, float pb) {
+                    if (pb != pb) return pb;
+                    return pb > 0.0f ? gr.prob * guide_pdf_lobes(l4, l2, dir) + (1.0f - gr.prob) * pb : 0.0f;
+                };
+                if (guided) {
+                    const uint4 blk3 = rng.block(v, 3);
+                    // The random walk's sampling density is not its getPDF (an
+                    // approximation, Main.cpp:245), so a walk-sampled direction is
+                    // weighted with its throughput, the unbiased estimate of f / p:
+                    // me = throughput * p^ / (c q + (1 - c) p^), the one-sample MIS
+                    // whose weights (computed with p^) still sum to one.
+                    const bool walk = NMAP && mat.type == XT_MAT_MICROFACET_NMAP;
+                    float pb;
+                    f3 fc;
+                    if (Rng::to_unit(blk3.x) < gr.prob) {
+                        wi = guide_sample_lobes(l4, l2, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z),
+                                                Rng::to_unit(blk3.w));
+                        fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
+                    } else if (walk) {
+                        fc = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pb) * pb;
+                    } else {
+                        mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
+                        fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
+                    }
+                    pdfB = strategy_pdf(wi, pb);
+                    if (!(pdfB > 0.0f)) pdfB = 0.0f;
+                    me = pdfB > 0.0f ? div(fc, pdfB) : splat(0.0f);
+                    if (!finite3(me)) me = splat(0.0f);
+                } else {
+                    me = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
+                }
+                if (!is_zero(me)) {                            // PathTracer.h:165-167
+                    push_ext = true;
+                    ext_d = wi;
+                    ext_o = s.p + wp.eps * wi;
+                    ext_w = weight;
+                    ext_e = me;
+                    ext_pdf = pdfB;
+                    ext_v = v + 1;
+                    ext_pe = (envn.on && pdfB >= 0.0f) ? envn.p_select * env_nee_pdf(sc.env, envn, normalize(wi), s.sn)
+                                                       : 0.0f;
+                }
+                // ---- light sample (PathTracer.h:202-233)
+                uint4 blk1 = rng.block(v, 1);
+                bool sample_env = false;
+                if (envn.on) sample_env = !can_sample || Rng::to_unit(blk1.x) < envn.p_select;
+                if (sample_env) {
+                    if (pdfB >= 0.0f) {
+                        f3 ld = env_nee_sample(sc.env, envn, fs, Rng::to_unit(blk1.w), Rng::to_unit(blk1.y),
+                                               Rng::to_unit(blk1.z));
+                        float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
+                        float pb;
+                        f3 fc = mat_eval<NMAP>(mat, s, ld, &pb, wk, D_WALK_EVAL);
+                        if (guided) pb = strategy_pdf(ld, pb);
+                        const float mis = (pe * pe) / (pb * pb + pe * pe);
+                        if (pe > 0.0f && !is_zero(fc) && mis > 0.0f) {   // NaN mis: dropped
+                            f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
+                            push_sh = true;
+                            sh_d = ld;
+                            sh_o = s.p + wp.eps * ld;
+                            sh_tmax = INFINITY;
+                            sh_c = c;
+                        }
+                    }
+                } else if (can_sample) {
+                    uint2 lo = sc.light_objs[blk1.y % sc.num_lights];
+                    uint32_t lslot = sc.light_slots[lo.x + blk1.z % lo.y];
+                    uint4 blk2 = rng.block(v, 2);
+                    float t0, t1;
+                    if (uniform_bary) {
+                        float su = sqrtf(Rng::to_unit(blk2.x));
+                        t0 = 1.0f - su;
+                        t1 = Rng::to_unit(blk2.y) * su;
+                    } else {                                   // TriangleIndexed.h:184-192
+                        float a = Rng::to_unit(blk2.x), b = Rng::to_unit(blk2.y), c = Rng::to_unit(blk2.z);
+                        float sum = a + b + c;
+                        if (sum == 0.0f) {
+                            a = b = c = 1.0f;
+                            sum = 3.0f;
+                        }
+                        float inv = 1.0f / sum;
+                        t0 = a * inv;
+                        t1 = b * inv;
+                    }
+                    const float4* lt = sc.tris + (size_t)lslot * TRI_F4;
+                    float4 l0 = lt[0], l1 = lt[1], l2 = lt[2];
+                    f3 q0 = xyz(l0), q1 = xyz(l1), q2 = xyz(l2);
+                    f3 lp = lerp3(q0, q1, q2, t0, t1);
+                    f3 ln = normalize(cross(q1 - q0, q2 - q0));
+                    f3 lsn = ln;
+                    if (__float_as_uint(l2.w) & TRI_FLAG_NORMALS) {
+                        const float4* nr = sc.normals + (size_t)lslot * TRI_F4;
+                        lsn = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), t0, t1));
+                    }
+                    ln = normalize(ln);
+                    lsn = normalize(lsn);
+                    float pdfL = sc.light_pdf[lslot] * p_area;
+                    float dist = length(lp - s.p);
+                    f3 sd = div(lp - s.p, dist);
+                    if (dot(sd, ln) < 0) {
+                        float distSq = dist * dist;
+                        float mis;
+                        float pb;
+                        f3 fc = mat_eval<NMAP>(mat, s, sd, &pb, wk, D_WALK_EVAL);
+                        if (guided) pb = strategy_pdf(sd, pb);
+                        if (pdfB >= 0.0f) {
+                            float cosL = fabsf(dot(-sd, lsn));
+                            pb *= cosL / distSq;
+                            mis = (pdfL * pdfL) / (pb * pb + pdfL * pdfL);
+                        } else {
+                            mis = 0.0f;
+                        }
+                        if (mis > 0.0f) {
+                            float G = fabsf(dot(-sd, lsn)) / distSq;
+                            const xt_material& lm = sc.mats[__float_as_uint(l1.w)];
+                            f3 c = div(weight * mis * fc * emitted(lm, -sd, ln) * G, pdfL);
+                            push_sh = true;
+                            sh_d = sd;
+                            sh_o = s.p + wp.eps * sd;
+                            sh_tmax = dist - wp.shadow_margin;
+                            sh_c = c;
+                        }
+                    }
+                }
+            }
+        } else if (live) {  // NaivePathTracer (PathTracer.h:86-100), then the roulette of the next loop head
+            uint4 blk0 = rng.block(v, 0);
+            f3 wi = splat(0.0f);
+            float pdfB = 0.0f;
+            const WalkKey wk{key, rng.pixel, rng.sample, v};
+            weight = weight * mat_sample<NMAP>(mat, s, make_frame(s.sn), blk0, wk, &wi, &pdfB);
+            if (is_zero(weight)) live = false;
+            if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
+            if (live && (int)(v + 1) > wp.rr_start) {
+                uint4 nb = rng.block(v + 1, 0);
+                if (Rng::to_unit(nb.x) >= wp.rr_prob) live = false;
+                else weight = div(weight, wp.rr_prob);
+            }
+            if (live) {
+                push_ext = true;
+                ext_d = wi;
+                ext_o = s.p + wp.eps * wi;
+                ext_w = weight;
+                ext_e = splat(1.0f);
+                ext_pdf = pdfB;
+                ext_v = v + 1;
+            }
+        }
+}
+
+// Compacted appends of a block's shaded entries (every thread of the block
+// calls it): the extension ray with its training record, the shadow ray.
+template <bool GUIDE>
+__device__ __forceinline__ void shade_emit(ShadeOut& o, const PathQueue& qout, uint32_t* __restrict__ count_out,
+                                           const ShadowQueue& sq, uint32_t* __restrict__ count_shadow,
+                                           const GuideRender& gr, uint32_t* append_lds) {
+    // guided light samples: a non-finite contribution is dropped, never summed
+    if (GUIDE && o.push_sh && !finite3(o.sh_c)) o.push_sh = false;
+    const uint2 slots = block_append2<SHADE_BLOCK>(count_out, o.push_ext, count_shadow, o.push_sh, append_lds);
+    const uint32_t ei = slots.x, si = slots.y;
+    if (o.push_ext) {
+        uint32_t fl = o.push_sh ? QF_PENDING : 0u;
+        qout.o[ei] = f4(o.ext_o, o.ext_pdf);
+        qout.d[ei] = f4(o.ext_d, __uint_as_float(o.id));
+        qout.w[ei] = f4(o.ext_w, __uint_as_float(o.ext_v | (fl << 24)));
+        qout.e[ei] = f4(o.ext_e, o.ext_pe);
+        if (GUIDE && gr.recs) {
+            const uint32_t ri = gr.rec_base + ei;
+            const bool keep = ri < gr.rec_cap && ri >= gr.rec_base;
+            qout.rec[ei] = keep ? ri : INVALID;
+            if (keep) {
+                TrainRec& rr = gr.recs[ri];
+                const Surf& s = o.s;
+                rr.a = make_float4(s.p.x, s.p.y, s.p.z, s.sn.x);
+                rr.b = make_float4(s.sn.y, s.sn.z, s.wo.x, s.wo.y);
+                rr.c = make_float4(s.wo.z, o.ext_d.x, o.ext_d.y, o.ext_d.z);
+                rr.d = make_float4(o.ext_pdf, o.ext_w.x, o.ext_w.y, o.ext_w.z);
+                rr.e = f4(o.ext_e, __uint_as_float(o.id | (o.guided ? TREC_GUIDED : 0u)));
+            }
+        }
+    }
+    if (o.push_sh) {
+        sq.o[si] = f4(o.sh_o, o.sh_tmax);
+        sq.d[si] = f4(o.sh_d, __uint_as_float(o.id));
+        sq.c[si] = f4(o.sh_c, o.push_ext ? 1.0f : 0.0f);
+    }
+}
+
+template <bool GUIDE, bool NMAP, bool SORT>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, DevScene sc, PathQueue qin,
                                                const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
                                                PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
                                                uint32_t* __restrict__ count_shadow, float4* __restrict__ radiance,
                                                float4* __restrict__ pending, GuideRender gr) {
     const uint32_t n = *count_in;
-    const bool standard = wp.integrator == XT_INTEGRATOR_STANDARD;
-    const bool can_sample = sc.num_lights > 0;
-    const EnvNee envn = env_nee_setup(sc.env, standard && (wp.flags & XT_FLAG_ENV_NEE) != 0, can_sample);
-    const float p_area = can_sample ? 1.0f - envn.p_select : 0.0f;
-    const bool uniform_bary = (wp.flags & XT_FLAG_UNIFORM_BARY) != 0;
-    const uint2 key = make_uint2(wp.seed, wp.stream);
     const uint32_t stride = gridDim.x * blockDim.x;
     __shared__ uint32_t append_lds[2 * (SHADE_BLOCK / 64) + 2];
+    __shared__ uint32_t sort_lds[SORT ? (SHADE_BLOCK / 64 + 1) * SHADE_KEYS + SHADE_BLOCK : 1];
+    const CursorMap pmap{wp};
+    const ShadeConst c0 = shade_const(wp, sc);
     // whole-block iterations so the appends below always see every thread
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        bool push_ext = false, push_sh = false;
-        f3 ext_o = splat(0.0f), ext_d = splat(0.0f), ext_w = splat(0.0f), ext_e = splat(0.0f);
-        float ext_pdf = 0.0f, ext_pe = 0.0f;
-        uint32_t ext_v = 0;
-        bool guided = false;
-        Surf s;
-        f3 sh_o = splat(0.0f), sh_d = splat(0.0f), sh_c = splat(0.0f);
-        float sh_tmax = 0.0f;
-        uint32_t id = 0;
-        if (i < n) {
-            const float4 o4 = qin.o[i], d4 = qin.d[i], w4 = qin.w[i], e4 = qin.e[i], h = hits[i];
-            id = __float_as_uint(d4.w);
-            const uint32_t vf = __float_as_uint(w4.w);
-            const uint32_t v = vf & 0xFFFFFFu, qflags = vf >> 24;
-            const float pdf_prev = o4.w, pe_prev = e4.w;
-            const f3 d = xyz(d4), meval = xyz(e4);
-            f3 weight = xyz(w4);
-            f3 rad = xyz(radiance[id]);
-            const f3 pend = (qflags & QF_PENDING) ? xyz(pending[id]) : splat(0.0f);
-            if (GUIDE && gr.recs) {
-                // the vertex that sampled this ray: radiance up to and including its light sample
-                const uint32_t rin = qin.rec[i];
-                if (rin != INVALID) gr.recs[rin].f = f4(rad + pend, 0.0f);
-            }
-            const uint32_t slot = __float_as_uint(h.y);
-            const bool hit = slot != INVALID;
-            xt_material mat;
-            bool emissive = false;
-            if (hit) {
-                s = make_surf<NMAP>(sc, slot, d, h.z, h.w);
-                mat = sc.mats[s.mat];
-                emissive = mat.type == XT_MAT_EMISSION;
-            }
-            // ---- radiance arriving along the ray that reached this vertex
-            if (v == 1 && standard) {
-                if (!hit) {
-                    if (sc.env.has_sky) rad += weight * env_radiance(sc.env, d);          // PathTracer.h:133-136
-                } else if (emissive) {
-                    rad += weight * emitted(mat, -d, s.n);                                // PathTracer.h:143-145
-                }
-            } else if (standard) {
-                if (!hit) {
-                    if (sc.env.has_sky) {                                                  // PathTracer.h:192-195
-                        f3 Le = env_radiance(sc.env, d);
-                        if (envn.on && pdf_prev >= 0.0f) {
-                            // power heuristic; a direction the sky sampler cannot produce
-                            // (pe = 0) or an unbounded BSDF pdf keeps the full weight
-                            const float mis = (pe_prev > 0.0f && pdf_prev < INFINITY)
-                                                  ? (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe_prev * pe_prev)
-                                                  : 1.0f;
-                            rad += weight * mis * meval * Le;
-                        } else {
-                            rad += weight * meval * Le;
-                        }
-                    }
-                } else if (emissive) {                                                     // PathTracer.h:171-189
-                    float mis;
-                    if (pdf_prev >= 0.0f && can_sample) {
-                        float pl = sc.light_pdf[slot] * p_area;
-                        float cosL = fabsf(dot(d, s.sn));
-                        float distSq = h.x * h.x;
-                        pl *= distSq / cosL;
-                        mis = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pl * pl);
-                    } else {
-                        mis = 1.0f;
-                    }
-                    if (mis > 0.0f) rad += weight * mis * meval * emitted(mat, -d, s.n);
-                }
-            } else {                                                                       // PathTracer.h:75-98
-                if (!hit) {
-                    if (sc.env.has_sky) rad += clamp_pos(weight * env_radiance(sc.env, d));
-                } else if (emissive) {
-                    rad += weight * emitted(mat, -d, s.n);
-                }
-            }
-            if (qflags & QF_PENDING) rad += pend;  // light sample of the previous vertex
-            radiance[id] = f4(rad, 0.0f);
-
-            bool live = hit && !emissive;
-            Rng rng{key, 0u, 0u};
-            if (live) {
-                // path id -> (pixel, sample) as in k_generate
-                const BatchCursor bc = batch_of(wp);
-                uint32_t s_local = id / bc.num_pixels;
-                rng.pixel = bc.first_pixel + (id - s_local * bc.num_pixels);
-                rng.sample = bc.sample0 + s_local;
-            }
-            if (live && standard) {
-                if (v > 1) {                                       // PathTracer.h:239-245
-                    weight = weight * meval;
-                    if (is_zero(weight)) live = false;
-                }
-                if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
-                uint4 blk0 = make_uint4(0, 0, 0, 0);
-                if (live) blk0 = rng.block(v, 0);
-                if (live && (int)(v + 1) > wp.rr_start) {          // PathTracer.h:149-155
-                    if (Rng::to_unit(blk0.x) >= wp.rr_prob) live = false;
-                    else weight = div(weight, wp.rr_prob);
-                }
-                if (live) {
-                    const Frame fs = make_frame(s.sn);   // BasisVectors(shading.n), shared by both samplers
-                    f3 wi = splat(0.0f);
-                    float pdfB = 0.0f;
-                    f3 me;
-                    // Guided vertex: one-sample MIS of the learned mixture (probability
-                    // gr.prob) and the BSDF sampler; the strategy pdf is the mixture
-                    // pdf wherever the BSDF pdf is positive, and zero elsewhere.
-                    guided = GUIDE && gr.lobes_mk != nullptr && !mat_is_delta(mat.type) &&
-                             (gr.max_vertex == 0 || v <= gr.max_vertex);
-                    const WalkKey wk{key, rng.pixel, rng.sample, v};
-                    auto l4 = [&](int j) { return unpack_lobe(gr.lobes_mk[(size_t)j * gr.stride + i]); };
-                    auto l2 = [&](int j) { return gr.lobes2[(size_t)j * gr.stride + i]; };
-                    // (a NaN BSDF pdf -- GGX D at a grazing half vector -- stays NaN, so
-                    // the light-sample MIS below drops that sample as the plain path does)
-                    auto strategy_pdf = [&](f3 dir, float pb) {
-                        if (pb != pb) return pb;
-                        return pb > 0.0f ? gr.prob * guide_pdf_lobes(l4, l2, dir) + (1.0f - gr.prob) * pb : 0.0f;
-                    };
-                    if (guided) {
-                        const uint4 blk3 = rng.block(v, 3);
-                        // The random walk's sampling density is not its getPDF (an
-                        // approximation, Main.cpp:245), so a walk-sampled direction is
-                        // weighted with its throughput, the unbiased estimate of f / p:
-                        // me = throughput * p^ / (c q + (1 - c) p^), the one-sample MIS
-                        // whose weights (computed with p^) still sum to one.
-                        const bool walk = NMAP && mat.type == XT_MAT_MICROFACET_NMAP;
-                        float pb;
-                        f3 fc;
-                        if (Rng::to_unit(blk3.x) < gr.prob) {
-                            wi = guide_sample_lobes(l4, l2, Rng::to_unit(blk3.y), Rng::to_unit(blk3.z),
-                                                    Rng::to_unit(blk3.w));
-                            fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
-                        } else if (walk) {
-                            fc = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pb) * pb;
-                        } else {
-                            mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
-                            fc = mat_eval<NMAP>(mat, s, wi, &pb, wk, D_WALK_EVAL2);
-                        }
-                        pdfB = strategy_pdf(wi, pb);
-                        if (!(pdfB > 0.0f)) pdfB = 0.0f;
-                        me = pdfB > 0.0f ? div(fc, pdfB) : splat(0.0f);
-                        if (!finite3(me)) me = splat(0.0f);
-                    } else {
-                        me = mat_sample<NMAP>(mat, s, fs, blk0, wk, &wi, &pdfB);
-                    }
-                    if (!is_zero(me)) {                            // PathTracer.h:165-167
-                        push_ext = true;
-                        ext_d = wi;
-                        ext_o = s.p + wp.eps * wi;
-                        ext_w = weight;
-                        ext_e = me;
-                        ext_pdf = pdfB;
-                        ext_v = v + 1;
-                        ext_pe = (envn.on && pdfB >= 0.0f) ? envn.p_select * env_nee_pdf(sc.env, envn, normalize(wi), s.sn)
-                                                           : 0.0f;
-                    }
-                    // ---- light sample (PathTracer.h:202-233)
-                    uint4 blk1 = rng.block(v, 1);
-                    bool sample_env = false;
-                    if (envn.on) sample_env = !can_sample || Rng::to_unit(blk1.x) < envn.p_select;
-                    if (sample_env) {
-                        if (pdfB >= 0.0f) {
-                            f3 ld = env_nee_sample(sc.env, envn, fs, Rng::to_unit(blk1.w), Rng::to_unit(blk1.y),
-                                                   Rng::to_unit(blk1.z));
-                            float pe = envn.p_select * env_nee_pdf(sc.env, envn, ld, s.sn);
-                            float pb;
-                            f3 fc = mat_eval<NMAP>(mat, s, ld, &pb, wk, D_WALK_EVAL);
-                            if (guided) pb = strategy_pdf(ld, pb);
-                            const float mis = (pe * pe) / (pb * pb + pe * pe);
-                            if (pe > 0.0f && !is_zero(fc) && mis > 0.0f) {   // NaN mis: dropped
-                                f3 c = div(weight * mis * fc * env_radiance(sc.env, ld), pe);
-                                push_sh = true;
-                                sh_d = ld;
-                                sh_o = s.p + wp.eps * ld;
-                                sh_tmax = INFINITY;
-                                sh_c = c;
-                            }
-                        }
-                    } else if (can_sample) {
-                        uint2 lo = sc.light_objs[blk1.y % sc.num_lights];
-                        uint32_t lslot = sc.light_slots[lo.x + blk1.z % lo.y];
-                        uint4 blk2 = rng.block(v, 2);
-                        float t0, t1;
-                        if (uniform_bary) {
-                            float su = sqrtf(Rng::to_unit(blk2.x));
-                            t0 = 1.0f - su;
-                            t1 = Rng::to_unit(blk2.y) * su;
-                        } else {                                   // TriangleIndexed.h:184-192
-                            float a = Rng::to_unit(blk2.x), b = Rng::to_unit(blk2.y), c = Rng::to_unit(blk2.z);
-                            float sum = a + b + c;
-                            if (sum == 0.0f) {
-                                a = b = c = 1.0f;
-                                sum = 3.0f;
-                            }
-                            float inv = 1.0f / sum;
-                            t0 = a * inv;
-                            t1 = b * inv;
-                        }
-                        const float4* lt = sc.tris + (size_t)lslot * TRI_F4;
-                        float4 l0 = lt[0], l1 = lt[1], l2 = lt[2];
-                        f3 q0 = xyz(l0), q1 = xyz(l1), q2 = xyz(l2);
-                        f3 lp = lerp3(q0, q1, q2, t0, t1);
-                        f3 ln = normalize(cross(q1 - q0, q2 - q0));
-                        f3 lsn = ln;
-                        if (__float_as_uint(l2.w) & TRI_FLAG_NORMALS) {
-                            const float4* nr = sc.normals + (size_t)lslot * TRI_F4;
-                            lsn = normalize(lerp3(xyz(nr[0]), xyz(nr[1]), xyz(nr[2]), t0, t1));
-                        }
-                        ln = normalize(ln);
-                        lsn = normalize(lsn);
-                        float pdfL = sc.light_pdf[lslot] * p_area;
-                        float dist = length(lp - s.p);
-                        f3 sd = div(lp - s.p, dist);
-                        if (dot(sd, ln) < 0) {
-                            float distSq = dist * dist;
-                            float mis;
-                            float pb;
-                            f3 fc = mat_eval<NMAP>(mat, s, sd, &pb, wk, D_WALK_EVAL);
-                            if (guided) pb = strategy_pdf(sd, pb);
-                            if (pdfB >= 0.0f) {
-                                float cosL = fabsf(dot(-sd, lsn));
-                                pb *= cosL / distSq;
-                                mis = (pdfL * pdfL) / (pb * pb + pdfL * pdfL);
-                            } else {
-                                mis = 0.0f;
-                            }
-                            if (mis > 0.0f) {
-                                float G = fabsf(dot(-sd, lsn)) / distSq;
-                                const xt_material& lm = sc.mats[__float_as_uint(l1.w)];
-                                f3 c = div(weight * mis * fc * emitted(lm, -sd, ln) * G, pdfL);
-                                push_sh = true;
-                                sh_d = sd;
-                                sh_o = s.p + wp.eps * sd;
-                                sh_tmax = dist - wp.shadow_margin;
-                                sh_c = c;
-                            }
-                        }
-                    }
-                }
-            } else if (live) {  // NaivePathTracer (PathTracer.h:86-100), then the roulette of the next loop head
-                uint4 blk0 = rng.block(v, 0);
-                f3 wi = splat(0.0f);
-                float pdfB = 0.0f;
-                const WalkKey wk{key, rng.pixel, rng.sample, v};
-                weight = weight * mat_sample<NMAP>(mat, s, make_frame(s.sn), blk0, wk, &wi, &pdfB);
-                if (is_zero(weight)) live = false;
-                if (live && wp.max_depth > 0 && (int)v > wp.max_depth) live = false;
-                if (live && (int)(v + 1) > wp.rr_start) {
-                    uint4 nb = rng.block(v + 1, 0);
-                    if (Rng::to_unit(nb.x) >= wp.rr_prob) live = false;
-                    else weight = div(weight, wp.rr_prob);
-                }
-                if (live) {
-                    push_ext = true;
-                    ext_d = wi;
-                    ext_o = s.p + wp.eps * wi;
-                    ext_w = weight;
-                    ext_e = splat(1.0f);
-                    ext_pdf = pdfB;
-                    ext_v = v + 1;
-                }
-            }
-        }
-        // guided light samples: a non-finite contribution is dropped, never summed
-        if (GUIDE && push_sh && !finite3(sh_c)) push_sh = false;
-        // ---- compacted appends (whole block participates)
-        const uint2 slots = block_append2<SHADE_BLOCK>(count_out, push_ext, count_shadow, push_sh, append_lds);
-        const uint32_t ei = slots.x, si = slots.y;
-        if (push_ext) {
-            uint32_t fl = push_sh ? QF_PENDING : 0u;
-            qout.o[ei] = f4(ext_o, ext_pdf);
-            qout.d[ei] = f4(ext_d, __uint_as_float(id));
-            qout.w[ei] = f4(ext_w, __uint_as_float(ext_v | (fl << 24)));
-            qout.e[ei] = f4(ext_e, ext_pe);
-            if (GUIDE && gr.recs) {
-                const uint32_t ri = gr.rec_base + ei;
-                const bool keep = ri < gr.rec_cap && ri >= gr.rec_base;
-                qout.rec[ei] = keep ? ri : INVALID;
-                if (keep) {
-                    TrainRec& rr = gr.recs[ri];
-                    rr.a = make_float4(s.p.x, s.p.y, s.p.z, s.sn.x);
-                    rr.b = make_float4(s.sn.y, s.sn.z, s.wo.x, s.wo.y);
-                    rr.c = make_float4(s.wo.z, ext_d.x, ext_d.y, ext_d.z);
-                    rr.d = make_float4(ext_pdf, ext_w.x, ext_w.y, ext_w.z);
-                    rr.e = f4(ext_e, __uint_as_float(id | (guided ? TREC_GUIDED : 0u)));
-                }
-            }
-        }
-        if (push_sh) {
-            sq.o[si] = f4(sh_o, sh_tmax);
-            sq.d[si] = f4(sh_d, __uint_as_float(id));
-            sq.c[si] = f4(sh_c, push_ext ? 1.0f : 0.0f);
-        }
+        const uint32_t i = base + (SORT ? block_sort_by_material<SHADE_BLOCK>(sc, hits, base, n, sort_lds) : threadIdx.x);
+        ShadeOut o;
+        if (i < n) shade_entry<GUIDE, NMAP>(wp, sc, c0, qin, hits, radiance, pending, gr, i, pmap, o);
+        shade_emit<GUIDE>(o, qout, count_out, sq, count_shadow, gr, append_lds);
     }
 }
 
-#define XT_INST_SHADE(G, N)                                                                                   \
-    template __global__ void k_shade<G, N>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue, \
-                                           uint32_t*, ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
-XT_INST_SHADE(false, false)
-XT_INST_SHADE(true, false)
-XT_INST_SHADE(false, true)
-XT_INST_SHADE(true, true)
+#define XT_INST_SHADE(G, N, S)                                                                                   \
+    template __global__ void k_shade<G, N, S>(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*,        \
+                                              PathQueue, uint32_t*, ShadowQueue, uint32_t*, float4*, float4*,         \
+                                              GuideRender);
+XT_INST_SHADE(false, false, false)
+XT_INST_SHADE(true, false, false)
+XT_INST_SHADE(false, true, false)
+XT_INST_SHADE(true, true, false)
+XT_INST_SHADE(false, false, true)
+XT_INST_SHADE(true, false, true)
+XT_INST_SHADE(false, true, true)
+XT_INST_SHADE(true, true, true)
 #undef XT_INST_SHADE
 
 // Any-hit for queued shadow rays; unoccluded contributions go straight to the

@@ -156,9 +156,17 @@ struct xt_scene {
 
 using ShadeKernel = void (*)(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue, uint32_t*,
                             ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
-static ShadeKernel shade_kernel(bool guide, bool nmap) {
-    if (nmap) return guide ? k_shade<true, true> : k_shade<false, true>;
-    return guide ? k_shade<true, false> : k_shade<false, false>;
+// The shade kernel for a render: guided or not, with the normal-mapping
+// materials or not, and with the block-local material partition
+// (XT_FLAG_SORT_MATERIALS) or not -- separate instantiations, so the plain
+// kernel carries none of the others' registers.
+static ShadeKernel shade_kernel(bool guide, bool nmap, uint32_t flags) {
+    if (flags & XT_FLAG_SORT_MATERIALS) {
+        if (nmap) return guide ? k_shade<true, true, true> : k_shade<false, true, true>;
+        return guide ? k_shade<true, false, true> : k_shade<false, false, true>;
+    }
+    if (nmap) return guide ? k_shade<true, true, false> : k_shade<false, true, false>;
+    return guide ? k_shade<true, false, false> : k_shade<false, false, false>;
 }
 
 extern "C" {
@@ -1092,7 +1100,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                         }
                     }
                     a = tm.mark();
-                    hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(shade_blocks(bound)), dim3(SHADE_BLOCK), 0,
+                    hipLaunchKernelGGL(shade_kernel(guide, s->nmap, p->flags), dim3(shade_blocks(bound)), dim3(SHADE_BLOCK), 0,
                                        s->stream, wp, s->ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
                                        rb.counts + (1 - cur), rb.sq, rb.counts + 2, rb.radiance, rb.pending, gr);
                     XT_LAUNCHED(result);
@@ -1310,7 +1318,7 @@ static void enqueue_cursor_batch(xt_scene* s, const xt_render_params* p, const W
             gr.prob = g.cfg.guide_prob;
             gr.max_vertex = g.cfg.max_guided_vertex;
         }
-        hipLaunchKernelGGL(shade_kernel(guide, s->nmap), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
+        hipLaunchKernelGGL(shade_kernel(guide, s->nmap, p->flags), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
                            b.q[cur], b.counts + cur, b.hits, b.q[1 - cur], b.counts + (1 - cur), b.sq,
                            b.counts + 2, b.radiance, b.pending, gr);
         hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, b.sq, b.counts + 2,

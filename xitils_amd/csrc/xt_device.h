@@ -149,8 +149,7 @@ XT_HD DiskPt disk_pt(float u1, float u2) {
     DiskPt d;
     d.r = sqrtf(u1);
     float phi = u2 * (2.0f * PI);
-    d.c = cosf(phi);
-    d.s = sinf(phi);
+    xt_sincosf(phi, &d.s, &d.c);
     return d;
 }
 
@@ -169,7 +168,7 @@ XT_HD f3 ggx_sample_vndf(float alpha, f3 eye, const Frame& fr, DiskPt dp) {
     f3 Ne = normalize(mk(clamp_pos(Nh.x), alpha * Nh.y, alpha * Nh.z));
     return to_world(fr, Ne.x, Ne.y, Ne.z);
 }
-XT_HD f3 schlick(f3 f0, float c) { return f0 + (splat(1.0f) - f0) * powf(1 - c, 5.0f); }
+XT_HD f3 schlick(f3 f0, float c) { return f0 + (splat(1.0f) - f0) * xt_pow5f(1 - c); }
 
 // cosine-weighted hemisphere in frame fr (Vector.h:817-824)
 XT_HD f3 sample_cos_hemisphere(const Frame& fr, float u, DiskPt dp) {
@@ -342,11 +341,13 @@ XT_HD f3 tex_bilinear(const TexRef& m, float u, float v) {
     return wx0 * wy0 * tex_texel(m, x0, y0) + wx0 * wy1 * tex_texel(m, x0, y0 + 1) + wx1 * wy0 * tex_texel(m, x0 + 1, y0) +
            wx1 * wy1 * tex_texel(m, x0 + 1, y0 + 1);
 }
-// lat-long coordinates of a unit direction (y up, row 0 at the zenith)
-XT_HD void env_latlong_uv(f3 d, float* u, float* v) {
+// lat-long coordinates of a unit direction (y up, row 0 at the zenith).  Out
+// of line: the double-precision atan2 and acos would otherwise add their
+// registers to every kernel that can evaluate a sky pdf.
+static __host__ __device__ __noinline__ void env_latlong_uv(f3 d, float* u, float* v) {
     const float PI = 3.14159265358979323846f;
-    *u = (atan2f(d.z, d.x) + PI) / (2.0f * PI);
-    *v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI;
+    *u = (xt_atan2f(d.z, d.x) + PI) / (2.0f * PI);
+    *v = xt_acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI;
 }
 // SkySphereFromImage::getRadiance (SkySphere.h:20-33) or the lat-long map.
 // Kept out of line: inlined, the lookup's registers lower the occupancy of
@@ -357,7 +358,7 @@ static __host__ __device__ __noinline__ f3 env_sky_map(int kind, int w, int h, c
         const f3 d = normalize(-wi);
         const float sq = sqrtf(d.x * d.x + d.y * d.y);
         if (sq == 0.0f) return splat(0.0f);
-        const float r = (float)((1.0 / 3.14159265358979323846) * (double)acosf(d.z) / (double)sq);
+        const float r = (float)((1.0 / 3.14159265358979323846) * (double)xt_acosf(d.z) / (double)sq);
         return tex_bilinear(m, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
     }
     float u, v;
@@ -398,8 +399,10 @@ XT_HD f3 env_map_sample(const DevEnv& e, float u1, float u2) {
     const float fx = dx > 0.0f ? (u2 - c[x]) / dx : 0.5f;
     const float u = (x + fx) / (float)W, v = (y + fy) / (float)H;
     const float phi = 2.0f * PI * u - PI, theta = PI * v;
-    const float st = sinf(theta);
-    return mk(st * cosf(phi), cosf(theta), st * sinf(phi));
+    float st, ct, sp, cp;
+    xt_sincosf(theta, &st, &ct);
+    xt_sincosf(phi, &sp, &cp);
+    return mk(st * cp, ct, st * sp);
 }
 XT_HD float env_map_pdf(const DevEnv& e, f3 du) {
     float u, v;

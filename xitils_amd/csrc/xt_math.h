@@ -14,6 +14,8 @@
 #include <stdint.h>
 
 #define XT_HD __host__ __device__ __forceinline__
+#define XT_LIBM_FN __host__ __device__ __forceinline__
+#include "xt_libm.h"
 
 namespace xt {
 

@@ -61,7 +61,7 @@ XT_HD int nmap_frame(const Surf& s, f3* wg, f3* wp, f3* wt) {
 XT_HD float nmap_ap(f3 w, f3 wp, f3 wg) { return clamp_pos(dot(w, wp)) / fabsf(dot(wp, wg)); }
 XT_HD float nmap_at(f3 w, f3 wp, f3 wt, f3 wg) {
     const float d = dot(wp, wg);
-    return clamp_pos(dot(w, wt)) * safe_sqrt(1 - powf(d, 2.0f)) / fabsf(d);
+    return clamp_pos(dot(w, wt)) * safe_sqrt(1 - d * d) / fabsf(d);   // pow(d, 2): the exact square, rounded once
 }
 XT_HD float nmap_lambda_p(f3 w, f3 wp, f3 wt, f3 wg) {
     const float ap = nmap_ap(w, wp, wg);

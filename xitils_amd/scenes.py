@@ -36,6 +36,7 @@ import numpy as np
 
 FLAG_UNIFORM_BARY = 1
 FLAG_ENV_NEE = 2
+FLAG_SORT_MATERIALS = 16
 
 INTEGRATOR_STANDARD = 0
 INTEGRATOR_NAIVE = 1
