@@ -260,6 +260,21 @@ typedef struct xt_batch_item {
  * (1 <= max_depth <= 1024) and no XT_FLAG_GUIDE_TRAIN.  Graphs are cached
  * per scene, camera, parameters and film. */
 xt_status xt_render_batch(xt_batch_item* items, uint32_t n);
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): xt_render into a device film (film_device) with every buffer of
+ * the bounce loop -- path and shadow queues, hits, radiance, queue counters,
+ * batch cursor, traversal spill stack -- carved from one static device array
+ * of XT_STATIC_SCRATCH_BYTES per GPU, which the GPU holds until the process
+ * ends.  Calls no hipMalloc, hipFree or allocating library routine.  Renders
+ * in batches of as many paths as the array holds (about 220 k), so any image
+ * size works; unguided only, with a bounce cap (1 <= max_depth <= 1024).
+ * Gives xt_render's film bit for bit.  XT_ERR_TOO_LARGE if the scene's
+ * traversal stack does not fit the array's spill part (16 MiB).  Every call
+ * on one GPU uses that GPU's one array: calls that overlap on one GPU race. */
+#define XT_STATIC_SCRATCH_BYTES (64u << 20)
+xt_status xt_render_static(xt_scene* scene, const xt_camera* camera, const xt_render_params* params, void* film_device,
+                           int film_is_f64);
 /* Autotuner: renders one batch of (camera, params) as a graph under 9
  * launch shapes and keeps the fastest for this scene's later renders;
  * out = {traversal blocks per CU, shade blocks per CU}. */

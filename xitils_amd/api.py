@@ -25,7 +25,7 @@ EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", 
             "xt_scene_set_counting", "xt_guide_config_default", "xt_guide_init", "xt_guide_get_params",
             "xt_guide_set_params", "xt_guide_get_bounds", "xt_guide_forward", "xt_guide_gradient",
             "xt_guide_train_step", "xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit", "xt_render_batch",
-            "xt_scene_autotune"]
+            "xt_scene_autotune", "xt_render_static"]
 
 XT_FLAG_UNIFORM_BARY = 1
 XT_FLAG_ENV_NEE = 2
@@ -143,8 +143,11 @@ def lib():
         L.xt_vmf_fit.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_int]
         L.xt_render_batch.argtypes = [C.c_void_p, C.c_uint32]
+        L.xt_render_static.argtypes = [C.c_void_p, C.POINTER(xt_camera), C.POINTER(xt_render_params), C.c_void_p,
+                                       C.c_int]
         L.xt_scene_autotune.argtypes = [C.c_void_p, C.POINTER(xt_camera), C.POINTER(xt_render_params), C.c_void_p]
-        for f in ("xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit", "xt_render_batch", "xt_scene_autotune"):
+        for f in ("xt_vmf_eval", "xt_vmf_sample", "xt_vmf_fit", "xt_render_batch", "xt_scene_autotune",
+                  "xt_render_static"):
             getattr(L, f).restype = C.c_int
         for f in ("xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy", "xt_scene_info",
                   "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats", "xt_scene_set_counting",
@@ -396,6 +399,13 @@ class Scene:
             film = np.zeros((height, width, 3), np.float64 if f64 else np.float32)
         check(lib().xt_render(self.h, C.byref(cam), C.byref(p), film.ctypes.data, int(film.dtype == np.float64), 0))
         return film
+
+    def render_static(self, film_ptr, width, height, spp, f64=False, camera=None, **kw):
+        """xt_render_static: renders into a device film with no device
+        allocation (the library's static scratch; needs max_depth >= 1)."""
+        cam = camera_struct(camera or self.desc.camera)
+        p = self.params(width, height, spp, **kw)
+        check(lib().xt_render_static(self.h, C.byref(cam), C.byref(p), C.c_void_p(film_ptr), int(f64)))
 
     def render_device(self, film_ptr, width, height, spp, f64=False, camera=None, **kw):
         """Renders into a device film (e.g. a torch tensor's data_ptr())."""

@@ -12,6 +12,7 @@ struct xt_device {
     int ordinal = 0;
     int num_cu = 0;
     std::string arch;
+    unsigned char* scratch = nullptr;   // this device's address of the static scratch array (xt_render_static)
 };
 
 // Records msg as this thread's last error (xt_last_error) and returns s.

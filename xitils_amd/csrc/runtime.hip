@@ -176,6 +176,14 @@ extern "C" {
 const char* xt_version(void) { return "xitils_amd 0.1.0 (gfx950, BVH" XT_STR(XT_BVH_WIDTH) " wavefront)"; }
 const char* xt_last_error(void) { return g_last_error.c_str(); }
 
+// Static scratch of xt_render_static: every buffer of its bounce loop is
+// carved from this array, so that entry point allocates nothing.  Each GPU
+// the library runs on holds its own copy from code-object load until the
+// process ends.  In host code `scratch` names a host-side stand-in, not
+// device memory: the device address comes from hipGetSymbolAddress, once per
+// device at xt_device_open, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[XT_STATIC_SCRATCH_BYTES];
+
 xt_status xt_device_open(int ordinal, xt_device** out) {
     if (!out) return fail(XT_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
@@ -187,8 +195,12 @@ xt_status xt_device_open(int ordinal, xt_device** out) {
     std::string arch = prop.gcnArchName;
     if (arch.rfind("gfx950", 0) != 0) return fail(XT_ERR_NO_DEVICE, "device is " + arch + ", this build targets gfx950");
     XT_HIP(hipSetDevice(ordinal));
+    void* scratch_ptr = nullptr;   // this device's copy of the static scratch (the device is current)
+    XT_HIP(hipGetSymbolAddress(&scratch_ptr, HIP_SYMBOL(scratch)));
+    if (!scratch_ptr) return fail(XT_ERR_DEVICE, "hipGetSymbolAddress returned no address for the static scratch");
     xt_device* d = new (std::nothrow) xt_device();
     if (!d) return fail(XT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    d->scratch = (unsigned char*)scratch_ptr;
     d->ordinal = ordinal;
     d->num_cu = prop.multiProcessorCount;
     d->arch = arch;
@@ -1235,7 +1247,8 @@ static uint32_t graph_capacity(uint64_t npix, uint32_t spp) {
     return (uint32_t)std::min<uint64_t>(std::min<uint32_t>(path_capacity(), 1u << 22), npix * spp);
 }
 
-// Path state of one count-driven batch (the graph renders' buffers).
+// Path state of one count-driven batch (the graph renders' buffers, or the
+// static scratch of xt_render_static).
 struct BounceBufs {
     PathQueue q[2];
     ShadowQueue sq;
@@ -1497,6 +1510,80 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
     }
     cleanup();
     return result;
+}
+
+// ---------------------------------------------------------------- render without allocation
+
+// xt_render_static's carving of the static scratch: a 256-byte header (batch
+// cursor at 0, queue counters at 64), at most 16 MiB of traversal spill
+// columns, then 14 float4 arrays of `capacity` paths (224 bytes per path).
+constexpr size_t STATIC_HEADER = 256;
+constexpr size_t STATIC_SPILL_MAX = (size_t)16 << 20;
+constexpr size_t STATIC_PATH_BYTES = 14 * sizeof(float4);
+
+xt_status xt_render_static(xt_scene* s, const xt_camera* cam, const xt_render_params* p, void* film, int film_f64) {
+    xt_status st = check_render_args(s, cam, p);
+    if (st != XT_OK) return st;
+    if (!film) return fail(XT_ERR_INVALID_ARGUMENT, "film is NULL");
+    if (p->flags & XT_FLAG_GUIDE) return fail(XT_ERR_INVALID_ARGUMENT, "xt_render_static renders unguided (no XT_FLAG_GUIDE)");
+    unsigned char* base = s->dev->scratch;
+    if (!base) return fail(XT_ERR_DEVICE, "device has no static scratch address");
+    XT_HIP(hipSetDevice(s->dev->ordinal));
+    if (p->spp == 0) {
+        s->stats = xt_render_stats{};
+        return XT_OK;
+    }
+    // traversal spill columns: ovf_depth entries per lane of every block launched
+    const size_t per_block = (size_t)s->ovf_depth * TRACE_BLOCK * sizeof(uint32_t);
+    uint32_t tcap = s->trace_grid;
+    if (per_block > 0) tcap = (uint32_t)std::min<size_t>(tcap, STATIC_SPILL_MAX / per_block);
+    if (tcap == 0)
+        return fail(XT_ERR_TOO_LARGE, "the scene's traversal stack (" + std::to_string(s->ovf_depth) +
+                                          " spill entries per lane) exceeds the static scratch");
+    const size_t spill_bytes = ((per_block * tcap + 255) / 256) * 256;
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    const uint64_t fit = (XT_STATIC_SCRATCH_BYTES - STATIC_HEADER - spill_bytes) / STATIC_PATH_BYTES;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(fit, npix * p->spp);
+    BatchCursor* cursor = (BatchCursor*)base;
+    BounceBufs b{};
+    b.counts = (uint32_t*)(base + 64);
+    b.ovf = per_block ? (uint32_t*)(base + STATIC_HEADER) : nullptr;
+    b.capacity = cap;
+    float4* arr = (float4*)(base + STATIC_HEADER + spill_bytes);
+    auto take = [&]() {
+        float4* r = arr;
+        arr += cap;
+        return r;
+    };
+    for (int k = 0; k < 2; ++k) {
+        b.q[k].o = take();
+        b.q[k].d = take();
+        b.q[k].w = take();
+        b.q[k].e = take();
+        b.q[k].rec = nullptr;
+    }
+    b.sq.o = take();
+    b.sq.d = take();
+    b.sq.c = take();
+    b.hits = take();
+    b.radiance = take();
+    b.pending = take();
+    uint32_t nbatches = 0;
+    const BatchPlan plan = cursor_plan(p, cap, &nbatches);
+    const WaveParams wp = cursor_wave_params(cam, p, cursor);
+    const uint32_t tblocks = std::min(trace_blocks(s, (size_t)plan.pix_chunk * plan.S), tcap);
+    // the cursor is written by k_cursor_init and the counters by the kernels
+    // (k_count_from_cursor, k_trace), so nothing needs clearing beforehand
+    hipLaunchKernelGGL(k_cursor_init, dim3(1), dim3(1), 0, s->stream, cursor, plan);
+    for (uint32_t k = 0; k < nbatches; ++k)
+        enqueue_cursor_batch(s, p, wp, b, plan, cursor, tblocks, BounceGuide{}, film, film_f64);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(s->stream);
+    if (le != hipSuccess) return fail(XT_ERR_DEVICE, std::string("xt_render_static launch failed: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(XT_ERR_DEVICE, std::string("xt_render_static failed: ") + hipGetErrorString(se));
+    s->stats = xt_render_stats{};
+    s->stats.camera_rays = npix * p->spp;
+    return XT_OK;
 }
 
 // Times the graph-replayed render of one batch for a few launch shapes and
