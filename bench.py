@@ -373,8 +373,12 @@ def batch_main(args):
     """Aggregate Mpaths/s and per-scene latency over the 64-scene batch
     (master seed 3: unit rooms with tessellated spheres and boxes, 1e4..2e6
     triangles, 128^2..1024^2 pixels, 4..256 spp, depth 4..32, one area light
-    each), rendered (a) one xt_render call after another and (b) as one
-    xt_render_batch call (HIP-graph bounce loops, concurrent streams)."""
+    each), rendered (a) one xt_render call after another and as one
+    xt_render_batch call in its three forms: (b) the default -- items of at
+    most one graph replay share one wavefront (the items' paths walked in
+    windows, each window one captured graph replay), larger ones run per item
+    concurrently -- (c) every item in the wavefront and (d) a graph per item
+    on per-scene streams.  value is (b)."""
     import numpy as np
     import torch
     from xitils_amd import api, scenes
@@ -402,30 +406,39 @@ def batch_main(args):
         log(f"[bench] autotuned: {tuned}")
 
     def sequential():
-        lat = []
+        done = []
         torch.cuda.synchronize()
         t = time.perf_counter()
         for sc, w, h, spp, depth, film in jobs:
-            a = time.perf_counter()
-            sc.render_device(film.data_ptr(), w, h, spp, seed=1, max_depth=depth)
-            lat.append((time.perf_counter() - a) * 1e3)
+            sc.render_device(film.data_ptr(), w, h, spp, seed=1, max_depth=depth)   # synchronous
+            done.append((time.perf_counter() - t) * 1e3)
         torch.cuda.synchronize()
-        return time.perf_counter() - t, lat
+        return time.perf_counter() - t, done
 
-    def batched():
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        _, ms = api.render_batch([(sc, w, h, spp, (film.data_ptr(), False), dict(seed=1, max_depth=depth))
-                                  for sc, w, h, spp, depth, film in jobs])
-        torch.cuda.synchronize()
-        return time.perf_counter() - t, ms
+    def batched(mode):
+        if mode != "hybrid":
+            os.environ["XITILS_AMD_BATCH"] = mode
+        try:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _, ms = api.render_batch([(sc, w, h, spp, (film.data_ptr(), False), dict(seed=1, max_depth=depth))
+                                      for sc, w, h, spp, depth, film in jobs])
+            torch.cuda.synchronize()
+            return time.perf_counter() - t, ms
+        finally:
+            os.environ.pop("XITILS_AMD_BATCH", None)
 
     sequential()                       # warm-up (first-touch allocations)
-    seq_s, seq_lat = sequential()
+    seq_s, seq_done = sequential()
     log(f"[bench] sequential: {seq_s:.3f} s")
-    batched()                          # warm-up: captures the graphs
-    bat_s, bat_lat = batched()
-    log(f"[bench] batched: {bat_s:.3f} s")
+    timed = {}
+    for mode in ("items", "unified", "hybrid"):
+        batched(mode)                  # warm-up: graph captures, the wavefront's buffers
+        timed[mode] = batched(mode)
+        log(f"[bench] xt_render_batch ({mode}): {timed[mode][0]:.3f} s")
+    bat_s, bat_lat = timed["hybrid"]
+    item_s, item_ms = timed["items"]
+    uni_s, uni_lat = timed["unified"]
     pct = lambda v, q: float(np.percentile(np.asarray(v), q))  # noqa: E731
     out = {
         "metric": "aggregate Mpaths/s over a 64-scene batch",
@@ -444,9 +457,18 @@ def batch_main(args):
                                (f"_spp{args.batch_spp}" if args.batch_spp else ""), "scenes": len(jobs), "triangles": tris,
                    "paths": paths, "autotune": bool(args.autotune), "parallelism": "1 GPU"},
         "latency_ms": {"p50": round(pct(bat_lat, 50), 3), "p99": round(pct(bat_lat, 99), 3),
-                       "note": "GPU time of each item on its scene's stream (HIP events), items running concurrently"},
+                       "note": "xt_render_batch's item times: a wavefront item's is the end of the window holding its "
+                               "last path, a graph item's its graph on its stream, both from HIP events"},
         "sequential": {"mpaths_per_s": round(paths / seq_s / 1e6, 2), "seconds": round(seq_s, 3),
-                       "latency_ms": {"p50": round(pct(seq_lat, 50), 3), "p99": round(pct(seq_lat, 99), 3)}},
+                       "latency_ms": {"p50": round(pct(seq_done, 50), 3), "p99": round(pct(seq_done, 99), 3)},
+                       "note": "one synchronous xt_render per scene; latency = host completion time from the start"},
+        "per_item_graphs": {"mpaths_per_s": round(paths / item_s / 1e6, 2), "seconds": round(item_s, 3),
+                            "item_gpu_ms": {"p50": round(pct(item_ms, 50), 3), "p99": round(pct(item_ms, 99), 3)},
+                            "note": "XITILS_AMD_BATCH=items: a graph per item on per-scene streams"},
+        "unified_all": {"mpaths_per_s": round(paths / uni_s / 1e6, 2), "seconds": round(uni_s, 3),
+                        "latency_ms": {"p50": round(pct(uni_lat, 50), 3), "p99": round(pct(uni_lat, 99), 3)},
+                        "note": "XITILS_AMD_BATCH=unified: every item in the one wavefront"},
+        "speedup_vs_sequential": round(seq_s / bat_s, 3),
         "build_s": round(build_s, 1),
     }
     if tuned:

@@ -246,19 +246,24 @@ typedef struct xt_batch_item {
     void* film;
     int32_t film_is_f64;
     int32_t film_on_device;
-    float ms;                /* out: GPU time of this item on its scene's stream (HIP events) */
+    float ms;                /* out: GPU time of this item (HIP events; see xt_render_batch) */
 } xt_batch_item;
 
 /* Many RenderTarget::render calls at once (no reference counterpart; the
  * closest is the tile scheduler of RenderTarget.h:84-127).  Each item gives
- * the film xt_render would give, bit for bit.  Items of at most 4 Mi paths
- * (one graph replay) run their bounce loop as a HIP graph with no host round
- * trip (every kernel reads its queue count from device memory), items on
- * different scenes concurrently on their scenes' streams; larger items then
- * render one after another through xt_render's bounce loop while those
- * graphs run.  Needs a bounce cap
- * (1 <= max_depth <= 1024) and no XT_FLAG_GUIDE_TRAIN.  Graphs are cached
- * per scene, camera, parameters and film. */
+ * the film xt_render would give, bit for bit.  Unguided items of at most 4 Mi
+ * paths share one wavefront: their paths are numbered one after another and
+ * walked in windows of the path capacity, each window one replay of a
+ * captured graph that traces and shades every path in its own item's scene.
+ * Other items of at most 4 Mi paths run their bounce loop as a per-scene HIP
+ * graph with no host round trip, concurrently on their scenes' streams;
+ * larger items then render one after another through xt_render's bounce loop
+ * while those run.  Items that share a film accumulate into it in item order
+ * (all of them through the wavefront when none is guided, else each through
+ * xt_render in turn).  Needs a bounce cap (1 <= max_depth <= 1024) and no
+ * XT_FLAG_GUIDE_TRAIN.  Graphs are cached per scene, camera, parameters and
+ * film.  ms: an item's GPU time on its stream; for a wavefront item, from
+ * the start of the wavefront to the end of the window holding its last path. */
 xt_status xt_render_batch(xt_batch_item* items, uint32_t n);
 /* No reference counterpart.  For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

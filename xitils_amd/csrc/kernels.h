@@ -123,11 +123,66 @@ struct GuideRender {
     uint32_t max_vertex;   // guide surface vertices 1..max_vertex only (0: all)
 };
 
-template <bool ANY, int LDS_DEPTH, bool COUNT>
+// ---- unified batches: many renders (items) share one wavefront
+//
+// The items' paths are numbered one after another (item k owns
+// [path_base, path_base + paths), sample-major inside the item as a single
+// render numbers them); the batch walks that path space in windows of up to
+// `window` paths, each window one pass of the count-driven bounce loop.  A
+// queue entry's path id is window-relative; item_of[id] names its item, whose
+// record gives the render's parameters, scene and film.
+
+struct MultiItem {
+    WaveParams wp;          // the render's parameters (the batch fields are unused)
+    const float4* nodes;    // its scene's BVH nodes and leaf-ordered triangles
+    const float4* tris;
+    uint32_t scene;         // index into MultiBatch::scenes
+    uint32_t npix;          // width * height
+    uint32_t spp_offset;
+    uint32_t film_f64;
+    void* film;             // device film (+= radiance sums, as xt_render)
+    uint64_t path_base;     // first path of the item in the batch
+    uint64_t paths;         // npix * spp
+};
+
+// The current window and the items it overlaps (written by k_window_plan).
+struct MultiCursor {
+    uint64_t g0;            // first path of the window
+    uint32_t n;             // paths in the window (0: past the end)
+    uint32_t first_item;    // first item overlapping the window
+    uint32_t num_overlap;   // items overlapping the window
+    uint32_t pad;
+};
+
+struct MultiBatch {
+    const MultiItem* items;
+    const DevScene* scenes;
+    uint32_t num_items;
+    uint32_t window;
+    uint64_t total_paths;
+    MultiCursor* cursor;
+    uint32_t* item_of;      // per path of the window
+    uint32_t* units;        // window plan: accumulation units before each overlapping item (num_overlap + 1)
+};
+
+template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI = false>
 __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, int ray_stride,
                         const uint32_t* count_ptr, uint32_t count_fixed, float4* hits, uint8_t* occluded,
                         int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals,
-                        uint32_t* clear_a, uint32_t* clear_b);
+                        uint32_t* clear_a, uint32_t* clear_b, const uint32_t* item_of = nullptr,
+                        const MultiItem* items = nullptr);
+__global__ void k_shadow_multi(ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending, uint32_t* ovf,
+                               const uint32_t* item_of, const MultiItem* items);
+__global__ void k_window_init(MultiBatch mb);
+__global__ void k_window_plan(MultiBatch mb);
+__global__ void k_window_advance(MultiBatch mb);
+__global__ void k_count_from_window(uint32_t* count, const MultiCursor* c);
+__global__ void k_generate_multi(MultiBatch mb, PathQueue q, float4* radiance);
+template <bool NMAP>
+__global__ void k_shade_multi(MultiBatch mb, PathQueue qin, const uint32_t* count_in, const float4* hits,
+                              PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
+                              float4* radiance, float4* pending);
+__global__ void k_accumulate_multi(MultiBatch mb, const float4* radiance);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
 template <bool GUIDE, bool NMAP, bool SORT>

@@ -77,14 +77,18 @@ __device__ __forceinline__ void flush_counts(unsigned long long* totals, const V
 // Ray queries over a ray array (stride in float4 between rays): closest hit
 // (ANY = false) -> hits[i] = (t, slot or prim id, b0, b1); any hit ->
 // occluded[i].  Persistent waves with in-place lane refill (xt_traverse.h).
-template <bool ANY, int LDS_DEPTH, bool COUNT>
+// MULTI: the rays of a unified batch's queue, each traced in its own item's
+// scene (item_of[path id] -> the item's nodes and triangles, per lane).
+template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
                                                         const float4* __restrict__ ray_d, int ray_stride,
                                                         const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
                                                         float4* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                         int hit_prim_ids, int tmax_from_ray, uint32_t* __restrict__ ovf,
                                                         unsigned long long* __restrict__ visit_totals,
-                                                        uint32_t* __restrict__ clear_a, uint32_t* __restrict__ clear_b) {
+                                                        uint32_t* __restrict__ clear_a, uint32_t* __restrict__ clear_b,
+                                                        const uint32_t* __restrict__ item_of,
+                                                        const MultiItem* __restrict__ items) {
     __shared__ uint32_t lds[LDS_DEPTH * TRACE_BLOCK];
     // the next round's queue counters (their last readers ran before this
     // kernel in stream order), so the bounce loop needs no clearing launches
@@ -109,6 +113,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
     uint32_t rid = 0;
     RayPre r;
     Trav t;
+    TravView view{sc.nodes, sc.tris, MULTI ? 0u : sc.top_nodes};
     while (true) {
         const uint64_t busy = __ballot(active);
         const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
@@ -118,6 +123,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
                 rid = my;
                 float4 o4 = ray_o[(size_t)rid * ray_stride];
                 float4 d4 = ray_d[(size_t)rid * ray_stride];
+                if constexpr (MULTI) {
+                    const MultiItem& it = items[item_of[__float_as_uint(d4.w)]];
+                    view.nodes = it.nodes;
+                    view.tris = it.tris;
+                }
                 r = ray_setup(xyz(o4), xyz(d4));
                 trav_init(t, tmax_from_ray ? o4.w : INFINITY);   // extension rays: tMax = Infinity (PathTracer.h:167)
                 active = true;
@@ -127,7 +137,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
         if (__ballot(active) == 0) break;
         if (active) {
             bool occ = false;
-            if (trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+            bool done;
+            if constexpr (MULTI)
+                done = trav_step<ANY, LDS_DEPTH, COUNT>(view, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
+            else
+                done = trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
+            if (done) {
                 if (ANY) {
                     occluded[rid] = occ ? 1 : 0;
                 } else {
@@ -143,15 +158,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
     if (COUNT) flush_counts(visit_totals, vc);
 }
 
-#define XT_INST_TRACE(ANY, COUNT)                                                                                  \
-    template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT>(DevScene, const float4*, const float4*, int,     \
-                                                                  const uint32_t*, uint32_t, float4*, uint8_t*, int, \
-                                                                  int, uint32_t*, unsigned long long*, uint32_t*,    \
-                                                                  uint32_t*);
-XT_INST_TRACE(false, false)
-XT_INST_TRACE(true, false)
-XT_INST_TRACE(false, true)
-XT_INST_TRACE(true, true)
+#define XT_INST_TRACE(ANY, COUNT, MULTI)                                                                           \
+    template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT, MULTI>(                                          \
+        DevScene, const float4*, const float4*, int, const uint32_t*, uint32_t, float4*, uint8_t*, int, int,       \
+        uint32_t*, unsigned long long*, uint32_t*, uint32_t*, const uint32_t*, const MultiItem*);
+XT_INST_TRACE(false, false, false)
+XT_INST_TRACE(true, false, false)
+XT_INST_TRACE(false, true, false)
+XT_INST_TRACE(true, true, false)
+XT_INST_TRACE(false, false, true)
 #undef XT_INST_TRACE
 
 // ---------------------------------------------------------------- wavefront PT
@@ -778,6 +793,249 @@ __global__ void k_advance_cursor(BatchCursor* c, BatchPlan plan) {
 
 __global__ void k_count_from_cursor(uint32_t* count, const BatchCursor* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = c->num_paths;
+}
+
+}  // namespace xt
+
+// ---------------------------------------------------------------- unified batches
+
+namespace xt {
+
+// Window [0, window) of the batch.
+__global__ void k_window_init(MultiBatch mb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MultiCursor c{};
+    c.g0 = 0;
+    c.n = (uint32_t)min((uint64_t)mb.window, mb.total_paths);
+    *mb.cursor = c;
+}
+
+// The next window (n = 0 past the end).
+__global__ void k_window_advance(MultiBatch mb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MultiCursor c = *mb.cursor;
+    c.g0 += mb.window;
+    c.n = c.g0 < mb.total_paths ? (uint32_t)min((uint64_t)mb.window, mb.total_paths - c.g0) : 0u;
+    *mb.cursor = c;
+}
+
+// The items the window overlaps and their accumulation units: item
+// first_item + j accumulates min(npix, its paths in the window) pixels,
+// numbered from units[j].
+__global__ void k_window_plan(MultiBatch mb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MultiCursor c = *mb.cursor;
+    const uint64_t lo = c.g0, hi = c.g0 + c.n;
+    uint32_t k = 0;
+    while (k < mb.num_items && mb.items[k].path_base + mb.items[k].paths <= lo) ++k;
+    c.first_item = k;
+    uint32_t j = 0, u = 0;
+    for (; c.n > 0 && k < mb.num_items && mb.items[k].path_base < hi; ++k, ++j) {
+        const MultiItem& it = mb.items[k];
+        const uint64_t a = max(it.path_base, lo), b = min(it.path_base + it.paths, hi);
+        mb.units[j] = u;
+        u += (uint32_t)min((uint64_t)it.npix, b - a);
+    }
+    mb.units[j] = u;
+    c.num_overlap = j;
+    *mb.cursor = c;
+}
+
+__global__ void k_count_from_window(uint32_t* count, const MultiCursor* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = c->n;
+}
+
+// The window's overlapping item holding window path id (largest j with
+// path_base <= g0 + id).
+__device__ __forceinline__ uint32_t window_item(const MultiBatch& mb, const MultiCursor& c, uint32_t id) {
+    const uint64_t g = c.g0 + id;
+    uint32_t lo = c.first_item, hi = c.first_item + c.num_overlap - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (mb.items[mid].path_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Camera rays of the window's paths (k_generate per item).
+__global__ __launch_bounds__(256) void k_generate_multi(MultiBatch mb, PathQueue q, float4* __restrict__ radiance) {
+    const MultiCursor c = *mb.cursor;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += gridDim.x * blockDim.x) {
+        const uint32_t k = window_item(mb, c, id);
+        const MultiItem& it = mb.items[k];
+        const uint64_t local = c.g0 + id - it.path_base;
+        const uint32_t s_local = (uint32_t)(local / it.npix);
+        const uint32_t pixel = (uint32_t)(local - (uint64_t)s_local * it.npix);
+        const WaveParams& wp = it.wp;
+        const uint32_t x = pixel % wp.width, y = pixel / wp.width;
+        Rng rng{make_uint2(wp.seed, wp.stream), pixel, it.spp_offset + s_local};
+        uint4 b0 = rng.block(0, 0);
+        float ux = Rng::to_unit(b0.x), uy = Rng::to_unit(b0.y);
+        float fx = (float)x, fy = (float)y;
+        fx /= (float)wp.width;
+        fy /= (float)wp.height;
+        fy = 1.0f - fy;
+        fx += (ux - 0.5f) / (float)wp.width;
+        fy += (uy - 0.5f) / (float)wp.height;
+        f3 dc = normalize(mk(wp.tan_x * (fx - 0.5f) * 2, wp.tan_y * (fy - 0.5f) * 2, 1));
+        f3 o = xf_point(wp.cam, mk(0, 0, 0));
+        f3 d = xf_vector(wp.cam, dc);
+        q.o[id] = make_float4(o.x, o.y, o.z, 0.0f);
+        q.d[id] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
+        q.w[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));
+        q.e[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        mb.item_of[id] = k;
+        radiance[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// Path id -> (pixel, sample) of a unified batch's path.
+struct ItemMap {
+    const MultiItem& it;
+    uint64_t g0;
+    __device__ __forceinline__ uint2 operator()(uint32_t id) const {
+        const uint64_t local = g0 + id - it.path_base;
+        const uint32_t s_local = (uint32_t)(local / it.npix);
+        return make_uint2((uint32_t)(local - (uint64_t)s_local * it.npix), it.spp_offset + s_local);
+    }
+};
+
+// The shade step of a unified batch: each entry with its item's parameters
+// and scene (unguided).
+template <bool NMAP>
+__global__ __launch_bounds__(SHADE_BLOCK, NMAP ? 1 : 4) void k_shade_multi(MultiBatch mb, PathQueue qin,
+                                                             const uint32_t* __restrict__ count_in,
+                                                             const float4* __restrict__ hits, PathQueue qout,
+                                                             uint32_t* __restrict__ count_out, ShadowQueue sq,
+                                                             uint32_t* __restrict__ count_shadow,
+                                                             float4* __restrict__ radiance, float4* __restrict__ pending) {
+    const uint32_t n = *count_in;
+    const uint64_t g0 = mb.cursor->g0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    __shared__ uint32_t append_lds[2 * (SHADE_BLOCK / 64) + 2];
+    const GuideRender gr{};
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        ShadeOut o;
+        if (i < n) {
+            const MultiItem& it = mb.items[mb.item_of[__float_as_uint(qin.d[i].w)]];
+            const DevScene& sc = mb.scenes[it.scene];
+            shade_entry<false, NMAP>(it.wp, sc, shade_const(it.wp, sc), qin, hits, radiance, pending, gr, i,
+                                     ItemMap{it, g0}, o);
+        }
+        shade_emit<false>(o, qout, count_out, sq, count_shadow, gr, append_lds);
+    }
+}
+template __global__ void k_shade_multi<false>(MultiBatch, PathQueue, const uint32_t*, const float4*, PathQueue,
+                                              uint32_t*, ShadowQueue, uint32_t*, float4*, float4*);
+template __global__ void k_shade_multi<true>(MultiBatch, PathQueue, const uint32_t*, const float4*, PathQueue,
+                                             uint32_t*, ShadowQueue, uint32_t*, float4*, float4*);
+
+// Any-hit for a unified batch's shadow queue, each ray in its item's scene.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
+                                                              float4* __restrict__ radiance, float4* __restrict__ pending,
+                                                              uint32_t* __restrict__ ovf,
+                                                              const uint32_t* __restrict__ item_of,
+                                                              const MultiItem* __restrict__ items) {
+    __shared__ uint32_t lds[TRACE_LDS_DEPTH * TRACE_BLOCK];
+    const uint32_t n = *count;
+    const size_t nthreads = (size_t)gridDim.x * blockDim.x;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* my_lds = lds + threadIdx.x;
+    uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
+    const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
+    const uint32_t per = (n + nwaves - 1) / nwaves;
+    uint32_t next = min(wave * per, n);
+    const uint32_t end = min(next + per, n);
+    const uint64_t below = (1ull << __lane_id()) - 1ull;
+    VisitCount vc;
+    bool active = false;
+    uint32_t rid = 0, id = 0;
+    RayPre r;
+    Trav t;
+    TravView view{nullptr, nullptr, 0u};
+    while (true) {
+        const uint64_t busy = __ballot(active);
+        const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
+        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
+            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < end) {
+                rid = my;
+                float4 o4 = sq.o[rid], d4 = sq.d[rid];
+                id = __float_as_uint(d4.w);
+                const MultiItem& it = items[item_of[id]];
+                view.nodes = it.nodes;
+                view.tris = it.tris;
+                r = ray_setup(xyz(o4), xyz(d4));
+                trav_init(t, o4.w);
+                active = true;
+            }
+            next = min(next + nidle, end);
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            bool occ = false;
+            if (trav_step<true, TRACE_LDS_DEPTH, false>(view, r, t, nullptr, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc,
+                                                        &occ)) {
+                const float4 c4 = sq.c[rid];
+                const f3 c = occ ? splat(0.0f) : xyz(c4);
+                if (c4.w != 0.0f) {
+                    pending[id] = f4(c, 0.0f);
+                } else if (!occ) {
+                    radiance[id] = f4(xyz(radiance[id]) + c, 0.0f);
+                }
+                active = false;
+            }
+        }
+    }
+}
+
+// film[pixel] += the window's samples of that pixel, in sample order, for
+// every item the window overlaps (one thread per item pixel).
+__global__ __launch_bounds__(256) void k_accumulate_multi(MultiBatch mb, const float4* __restrict__ radiance) {
+    const MultiCursor c = *mb.cursor;
+    if (c.n == 0) return;
+    const uint32_t total = mb.units[c.num_overlap];
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = c.num_overlap - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (mb.units[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const MultiItem& it = mb.items[c.first_item + lo];
+        const uint64_t a = max(it.path_base, c.g0), b = min(it.path_base + it.paths, c.g0 + c.n);
+        const uint64_t first = a - it.path_base + (u - mb.units[lo]);   // local index of this pixel's first sample
+        const uint64_t last = b - it.path_base;
+        const uint32_t pixel = (uint32_t)(first % it.npix);
+        const size_t o = 3 * (size_t)pixel;
+        if (it.film_f64) {
+            double* f = (double*)it.film;
+            double rr = f[o], gg = f[o + 1], bb = f[o + 2];
+            for (uint64_t l = first; l < last; l += it.npix) {
+                const float4 L = radiance[it.path_base + l - c.g0];
+                rr += (double)L.x;
+                gg += (double)L.y;
+                bb += (double)L.z;
+            }
+            f[o] = rr;
+            f[o + 1] = gg;
+            f[o + 2] = bb;
+        } else {
+            float* f = (float*)it.film;
+            float rr = f[o], gg = f[o + 1], bb = f[o + 2];
+            for (uint64_t l = first; l < last; l += it.npix) {
+                const float4 L = radiance[it.path_base + l - c.g0];
+                rr += L.x;
+                gg += L.y;
+                bb += L.z;
+            }
+            f[o] = rr;
+            f[o + 1] = gg;
+            f[o + 2] = bb;
+        }
+    }
 }
 
 }  // namespace xt

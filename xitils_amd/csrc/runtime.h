@@ -8,10 +8,13 @@
 
 #include "../../include/xitils_amd.h"
 
+struct MultiState;   // unified-batch buffers (runtime.hip)
+
 struct xt_device {
     int ordinal = 0;
     int num_cu = 0;
     std::string arch;
+    MultiState* multi = nullptr;        // created by the first unified xt_render_batch
     unsigned char* scratch = nullptr;   // this device's address of the static scratch array (xt_render_static)
 };
 

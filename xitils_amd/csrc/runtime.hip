@@ -208,7 +208,10 @@ xt_status xt_device_open(int ordinal, xt_device** out) {
     return XT_OK;
 }
 
+static void multi_release(xt_device* dev);
+
 xt_status xt_device_close(xt_device* dev) {
+    if (dev) multi_release(dev);
     delete dev;
     return XT_OK;
 }
@@ -1427,6 +1430,261 @@ static xt_status enqueue_graph_render(xt_scene* s, const RenderGraph& g, uint32_
     return XT_OK;
 }
 
+// ---------------------------------------------------------------- unified batches
+
+// Buffers of the unified wavefront, one set per device: the path state of a
+// window (as a render's RenderBuffers, plus each path's item), the traversal
+// spill columns, and the batch's item and scene tables.
+struct MultiState {
+    hipStream_t stream = nullptr;
+    uint32_t capacity = 0;
+    PathQueue q[2] = {};
+    ShadowQueue sq = {};
+    float4* hits = nullptr;
+    float4* radiance = nullptr;
+    float4* pending = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t* item_of = nullptr;
+    uint32_t* ovf = nullptr;
+    size_t ovf_entries = 0;
+    MultiCursor* cursor = nullptr;
+    MultiItem* items = nullptr;
+    uint32_t items_cap = 0;
+    DevScene* scenes = nullptr;
+    uint32_t scenes_cap = 0;
+    uint32_t* units = nullptr;
+    std::vector<hipEvent_t> events;   // one per window, plus the start
+    void release_paths() {
+        for (int k = 0; k < 2; ++k) {
+            hipFree(q[k].o); hipFree(q[k].d); hipFree(q[k].w); hipFree(q[k].e);
+        }
+        hipFree(sq.o); hipFree(sq.d); hipFree(sq.c);
+        hipFree(hits); hipFree(radiance); hipFree(pending); hipFree(item_of);
+        for (int k = 0; k < 2; ++k) q[k] = PathQueue{};
+        sq = ShadowQueue{};
+        hits = radiance = pending = nullptr;
+        item_of = nullptr;
+        capacity = 0;
+    }
+};
+
+static void multi_release(xt_device* dev) {
+    MultiState* m = dev->multi;
+    if (!m) return;
+    hipSetDevice(dev->ordinal);
+    if (m->stream) hipStreamSynchronize(m->stream);
+    m->release_paths();
+    hipFree(m->counts); hipFree(m->ovf); hipFree(m->cursor); hipFree(m->items); hipFree(m->scenes); hipFree(m->units);
+    for (hipEvent_t e : m->events) hipEventDestroy(e);
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+    dev->multi = nullptr;
+}
+
+// Which items share a wavefront.  Unguided items of at most one graph
+// replay (small renders, whose bounce loops are mostly launch and tail
+// overhead) by default; XITILS_AMD_BATCH=unified puts every unguided item in
+// it, =items none (measurement switches: the bench compares the forms).
+enum class BatchMode { Hybrid, Unified, Items };
+static BatchMode batch_mode() {
+    const char* e = getenv("XITILS_AMD_BATCH");
+    if (e && std::strcmp(e, "items") == 0) return BatchMode::Items;
+    if (e && std::strcmp(e, "unified") == 0) return BatchMode::Unified;
+    return BatchMode::Hybrid;
+}
+
+// One launched unified run: its graph, events and tables, finished after
+// the batch's final synchronisation.
+struct UnifiedRun {
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;     // start, then the end of each window
+    std::vector<uint32_t> item_index;   // batch item of each table entry
+    std::vector<uint64_t> last_window;  // window holding each table entry's last path
+    std::vector<MultiItem> mi;          // host tables, kept until the copies are done
+    std::vector<DevScene> ds;
+    void release() {
+        if (exec) hipGraphExecDestroy(exec);
+        for (hipEvent_t e : events) hipEventDestroy(e);
+        exec = nullptr;
+        events.clear();
+    }
+};
+
+// One wavefront for every item: the items' paths numbered one after another
+// and walked in windows of up to the path capacity; each window is one
+// replay of a captured graph (plan, generate, max_depth + 1 rounds of extend
+// / shade / shadow over whichever items' paths are in it, accumulate,
+// advance).  Each film equals xt_render's bit for bit: every path draws its
+// own random numbers and each pixel's samples are summed in sample order.
+static xt_status render_unified(xt_batch_item* items, const std::vector<uint32_t>& which, const std::vector<void*>& d_films,
+                                UnifiedRun& run) {
+    xt_device* dev = items[0].scene->dev;
+    XT_HIP(hipSetDevice(dev->ordinal));
+    if (!dev->multi) {
+        dev->multi = new (std::nothrow) MultiState();
+        if (!dev->multi) return fail(XT_ERR_OUT_OF_MEMORY, "host allocation failed");
+        XT_HIP(hipStreamCreateWithFlags(&dev->multi->stream, hipStreamNonBlocking));
+        XT_HIP(dalloc(&dev->multi->counts, 4));
+        XT_HIP(dalloc(&dev->multi->cursor, 1));
+    }
+    MultiState& m = *dev->multi;
+    // item and scene tables (items without samples are left out)
+    std::vector<MultiItem>& mi = run.mi;
+    std::vector<uint32_t>& item_index = run.item_index;
+    std::vector<xt_scene*> sc_list;
+    std::vector<DevScene>& ds = run.ds;
+    uint64_t total = 0;
+    int max_depth = 1;
+    bool nmap = false;
+    uint32_t ovf_depth = 0;
+    for (uint32_t i : which) {
+        const xt_batch_item& it = items[i];
+        if (it.params.spp == 0) continue;
+        xt_scene* s = it.scene;
+        uint32_t si = 0;
+        while (si < sc_list.size() && sc_list[si] != s) ++si;
+        if (si == sc_list.size()) {
+            sc_list.push_back(s);
+            ds.push_back(s->ds);
+            nmap |= s->nmap;
+            ovf_depth = std::max(ovf_depth, s->ovf_depth);
+        }
+        MultiItem e{};
+        e.wp = cursor_wave_params(&it.camera, &it.params, nullptr);
+        e.nodes = s->ds.nodes;
+        e.tris = s->ds.tris;
+        e.scene = si;
+        e.npix = it.params.width * it.params.height;
+        e.spp_offset = it.params.spp_offset;
+        e.film_f64 = it.film_is_f64 ? 1u : 0u;
+        e.film = d_films[i];
+        e.path_base = total;
+        e.paths = (uint64_t)e.npix * it.params.spp;
+        total += e.paths;
+        max_depth = std::max(max_depth, it.params.max_depth);
+        mi.push_back(e);
+        item_index.push_back(i);
+    }
+    if (mi.empty()) return XT_OK;
+    XT_HIP(hipStreamSynchronize(m.stream));   // an earlier batch's replays may still read the tables
+    const uint32_t window = (uint32_t)std::min<uint64_t>(path_capacity(), total);
+    const uint64_t nwin = (total + window - 1) / window;
+    if (nwin >= (1ull << 32)) return fail(XT_ERR_TOO_LARGE, "batch of more than 2^32 windows");
+    // buffers
+    if (m.capacity < window) {
+        hipStreamSynchronize(m.stream);
+        m.release_paths();
+        const size_t c = window;
+        for (int k = 0; k < 2; ++k) {
+            XT_HIP(dalloc(&m.q[k].o, c)); XT_HIP(dalloc(&m.q[k].d, c));
+            XT_HIP(dalloc(&m.q[k].w, c)); XT_HIP(dalloc(&m.q[k].e, c));
+        }
+        XT_HIP(dalloc(&m.sq.o, c)); XT_HIP(dalloc(&m.sq.d, c)); XT_HIP(dalloc(&m.sq.c, c));
+        XT_HIP(dalloc(&m.hits, c)); XT_HIP(dalloc(&m.radiance, c)); XT_HIP(dalloc(&m.pending, c));
+        XT_HIP(dalloc(&m.item_of, c));
+        m.capacity = window;
+    }
+    xt_scene* s0 = sc_list[0];
+    const uint32_t tblocks = std::max(1u, std::min(s0->trace_grid, (window + TRACE_BLOCK - 1) / TRACE_BLOCK));
+    const size_t ovf_need = (size_t)ovf_depth * tblocks * TRACE_BLOCK;
+    if (m.ovf_entries < ovf_need) {
+        hipStreamSynchronize(m.stream);
+        hipFree(m.ovf);
+        m.ovf = nullptr;
+        XT_HIP(dalloc(&m.ovf, ovf_need));
+        m.ovf_entries = ovf_need;
+    }
+    if (m.items_cap < mi.size()) {
+        hipStreamSynchronize(m.stream);
+        hipFree(m.items);
+        hipFree(m.units);
+        m.items = nullptr;
+        m.units = nullptr;
+        XT_HIP(dalloc(&m.items, mi.size()));
+        XT_HIP(dalloc(&m.units, mi.size() + 1));
+        m.items_cap = (uint32_t)mi.size();
+    }
+    if (m.scenes_cap < ds.size()) {
+        hipStreamSynchronize(m.stream);
+        hipFree(m.scenes);
+        m.scenes = nullptr;
+        XT_HIP(dalloc(&m.scenes, ds.size()));
+        m.scenes_cap = (uint32_t)ds.size();
+    }
+    // the tables are uploaded on the batch's stream (ordered after an earlier
+    // run's replays, which read the same buffers)
+    XT_HIP(hipMemcpyAsync(m.items, mi.data(), mi.size() * sizeof(MultiItem), hipMemcpyHostToDevice, m.stream));
+    XT_HIP(hipMemcpyAsync(m.scenes, ds.data(), ds.size() * sizeof(DevScene), hipMemcpyHostToDevice, m.stream));
+    MultiBatch mb{};
+    mb.items = m.items;
+    mb.scenes = m.scenes;
+    mb.num_items = (uint32_t)mi.size();
+    mb.window = window;
+    mb.total_paths = total;
+    mb.cursor = m.cursor;
+    mb.item_of = m.item_of;
+    mb.units = m.units;
+    const uint32_t ncu = (uint32_t)dev->num_cu;
+    const uint32_t gblocks = std::max(1u, std::min((window + 255u) / 256u, ncu * 8));
+    const uint32_t sblocks = std::max(1u, std::min((window + SHADE_BLOCK - 1) / SHADE_BLOCK, s0->shade_grid));
+    // capture one window's bounce loop
+    XT_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_window_plan, dim3(1), dim3(1), 0, m.stream, mb);
+    hipLaunchKernelGGL(k_count_from_window, dim3(1), dim3(1), 0, m.stream, m.counts, m.cursor);
+    hipLaunchKernelGGL(k_generate_multi, dim3(gblocks), dim3(256), 0, m.stream, mb, m.q[0], m.radiance);
+    int cur = 0;
+    for (int it = 0; it <= max_depth; ++it) {
+        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false, true>), dim3(tblocks), dim3(TRACE_BLOCK), 0,
+                           m.stream, s0->ds, m.q[cur].o, m.q[cur].d, 1, m.counts + cur, 0u, m.hits, nullptr, 0, 0,
+                           m.ovf, nullptr, m.counts + (1 - cur), m.counts + 2, m.item_of, m.items);
+        if (nmap)
+            hipLaunchKernelGGL(k_shade_multi<true>, dim3(sblocks), dim3(SHADE_BLOCK), 0, m.stream, mb, m.q[cur],
+                               m.counts + cur, m.hits, m.q[1 - cur], m.counts + (1 - cur), m.sq, m.counts + 2,
+                               m.radiance, m.pending);
+        else
+            hipLaunchKernelGGL(k_shade_multi<false>, dim3(sblocks), dim3(SHADE_BLOCK), 0, m.stream, mb, m.q[cur],
+                               m.counts + cur, m.hits, m.q[1 - cur], m.counts + (1 - cur), m.sq, m.counts + 2,
+                               m.radiance, m.pending);
+        hipLaunchKernelGGL(k_shadow_multi, dim3(tblocks), dim3(TRACE_BLOCK), 0, m.stream, m.sq, m.counts + 2,
+                           m.radiance, m.pending, m.ovf, m.item_of, m.items);
+        cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_accumulate_multi, dim3(gblocks), dim3(256), 0, m.stream, mb, m.radiance);
+    hipLaunchKernelGGL(k_window_advance, dim3(1), dim3(1), 0, m.stream, mb);
+    hipError_t launch_err = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(m.stream, &graph);
+    if (launch_err != hipSuccess || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(XT_ERR_DEVICE, std::string("unified batch capture failed: ") +
+                                       hipGetErrorString(launch_err != hipSuccess ? launch_err : e));
+    }
+    e = hipGraphInstantiate(&run.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+    run.events.assign(nwin + 1, nullptr);
+    for (hipEvent_t& ev : run.events) XT_HIP(hipEventCreate(&ev));
+    hipEventRecord(run.events[0], m.stream);
+    hipLaunchKernelGGL(k_window_init, dim3(1), dim3(1), 0, m.stream, mb);
+    for (uint64_t w = 0; w < nwin; ++w) {
+        e = hipGraphLaunch(run.exec, m.stream);
+        if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("graph launch failed: ") + hipGetErrorString(e));
+        hipEventRecord(run.events[w + 1], m.stream);
+    }
+    for (size_t k = 0; k < mi.size(); ++k) run.last_window.push_back((mi[k].path_base + mi[k].paths - 1) / window);
+    return XT_OK;
+}
+
+// After the batch's synchronisation: each unified item's latency, batch
+// start to the end of the window holding its last path.
+static void unified_finish(xt_batch_item* items, UnifiedRun& run) {
+    for (size_t k = 0; k < run.item_index.size(); ++k) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, run.events[0], run.events[run.last_window[k] + 1]);
+        items[run.item_index[k]].ms = ms;
+    }
+}
+
 // Items with more paths than one graph replay holds render through
 // xt_render's lagged bounce loop: per item on the 64-room batch it is 15-60%
 // faster than replaying the fixed max_depth + 1 rounds per 4 Mi paths, while
@@ -1445,11 +1703,12 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
     }
     struct Pending {
         void* d_film = nullptr;
-        bool owned = false;
+        bool owned = false;        // a device copy of a host film, copied back at the end
         size_t bytes = 0;
         hipEvent_t a = nullptr, b = nullptr;
     };
     std::vector<Pending> pend(n);
+    std::vector<UnifiedRun> runs;
     xt_status result = XT_OK;
     auto cleanup = [&]() {
         for (Pending& q : pend) {
@@ -1457,12 +1716,14 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
             if (q.a) hipEventDestroy(q.a);
             if (q.b) hipEventDestroy(q.b);
         }
+        for (UnifiedRun& r : runs) r.release();
     };
+    // films on the device: one device copy per distinct host film
+    std::vector<void*> d_films(n, nullptr);
     for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
         xt_batch_item& it = items[i];
-        xt_scene* s = it.scene;
         it.ms = 0.0f;
-        if (hipSetDevice(s->dev->ordinal) != hipSuccess) {
+        if (hipSetDevice(it.scene->dev->ordinal) != hipSuccess) {
             result = fail(XT_ERR_DEVICE, "hipSetDevice failed");
             break;
         }
@@ -1470,47 +1731,117 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
         q.bytes = (size_t)it.params.width * it.params.height * 3 * (it.film_is_f64 ? 8 : 4);
         q.d_film = it.film;
         if (!it.film_on_device) {
+            uint32_t j = 0;
+            while (j < i && !(!items[j].film_on_device && items[j].film == it.film)) ++j;
+            if (j < i) {
+                q.d_film = d_films[j];
+                d_films[i] = d_films[j];
+                continue;
+            }
             if (hipMalloc(&q.d_film, q.bytes) != hipSuccess) {
                 result = fail(XT_ERR_OUT_OF_MEMORY, "film allocation failed");
                 break;
             }
             q.owned = true;
-            hipMemcpyAsync(q.d_film, it.film, q.bytes, hipMemcpyHostToDevice, s->stream);
+            if (hipMemcpy(q.d_film, it.film, q.bytes, hipMemcpyHostToDevice) != hipSuccess)
+                result = fail(XT_ERR_DEVICE, "film copy failed");
         }
+        d_films[i] = q.d_film;
+    }
+    // which items share the wavefront; items that share a film with another
+    // item all go through it (it keeps their order), when they can
+    const BatchMode mode = batch_mode();
+    std::vector<uint8_t> uni(n, 0);
+    bool shared_film = false, any_guided = false, one_device = true;
+    for (uint32_t i = 0; i < n; ++i) {
+        any_guided |= (items[i].params.flags & XT_FLAG_GUIDE) != 0;
+        one_device &= items[i].scene->dev == items[0].scene->dev;
+        for (uint32_t j = 0; j < i; ++j) shared_film |= d_films[j] == d_films[i];
+    }
+    // a film shared by items that cannot all share one wavefront: every item
+    // through xt_render, in item order
+    const bool in_order = shared_film && (any_guided || !one_device);
+    for (uint32_t i = 0; i < n && in_order && result == XT_OK; ++i) {
+        xt_batch_item& it = items[i];
         if (it.params.spp == 0) continue;
-        if (batch_item_uses_loop(it)) continue;   // rendered after every graph item is under way
+        result = xt_render(it.scene, &it.camera, &it.params, d_films[i], it.film_is_f64, 1);
+        if (result == XT_OK) it.ms = (float)it.scene->stats.ms_total;
+    }
+    for (uint32_t i = 0; i < n && !in_order; ++i) {
+        const xt_batch_item& it = items[i];
+        const bool can = one_device && !(it.params.flags & XT_FLAG_GUIDE) && it.params.spp > 0;
+        if (!can || mode == BatchMode::Items) continue;
+        const bool small = (uint64_t)it.params.width * it.params.height * it.params.spp <=
+                           graph_capacity((uint64_t)it.params.width * it.params.height, it.params.spp);
+        uni[i] = mode == BatchMode::Unified || small || (shared_film && !any_guided);
+    }
+    // per-item graphs first (their scenes' streams), so the unified runs and
+    // the lagged-loop items below overlap them
+    for (uint32_t i = 0; i < n && result == XT_OK && !in_order; ++i) {
+        xt_batch_item& it = items[i];
+        if (uni[i] || it.params.spp == 0 || batch_item_uses_loop(it)) continue;
+        xt_scene* s = it.scene;
+        hipSetDevice(s->dev->ordinal);
         RenderGraph* g = nullptr;
-        result = find_graph(s, &it.camera, &it.params, q.d_film, it.film_is_f64, &g);
+        result = find_graph(s, &it.camera, &it.params, d_films[i], it.film_is_f64, &g);
         if (result != XT_OK) break;
+        Pending& q = pend[i];
         hipEventCreate(&q.a);
         hipEventCreate(&q.b);
         hipEventRecord(q.a, s->stream);
         result = enqueue_graph_render(s, *g, it.params.spp_offset);
         hipEventRecord(q.b, s->stream);
     }
+    // the wavefront: in runs whose films are distinct (an item whose film an
+    // earlier item of the run accumulates into starts the next run, so a
+    // shared film receives the items' samples in item order)
+    {
+        std::vector<uint32_t> cur;
+        auto flush = [&]() {
+            if (cur.empty() || result != XT_OK) return;
+            runs.emplace_back();
+            result = render_unified(items, cur, d_films, runs.back());
+            cur.clear();
+        };
+        for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
+            if (!uni[i]) continue;
+            bool clash = false;
+            for (uint32_t k : cur) clash |= d_films[k] == d_films[i];
+            if (clash) flush();
+            cur.push_back(i);
+        }
+        flush();
+    }
     // items larger than one graph replay: the lagged bounce loop of xt_render,
-    // one after another, while the graph items run on their own streams
-    for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
+    // one after another, while the graphs and the wavefront run
+    for (uint32_t i = 0; i < n && result == XT_OK && !in_order; ++i) {
         xt_batch_item& it = items[i];
-        if (it.params.spp == 0 || !batch_item_uses_loop(it)) continue;
-        result = xt_render(it.scene, &it.camera, &it.params, pend[i].d_film, it.film_is_f64, 1);
+        if (uni[i] || it.params.spp == 0 || !batch_item_uses_loop(it)) continue;
+        result = xt_render(it.scene, &it.camera, &it.params, d_films[i], it.film_is_f64, 1);
         if (result == XT_OK) it.ms = (float)it.scene->stats.ms_total;
     }
     for (uint32_t i = 0; i < n; ++i) {
-        xt_batch_item& it = items[i];
-        xt_scene* s = it.scene;
-        hipError_t e = hipStreamSynchronize(s->stream);
+        hipError_t e = hipStreamSynchronize(items[i].scene->stream);
         if (e != hipSuccess && result == XT_OK) result = fail(XT_ERR_DEVICE, std::string("batch render failed: ") + hipGetErrorString(e));
-        Pending& q = pend[i];
-        if (result == XT_OK && q.a) hipEventElapsedTime(&it.ms, q.a, q.b);
-        if (result == XT_OK && q.owned) {
-            e = hipMemcpy(it.film, q.d_film, q.bytes, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) result = fail(XT_ERR_DEVICE, "film copy failed");
+    }
+    if (!runs.empty()) {
+        hipError_t e = hipStreamSynchronize(items[0].scene->dev->multi->stream);
+        if (e != hipSuccess && result == XT_OK) result = fail(XT_ERR_DEVICE, std::string("batch render failed: ") + hipGetErrorString(e));
+    }
+    if (result == XT_OK) {
+        for (UnifiedRun& r : runs) unified_finish(items, r);
+        for (uint32_t i = 0; i < n; ++i)
+            if (pend[i].a) hipEventElapsedTime(&items[i].ms, pend[i].a, pend[i].b);
+        for (uint32_t i = 0; i < n && result == XT_OK; ++i) {
+            if (!pend[i].owned) continue;
+            if (hipMemcpy(items[i].film, pend[i].d_film, pend[i].bytes, hipMemcpyDeviceToHost) != hipSuccess)
+                result = fail(XT_ERR_DEVICE, "film copy failed");
         }
     }
     cleanup();
     return result;
 }
+
 
 // ---------------------------------------------------------------- render without allocation
 
@@ -1585,6 +1916,7 @@ xt_status xt_render_static(xt_scene* s, const xt_camera* cam, const xt_render_pa
     s->stats.camera_rays = npix * p->spp;
     return XT_OK;
 }
+
 
 // Times the graph-replayed render of one batch for a few launch shapes and
 // keeps the fastest (traversal blocks per CU x shade blocks per CU).

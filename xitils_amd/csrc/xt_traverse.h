@@ -190,6 +190,16 @@ __device__ __forceinline__ void stage_top_nodes(float4* top, const DevScene& sc)
     }
 }
 
+// What the traversal reads of a scene: nodes, leaf-ordered triangles and the
+// number of LDS-staged top nodes.  DevScene has the same members, so the
+// traversal takes either (SV = DevScene for one scene, uniform across the
+// grid; SV = TravView per lane when a queue mixes scenes).
+struct TravView {
+    const float4* nodes;
+    const float4* tris;
+    uint32_t top_nodes;
+};
+
 // Fetch counters of the counting build (algorithmic traffic accounting).
 struct VisitCount {
     uint32_t nodes = 0, tris = 0;
@@ -224,8 +234,8 @@ __device__ __forceinline__ void trav_init(Trav& t, float tmax) {
 // Visits internal node t.node: true when t.node is now the next child to
 // visit (the nearest hit one; the other hit children are pushed), false when
 // no child was hit.
-template <bool ANY, int LDS_DEPTH, bool COUNT>
-__device__ __forceinline__ bool visit_node(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
+template <bool ANY, int LDS_DEPTH, bool COUNT, class SV>
+__device__ __forceinline__ bool visit_node(const SV& sc, const RayPre& r, Trav& t, const float4* top,
                                            uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
                                            VisitCount& vc) {
     const float4* nd = sc.nodes + (size_t)t.node * NODE_F4;
@@ -346,8 +356,8 @@ __device__ __forceinline__ bool visit_node(const DevScene& sc, const RayPre& r, 
 
 // Tests the first triangle of leaf code `leaf` (< 0); returns the code of the
 // rest of the leaf, or 0 when the leaf is done (or, for ANY, when *occ was set).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ int visit_tri(const DevScene& sc, const RayPre& r, Trav& t, int leaf, VisitCount& vc,
+template <bool ANY, bool COUNT, class SV>
+__device__ __forceinline__ int visit_tri(const SV& sc, const RayPre& r, Trav& t, int leaf, VisitCount& vc,
                                          bool* occ) {
     uint32_t code = (uint32_t)(~leaf);
     uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
@@ -386,8 +396,8 @@ __device__ __forceinline__ bool trav_pop(Trav& t, const uint32_t* lds, int lds_s
 
 // One traversal step (one node or one triangle); returns true when the ray
 // is finished (for ANY, *occ tells whether a blocker was found).
-template <bool ANY, int LDS_DEPTH, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene& sc, const RayPre& r, Trav& t, const float4* top,
+template <bool ANY, int LDS_DEPTH, bool COUNT, class SV>
+__device__ __forceinline__ bool trav_step(const SV& sc, const RayPre& r, Trav& t, const float4* top,
                                           uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
                                           VisitCount& vc, bool* occ) {
     if (t.node >= 0) {
