@@ -52,7 +52,8 @@ def encode(q, lo, hi, dtype=np.float32):
         for k in range(G_BANDS):
             x[:, a * 2 * G_BANDS + 2 * k] = s
             x[:, a * 2 * G_BANDS + 2 * k + 1] = c
-            s, c = (dtype(2) * s * c).astype(dtype), (c * c - s * s).astype(dtype)
+            # sin 2x = (s + s) c, cos 2x = 1 - 2 s^2 with one rounding (an fma)
+            s, c = ((s + s) * c).astype(dtype), (1.0 - 2.0 * s.astype(np.float64) * s).astype(dtype)
     x[:, 72:75] = q[:, 3:6]
     x[:, 75:78] = q[:, 6:9]
     return x

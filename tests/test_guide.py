@@ -154,6 +154,48 @@ def test_guide_gradient_matches_reference(gscene):
 
 
 @pytest.mark.gpu
+def test_guide_gradient_over_several_block_passes(gscene):
+    # more samples than one pass of the gradient kernel's blocks holds (one
+    # 128-sample group per CU): the blocks add later groups into their rows
+    _, sc = gscene
+    lo, hi = _bounds(sc)
+    P = G.init_params(15)
+    sc.guide_set_params(P)
+    n = 70001
+    print(f"[gpu] guide gradient n={n}", flush=True)
+    q = G.random_queries(n, lo, hi, 7)
+    d = G.random_dirs(n, 8)
+    w = np.random.default_rng(9).exponential(1.0, n).astype(np.float32)
+    g = sc.guide_gradient(q, d, w, 1.0, 4.0)
+    gr = G.backward(P, q, d, w, lo, hi, emulate=True, weight_scale=1.0, weight_clip=4.0)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"[gpu]   rel err vs bf16 reference {rel:.2e}", flush=True)
+    assert rel <= 2e-2
+
+
+@pytest.mark.gpu
+def test_adam_repack_equals_prepare(gscene):
+    """Adam writes each updated parameter into the bf16 inference and backward
+    copies itself; those must equal the copies prepared from the parameters."""
+    _, sc = gscene
+    lo, hi = _bounds(sc)
+    sc.guide_set_params(G.init_params(14))
+    n = 3000
+    q = G.random_queries(n, lo, hi, 4)
+    d = G.random_dirs(n, 5)
+    w = np.random.default_rng(6).exponential(1.0, n).astype(np.float32)
+    for _ in range(3):
+        sc.guide_train_step(q, d, w)
+    y1 = sc.guide_forward(q)
+    g1 = sc.guide_gradient(q, d, w, 1.0, 5.0)
+    sc.guide_set_params(sc.guide_params())       # re-prepares both copies from the fp32 parameters
+    y2 = sc.guide_forward(q)
+    g2 = sc.guide_gradient(q, d, w, 1.0, 5.0)
+    assert np.array_equal(y1, y2)
+    assert np.array_equal(g1, g2)
+
+
+@pytest.mark.gpu
 def test_guide_train_step_is_adam(gscene):
     _, sc = gscene
     lo, hi = _bounds(sc)

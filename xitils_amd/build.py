@@ -19,6 +19,9 @@ LIB = os.path.join(OUT_DIR, "libxitils_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 
 SOURCES = ["runtime.hip", "kernels.hip", "guide.hip", "vmf.hip", "bvh.cpp"]
+# the guiding network's MFMA accumulators in ordinary VGPRs (the VALU reads
+# them for the activations; in AGPRs every element costs an extra copy)
+PER_FILE = {"guide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,8 +43,8 @@ def build(verbose: bool = False, out_lib: str = LIB, defines=()) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, "--offload-arch=gfx950", *COMMON, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-c",
-               os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, "--offload-arch=gfx950", *COMMON, *PER_FILE.get(src, []), *[f"-D{d}" for d in defines], "-I",
+               os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC, *COMMON, *[f"-D{d}" for d in defines], "-x", "c++", "-I", os.path.join(ROOT, "include"), "-c",
                    os.path.join(CSRC, src), "-o", obj]

@@ -46,13 +46,6 @@ struct GuideBack {
 };
 static_assert(sizeof(GuideBack) % 16 == 0, "GuideBack is copied in 16-byte pieces");
 
-// Per-sample activations and gradients of a training minibatch for the
-// weight-gradient products, bf16, unit-major: row u holds unit u of every
-// sample (leading dimension ld >= samples).  Rows: features x (96), h1, h2,
-// h3 (64 each), then dz1, dz2, dz3 (64 each) and dy (48; rows 40..47 zero).
-constexpr int GT_X = 0, GT_H1 = 96, GT_H2 = 160, GT_H3 = 224, GT_D1 = 288, GT_D2 = 352, GT_D3 = 416, GT_DY = 480,
-              GT_ROWS = 528;
-
 // One training sample (64 B, four float4): query (p, n, wo), sampled
 // direction, weight.
 struct GuideSample {
@@ -67,18 +60,16 @@ __global__ void k_guide_prepare(const float* P, GuideNet net);
 __global__ void k_guide_forward(GuideNet net, GuideBounds gb, const float* qin, uint32_t n, float* out);
 __global__ void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue q, const uint32_t* count,
                               const float4* hits, uint2* lobes_mk, float2* lobes2, uint32_t stride, uint32_t max_vertex);
-__global__ void k_guide_backprop(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
-                                 float weight_clip, __bf16* T, uint32_t ld);
-__global__ void k_guide_wgrad(const __bf16* T, uint32_t ld, uint32_t n, uint32_t chunk, float* partial);
+__global__ void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* samples, uint32_t n, float weight_scale,
+                             float weight_clip, float* partial);
 __global__ void k_guide_adam(float* P, float* m, float* v, const float* partial, uint32_t nparts, float lr, float b1,
-                             float b2, float eps, float b1t, float b2t, float grad_scale);
+                             float b2, float eps, float b1t, float b2t, float grad_scale, GuideNet net);
 __global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4* radiance, uint32_t target, GuideSample* out,
                                  uint32_t* count, float* weight_sum);
 
 constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
-constexpr int GB_BLOCK = 256;   // threads per block of k_guide_backprop (32 samples per wave pass); a
-                                // 32768-sample minibatch is 1024 waves, one 4-wave block per CU
-constexpr int GW_BLOCK = 256;   // threads per block of k_guide_wgrad (one k chunk per block)
+constexpr int GG_BLOCK = 256;   // threads per block of k_guide_grad; a 32768-sample minibatch is one
+constexpr int GG_BLOCK_SAMPLES = 128;   // block per CU, each writing one partial row
 constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize
 constexpr int GA_BLOCK = 1024;  // threads per block of k_guide_adam (32 row groups x 32 parameters)
 

@@ -107,48 +107,65 @@ __device__ __forceinline__ void load_chain(const GuideChain* src, GuideChain& ds
     __syncthreads();
 }
 
+// (lo, hi) rounded to bf16 (nearest even) in one packed conversion
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){lo, hi}, b2));
+}
+// the two bf16 halves of w clamped at zero as signed 16-bit integers: bf16
+// is sign-magnitude, so every negative value (-0 included) becomes +0
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t w) {
+    typedef short s2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2, w), (s2)0));
+}
+
 // This lane's 24 input features of one query (layer-0 input fragment, k
 // blocks 0..2): group g < 3 the frequency features of axis g, group 3 the
-// shading normal and view direction.  Same arithmetic as guide_encode.
+// shading normal and view direction in its first six positions.  Group 3
+// also runs the band recurrence (on z) and leaves those finite values in its
+// other 18 positions, whose layer-0 weights are zero, so only six values
+// depend on the group.  sin and cos of pi u by xt_sincosf (within 2 ulp of
+// sin and cos of the rounded argument, as a float libm gives them: the
+// recurrence doubles the error per band, so the base must be that close),
+// then the doubling recurrence sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x
+// (one fma).
 __device__ __forceinline__ void chain_features(const GuideBounds& gb, f3 p, f3 n, f3 wo, bf16x8 (&x)[3]) {
     const int g = __lane_id() >> 4;
+    const float PI = 3.14159265358979323846f;
+    const float pa = g == 0 ? p.x : (g == 1 ? p.y : p.z);
+    const float lo = g == 0 ? gb.lo.x : (g == 1 ? gb.lo.y : gb.lo.z);
+    const float inv = g == 0 ? gb.inv_ext.x : (g == 1 ? gb.inv_ext.y : gb.inv_ext.z);
+    const float ua = fminf(fmaxf((pa - lo) * inv, 0.0f), 1.0f);
+    float sn, cs;
+    xt_sincosf(PI * ua, &sn, &cs);
     float v[24];
-    if (g < 3) {
-        const float PI = 3.14159265358979323846f;
-        const float pa = g == 0 ? p.x : (g == 1 ? p.y : p.z);
-        const float lo = g == 0 ? gb.lo.x : (g == 1 ? gb.lo.y : gb.lo.z);
-        const float inv = g == 0 ? gb.inv_ext.x : (g == 1 ? gb.inv_ext.y : gb.inv_ext.z);
-        const float ua = fminf(fmaxf((pa - lo) * inv, 0.0f), 1.0f);
-        float sn = sinf(PI * ua), cs = cosf(PI * ua);
 #pragma unroll
-        for (int k = 0; k < G_BANDS; ++k) {
-            v[2 * k] = sn;
-            v[2 * k + 1] = cs;
-            const float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
-            sn = s2;
-            cs = c2;
-        }
-    } else {
+    for (int k = 0; k < G_BANDS; ++k) {
+        v[2 * k] = sn;
+        v[2 * k + 1] = cs;
+        const float d = sn + sn;
+        cs = __builtin_fmaf(-d, sn, 1.0f);
+        sn = d * v[2 * k + 1];
+    }
+    if (g == 3) {
         v[0] = n.x; v[1] = n.y; v[2] = n.z;
         v[3] = wo.x; v[4] = wo.y; v[5] = wo.z;
-#pragma unroll
-        for (int k = 6; k < 24; ++k) v[k] = 0.0f;
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) x[c][j] = (__bf16)v[8 * c + j];
+        x[c] = __builtin_bit_cast(bf16x8, make_uint4(pack_bf16(v[8 * c], v[8 * c + 1]), pack_bf16(v[8 * c + 2], v[8 * c + 3]),
+                                                     pack_bf16(v[8 * c + 4], v[8 * c + 5]),
+                                                     pack_bf16(v[8 * c + 6], v[8 * c + 7])));
 }
 
-// relu(acc) of two adjacent 16-unit output tiles -> one input fragment
+// relu(acc) of two adjacent 16-unit output tiles -> one input fragment:
+// rounded to bf16 first, then clamped (equal to rounding relu(acc)), one
+// packed conversion and one packed integer max per pair of units.
 __device__ __forceinline__ bf16x8 relu_pack(floatx4 a, floatx4 b) {
-    bf16x8 h;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        h[i] = (__bf16)fmaxf(a[i], 0.0f);
-        h[4 + i] = (__bf16)fmaxf(b[i], 0.0f);
-    }
-    return h;
+    const uint4 w = make_uint4(relu_bf16x2(pack_bf16(a[0], a[1])), relu_bf16x2(pack_bf16(a[2], a[3])),
+                               relu_bf16x2(pack_bf16(b[0], b[1])), relu_bf16x2(pack_bf16(b[2], b[3])));
+    return __builtin_bit_cast(bf16x8, w);
 }
 
 // The four layers for NT tiles of 16 queries, activations in registers.
@@ -163,13 +180,15 @@ __device__ __forceinline__ void chain_mlp(const GuideChain& s, const bf16x8 (&x)
         const floatx4 bias = *(const floatx4*)&s.b0[16 * b + 4 * g];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const bf16x8 a = frag(s.w0, GC_LD0, 16 * b, 32 * c);
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, x[t][c], acc[t][b]);
         }
-    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         h[t][0] = relu_pack(acc[t][0], acc[t][1]);
@@ -184,13 +203,15 @@ __device__ __forceinline__ void chain_mlp(const GuideChain& s, const bf16x8 (&x)
             const floatx4 bias = *(const floatx4*)&bl[layer][16 * b + 4 * g];
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[t][b] = bias;
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const bf16x8 a = frag(wl[layer], GC_LD, 16 * b, 32 * c);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) acc[t][b] = mfma(a, h[t][c], acc[t][b]);
             }
-        }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             h[t][0] = relu_pack(acc[t][0], acc[t][1]);
@@ -202,13 +223,15 @@ __device__ __forceinline__ void chain_mlp(const GuideChain& s, const bf16x8 (&x)
         const floatx4 bias = *(const floatx4*)&s.b3[16 * b + 4 * g];
 #pragma unroll
         for (int t = 0; t < NT; ++t) y[t][b] = bias;
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const bf16x8 a = frag(s.w3, GC_LD, 16 * b, 32 * c);
 #pragma unroll
             for (int t = 0; t < NT; ++t) y[t][b] = mfma(a, h[t][c], y[t][b]);
         }
-    }
 }
 
 // Slot k (0..9) of this lane's two lobes: raw output 10g + k of the query.
@@ -250,7 +273,17 @@ __global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, GuideBounds
 // shade kernel may guide (hit, not emissive, not a delta material):
 // lobes_mk[j * stride + i] = (mu, kappa) as halves, lobes2[j * stride + i] =
 // (alpha, alpha * vmf_norm(kappa)) of lobe j, kappa as stored.
-__global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue pq,
+#ifndef XT_GUIDE_INFER_NT
+#define XT_GUIDE_INFER_NT 2
+#endif
+constexpr int GI_NT = XT_GUIDE_INFER_NT;
+static_assert(GI_NT == 1 || GI_NT == 2 || GI_NT == 4, "tiles per inference pass");
+#ifndef XT_GUIDE_INFER_WAVES
+#define XT_GUIDE_INFER_WAVES 4
+#endif
+constexpr int GI_WAVES = XT_GUIDE_INFER_WAVES;   // waves per SIMD the register budget allows
+
+__global__ __launch_bounds__(256, GI_WAVES) void k_guide_infer(GuideNet net, GuideBounds gb, DevScene sc, PathQueue pq,
                                                      const uint32_t* __restrict__ count, const float4* __restrict__ hits,
                                                      uint2* __restrict__ lobes_mk, float2* __restrict__ lobes2,
                                                      uint32_t stride, uint32_t max_vertex) {
@@ -258,6 +291,10 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
     load_chain(net.chain, s);
     const uint32_t n = *count;
     const int l = __lane_id(), g = l >> 4, r = l & 15;
+    const float PI_F = 3.14159265358979323846f;
+    // this lane group's two lobes (2g, 2g + 1)
+    uint2* const lobe_mk[2] = {lobes_mk + (size_t)(2 * g) * stride, lobes_mk + (size_t)(2 * g + 1) * stride};
+    float2* const lobe2[2] = {lobes2 + (size_t)(2 * g) * stride, lobes2 + (size_t)(2 * g + 1) * stride};
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + l;
@@ -279,42 +316,55 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, GuideBounds g
             }
         }
         if (__ballot(use) == 0) continue;
-        bf16x8 x[4][3];
+        // the wave's 64 queries in passes of GI_NT tiles of 16 (fewer live
+        // registers per pass: more waves per SIMD)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int src = 16 * t + r;
-            chain_features(gb, mk(__shfl(P.x, src), __shfl(P.y, src), __shfl(P.z, src)),
-                           mk(__shfl(N.x, src), __shfl(N.y, src), __shfl(N.z, src)),
-                           mk(__shfl(WO.x, src), __shfl(WO.y, src), __shfl(WO.z, src)), x[t]);
-        }
-        floatx4 y[4][3];
-        chain_mlp<4>(s, x, y);
+        for (int pass = 0; pass < 4 / GI_NT; ++pass) {
+            bf16x8 x[GI_NT][3];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int src = 16 * t + r;
-            const bool useq = __shfl((int)use, src) != 0;
-            // softmax over the 8 alpha logits held by the 4 lanes of this query
-            const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
-            float mx = fmaxf(la, lb);
-            mx = fmaxf(mx, __shfl_xor(mx, 16));
-            mx = fmaxf(mx, __shfl_xor(mx, 32));
-            const float ea = expf(la - mx), eb = expf(lb - mx);
-            float sum = ea + eb;
-            sum += __shfl_xor(sum, 16);
-            sum += __shfl_xor(sum, 32);
-            if (useq) {
-                const uint32_t q = base + src;
+            for (int t = 0; t < GI_NT; ++t) {
+                const int src = 16 * (GI_NT * pass + t) + r;
+                chain_features(gb, mk(__shfl(P.x, src), __shfl(P.y, src), __shfl(P.z, src)),
+                               mk(__shfl(N.x, src), __shfl(N.y, src), __shfl(N.z, src)),
+                               mk(__shfl(WO.x, src), __shfl(WO.y, src), __shfl(WO.z, src)), x[t]);
+            }
+            floatx4 y[GI_NT][3];
+            chain_mlp<GI_NT>(s, x, y);
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
-                    const float len = length(m);
-                    const f3 mu = len > 1e-12f ? div(m, len) : mk(0.0f, 0.0f, 1.0f);
-                    const float kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
-                    const float alpha = (k == 0 ? ea : eb) / sum;
-                    const size_t o = (size_t)(2 * g + k) * stride + q;
-                    const uint2 mk2 = make_uint2(pack_half2(mu.x, mu.y), pack_half2(mu.z, kappa));
-                    lobes_mk[o] = mk2;
-                    lobes2[o] = make_float2(alpha, alpha * vmf_norm(half_hi(mk2.y)));
+            for (int t = 0; t < GI_NT; ++t) {
+                const int src = 16 * (GI_NT * pass + t) + r;
+                const bool useq = __shfl((int)use, src) != 0;
+                // softmax over the 8 alpha logits held by the 4 lanes of this query
+                // (hardware exp2 and reciprocals: the lobes are stored as halves
+                // and alpha as a float the shade kernel uses for sampling and pdf alike)
+                const float LOG2E = 1.44269504088896341f;
+                const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
+                float mx = fmaxf(la, lb);
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                const float ea = __builtin_amdgcn_exp2f((la - mx) * LOG2E), eb = __builtin_amdgcn_exp2f((lb - mx) * LOG2E);
+                float sum = ea + eb;
+                sum += __shfl_xor(sum, 16);
+                sum += __shfl_xor(sum, 32);
+                const float rsum = __builtin_amdgcn_rcpf(sum);
+                if (useq) {
+                    const uint32_t q = base + src;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
+                        const float l2 = dot(m, m);
+                        const f3 mu = l2 > 1e-24f ? m * __builtin_amdgcn_rsqf(l2) : mk(0.0f, 0.0f, 1.0f);
+                        const float kl = fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX);
+                        const float kappa = __builtin_amdgcn_exp2f(kl * LOG2E);
+                        const float alpha = (k == 0 ? ea : eb) * rsum;
+                        const uint2 mk2 = make_uint2(pack_half2(mu.x, mu.y), pack_half2(mu.z, kappa));
+                        // alpha k / (2 pi (1 - e^{-2k})) with k the stored half (>= e^-3, no cancellation)
+                        const float kh = half_hi(mk2.y);
+                        const float norm =
+                            kh * __builtin_amdgcn_rcpf(2.0f * PI_F * (1.0f - __builtin_amdgcn_exp2f(-2.0f * LOG2E * kh)));
+                        lobe_mk[k][q] = mk2;
+                        lobe2[k][q] = make_float2(alpha, alpha * norm);
+                    }
                 }
             }
         }
@@ -421,75 +471,170 @@ __device__ __forceinline__ void back_step(const __bf16* w, const bf16x8 (&z)[NT]
     }
 }
 
-// Per-wave staging of T rows in LDS: [row][32 samples] bf16 with an 80-byte
-// row stride (16-byte aligned rows; the four lane groups of a store land 16
-// banks apart).  Lanes store single values at immediate offsets, then the
-// wave copies whole 64-byte rows to T with 16-byte stores.
-constexpr int GS_STRIDE = 40;                 // bf16 per staged row
-constexpr int GS_ROWS = 96;                   // rows per flush
-struct GradStage {
-    __bf16 v[GS_ROWS * GS_STRIDE];
+// Gradient of the loss -min(w weight_scale, weight_clip) log q(dir | query)
+// with respect to the raw outputs y of NT tiles of 16 samples: lane group g
+// holds lobes 2g, 2g + 1 (slots 0..9) of sample 16t + r; wgt is the clipped
+// weight (0 for padding samples, whose gradient is then 0).
+template <int NT>
+__device__ __forceinline__ void loss_grad(const floatx4 (&y)[NT][3], const f3 (&dir)[NT], const float (&wgt)[NT],
+                                          floatx4 (&dy)[NT][3]) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
+        float mx = fmaxf(la, lb);
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        const float ea = expf(la - mx), eb = expf(lb - mx);
+        float esum = ea + eb;
+        esum += __shfl_xor(esum, 16);
+        esum += __shfl_xor(esum, 32);
+        VmfLobe lob[2];
+        float len[2], v[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
+            len[k] = length(m);
+            lob[k].mu = len[k] > 1e-12f ? div(m, len[k]) : mk(0.0f, 0.0f, 1.0f);
+            lob[k].kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
+            lob[k].alpha = (k == 0 ? ea : eb) / esum;
+            v[k] = vmf_lobe_pdf(lob[k], dir[t]);
+        }
+        float q = lob[0].alpha * v[0] + lob[1].alpha * v[1];
+        q += __shfl_xor(q, 16);
+        q += __shfl_xor(q, 32);
+        q = fmaxf(q, 1e-30f);
+        float d[12];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float rsp = lob[k].alpha * v[k] / q;
+            const float kap = lob[k].kappa;
+            const f3 mu = lob[k].mu;
+            const float mw = dot(mu, dir[t]);
+            const f3 gm = (dir[t] - mw * mu) * (rsp * kap / fmaxf(len[k], 1e-12f));
+            const float dk = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (mw - 1.0f);
+            const float kl = out_slot(y[t], 5 * k + 3);
+            const float gk = (kl > G_KLOG_MIN && kl < G_KLOG_MAX) ? rsp * dk * kap : 0.0f;
+            const float ga = rsp - lob[k].alpha;
+            d[5 * k] = -wgt[t] * gm.x;
+            d[5 * k + 1] = -wgt[t] * gm.y;
+            d[5 * k + 2] = -wgt[t] * gm.z;
+            d[5 * k + 3] = -wgt[t] * gk;
+            d[5 * k + 4] = -wgt[t] * ga;
+        }
+        d[10] = d[11] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dy[t][b][i] = d[4 * b + i];
+    }
+}
+
+// ---------------------------------------------------------------- training: one fused gradient kernel
+
+// k_guide_grad: a block of GG_BLOCK threads takes GG_SAMPLES samples (each
+// wave 16 GG_NT of them), runs their forward and backward chains with the
+// activations in registers, and forms the weight and bias gradients of the
+// block's samples layer by layer as MFMA products with samples as K: the
+// pair a layer's gradient needs (its pre-activation gradient and its input
+// activations) is staged in LDS unit-major, [unit][sample] bf16, the block
+// synchronises, and each wave multiplies one 16-row strip.  Nothing per
+// sample leaves the chip; each block writes one partial row (G_NPARAMS
+// floats), summed by k_guide_adam in a fixed order.
+constexpr int GG_WAVES = GG_BLOCK / 64;
+constexpr int GG_NT = 2;                            // tiles of 16 samples per wave
+constexpr int GG_SAMPLES = GG_WAVES * 16 * GG_NT;
+static_assert(GG_SAMPLES == GG_BLOCK_SAMPLES, "samples per block as the host sizes the grid");
+constexpr int GG_LD = GG_SAMPLES + 8;               // bf16 per staged row: 272-byte rows, 16-byte aligned
+constexpr int GG_ROWS = G_HID + G_IN;               // the largest pair: dz1 and the features
+struct GradTile {
+    __bf16 v[GG_ROWS * GG_LD];
 };
-// this lane's units of NB output tiles: row 16b + 4g + i, sample column col (0..31)
+static_assert(GG_NT == 2, "a lane stages its two tiles' values of a unit as one 32-bit word");
+
+// Column of this lane's sample 16t + r of wave w: the two tiles side by side
+// (the column order is the same for every row, so the products over samples
+// do not depend on it).
+__device__ __forceinline__ int grad_col(int w, int r) { return w * 16 * GG_NT + GG_NT * r; }
+
+__device__ __forceinline__ void put_pair(GradTile& S, int row, int col, uint32_t pair) {
+    *(uint32_t*)&S.v[row * GG_LD + col] = pair;
+}
+__device__ __forceinline__ uint32_t bf16_pair(__bf16 a, __bf16 b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+// output tiles (units 16b + 4g + i) of both tiles -> rows row0 + unit
 template <int NB>
-__device__ __forceinline__ void stage_tiles(GradStage& st, int col, const floatx4 (&v)[NB]) {
+__device__ __forceinline__ void stage_units(GradTile& S, int row0, int col, const floatx4 (&v)[GG_NT][NB]) {
     const int g = __lane_id() >> 4;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) st.v[(16 * b + 4 * g + i) * GS_STRIDE + col] = (__bf16)v[b][i];
+        for (int i = 0; i < 4; ++i) put_pair(S, row0 + 16 * b + 4 * g + i, col, pack_bf16(v[0][b][i], v[1][b][i]));
 }
-// one input fragment set: k position 32c + 8g + j holds unit unit_of_kpos(...)
-__device__ __forceinline__ void stage_frags(GradStage& st, int col, const bf16x8 (&h)[2]) {
+// input fragments (k position 32c + 8g + j = unit unit_of_kpos) -> rows row0 + unit
+__device__ __forceinline__ void stage_frags(GradTile& S, int row0, int col, const bf16x8 (&h)[GG_NT][2]) {
     const int g = __lane_id() >> 4;
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) st.v[unit_of_kpos(32 * c + 8 * g + j) * GS_STRIDE + col] = h[c][j];
-}
-// rows [0, R) of the stage -> T[row0 + row][base .. base + 31]
-template <int R>
-__device__ __forceinline__ void flush_stage(GradStage& st, __bf16* T, uint32_t ld, int row0, uint32_t base) {
-    wave_sync();
-    const int l = __lane_id();
-#pragma unroll
-    for (int k = l; k < 4 * R; k += 64) {
-        const int row = k >> 2, part = k & 3;
-        const uint4 v = *(const uint4*)&st.v[row * GS_STRIDE + 8 * part];
-        *(uint4*)&T[(size_t)(row0 + row) * ld + base + 8 * part] = v;
-    }
-    wave_sync();
+        for (int j = 0; j < 8; ++j)
+            put_pair(S, row0 + unit_of_kpos(32 * c + 8 * g + j), col, bf16_pair(h[0][c][j], h[1][c][j]));
 }
 
-// Per-sample part of the gradient of sum_i -min(w_i * weight_scale,
-// weight_clip) log q(dir_i | query_i): forward and backward chains with
-// activations in registers, 32 samples per wave pass; writes the features,
-// hidden activations and pre-activation gradients of every sample to T (see
-// GT_*) for the weight-gradient products (k_guide_wgrad).  Samples >= n are
-// written as zeros up to the next multiple of 32.
-__global__ __launch_bounds__(GB_BLOCK) void k_guide_backprop(GuideNet net, GuideBounds gb,
-                                                             const GuideSample* __restrict__ smp, uint32_t n,
-                                                             float weight_scale, float weight_clip, __bf16* __restrict__ T,
-                                                             uint32_t ld) {
-    constexpr int NT = 2;
+// One 16-row strip of a layer's gradient over the block's samples: dW[m][n] =
+// sum_k D[drow + m][k] A[arow + n][k], db[m] = sum_k D[drow + m][k], added to
+// (first pass: written to) the block's partial row.
+template <int IN>
+__device__ __forceinline__ void grad_strip(const GradTile& S, int drow, int arow, float* row, int off_w, int off_b,
+                                           bool first) {
+    const int l = __lane_id(), g = l >> 4, r = l & 15;
+    floatx4 acc[IN / 16];
+#pragma unroll
+    for (int c = 0; c < IN / 16; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < GG_SAMPLES; k += 32) {
+        const bf16x8 a = frag(S.v, GG_LD, drow, k);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bsum += (float)a[j];
+#pragma unroll
+        for (int c = 0; c < IN / 16; ++c) acc[c] = mfma(a, frag(S.v, GG_LD, arow + 16 * c, k), acc[c]);
+    }
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+#pragma unroll
+    for (int c = 0; c < IN / 16; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float* d = &row[off_w + (4 * g + i) * IN + 16 * c + r];
+            *d = first ? acc[c][i] : *d + acc[c][i];
+        }
+    if (g == 0) row[off_b + r] = first ? bsum : row[off_b + r] + bsum;
+}
+
+// Gradient of sum_i -min(w_i * weight_scale, weight_clip) log q(dir_i |
+// query_i) over samples [0, n): block x writes partial row x, taking sample
+// groups x, x + gridDim.x, ... (every block takes at least one; samples >= n
+// contribute zero).
+__global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBounds gb, const GuideSample* __restrict__ smp,
+                                                         uint32_t n, float weight_scale, float weight_clip,
+                                                         float* __restrict__ partial) {
+    constexpr int NT = GG_NT;
     __shared__ __attribute__((aligned(16))) GuideChain s;
     __shared__ __attribute__((aligned(16))) GuideBack bk;
-    __shared__ __attribute__((aligned(16))) GradStage stage[GB_BLOCK / 64];
+    __shared__ __attribute__((aligned(16))) GradTile S;
     {
         const uint4* src = (const uint4*)net.back;
         uint4* dst = (uint4*)&bk;
         for (int k = threadIdx.x; k < (int)(sizeof(GuideBack) / 16); k += blockDim.x) dst[k] = src[k];
     }
     load_chain(net.chain, s);   // includes the block barrier
-    GradStage& st = stage[threadIdx.x >> 6];
-    const int l = __lane_id(), g = l >> 4, r = l & 15;
-    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
-    const uint32_t npad = (n + 31u) & ~31u;
-    for (uint32_t base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 32u; base < npad;
-         base += nwaves * 32u) {
-        // compiler-only fence: the weight fragments are re-read from LDS in each
-        // pass instead of being hoisted into registers for the whole loop
-        asm volatile("" ::: "memory");
+    const int w = threadIdx.x >> 6, l = __lane_id(), g = l >> 4, r = l & 15;
+    const int col = grad_col(w, r);
+    float* row = partial + (size_t)blockIdx.x * G_NPARAMS;
+    bool first = true;
+    for (uint32_t blk = blockIdx.x * GG_SAMPLES; first || blk < n; blk += gridDim.x * GG_SAMPLES) {
+        const uint32_t base = blk + w * 16 * NT;
         bf16x8 x[NT][3];
         f3 dir[NT];
         float wgt[NT];
@@ -513,84 +658,24 @@ __global__ __launch_bounds__(GB_BLOCK) void k_guide_backprop(GuideNet net, Guide
             if (!valid)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) x[t][c] = bf16x8{};
-            // features: k position 32c + 8g + j holds feature 24g + 8c + j (g < 3) or 72 + 8c + j (g = 3)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int f = g < 3 ? 24 * g + 8 * c + j : 72 + 8 * c + j;
-                    st.v[f * GS_STRIDE + 16 * t + r] = x[t][c][j];
-                }
         }
-        flush_stage<G_IN>(st, T, ld, GT_X, base);
         bf16x8 h1[NT][2], h2[NT][2], h3[NT][2];
         floatx4 y[NT][3];
         chain_forward_keep<NT>(s, x, h1, h2, h3, y);
-        // loss gradient: lane group g holds lobes 2g, 2g + 1 (slots 0..9) of sample 16t + r
         floatx4 dy[NT][3];
+        loss_grad<NT>(y, dir, wgt, dy);
+
+        // layer 3: dW3 = dy^T h3 (dy rows: raw output index, 40..47 zero)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
-            float mx = fmaxf(la, lb);
-            mx = fmaxf(mx, __shfl_xor(mx, 16));
-            mx = fmaxf(mx, __shfl_xor(mx, 32));
-            const float ea = expf(la - mx), eb = expf(lb - mx);
-            float esum = ea + eb;
-            esum += __shfl_xor(esum, 16);
-            esum += __shfl_xor(esum, 32);
-            VmfLobe lob[2];
-            float len[2], v[2];
+        for (int k = 0; k < 10; ++k) put_pair(S, 10 * g + k, col, pack_bf16(out_slot(dy[0], k), out_slot(dy[1], k)));
+        if (g == 0)
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
-                len[k] = length(m);
-                lob[k].mu = len[k] > 1e-12f ? div(m, len[k]) : mk(0.0f, 0.0f, 1.0f);
-                lob[k].kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
-                lob[k].alpha = (k == 0 ? ea : eb) / esum;
-                v[k] = vmf_lobe_pdf(lob[k], dir[t]);
-            }
-            float q = lob[0].alpha * v[0] + lob[1].alpha * v[1];
-            q += __shfl_xor(q, 16);
-            q += __shfl_xor(q, 32);
-            q = fmaxf(q, 1e-30f);
-            float d[12];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const float rsp = lob[k].alpha * v[k] / q;
-                const float kap = lob[k].kappa;
-                const f3 mu = lob[k].mu;
-                const float mw = dot(mu, dir[t]);
-                const f3 gm = (dir[t] - mw * mu) * (rsp * kap / fmaxf(len[k], 1e-12f));
-                const float dk = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (mw - 1.0f);
-                const float kl = out_slot(y[t], 5 * k + 3);
-                const float gk = (kl > G_KLOG_MIN && kl < G_KLOG_MAX) ? rsp * dk * kap : 0.0f;
-                const float ga = rsp - lob[k].alpha;
-                d[5 * k] = -wgt[t] * gm.x;
-                d[5 * k + 1] = -wgt[t] * gm.y;
-                d[5 * k + 2] = -wgt[t] * gm.z;
-                d[5 * k + 3] = -wgt[t] * gk;
-                d[5 * k + 4] = -wgt[t] * ga;
-            }
-            d[10] = d[11] = 0.0f;
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) dy[t][b][i] = d[4 * b + i];
-        }
-        // activations, then dy (rows GT_DY + raw output index, rounded as the backward chain reads it)
-        const bf16x8 (*hs[3])[2] = {h1, h2, h3};
-#pragma unroll
-        for (int layer = 0; layer < 3; ++layer) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) stage_frags(st, 16 * t + r, hs[layer][t]);
-            flush_stage<G_HID>(st, T, ld, GT_H1 + G_HID * layer, base);
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int k = 0; k < 10; ++k) st.v[(10 * g + k) * GS_STRIDE + 16 * t + r] = (__bf16)out_slot(dy[t], k);
-        flush_stage<G_OUT>(st, T, ld, GT_DY, base);
-        // backward chain: dz3 = (W3^T dy) * [h3 > 0], dz2 = (W2^T dz3) * [h2 > 0], dz1 = (W1^T dz2) * [h1 > 0]
+            for (int k = G_OUT; k < G_OUT_PAD; ++k) put_pair(S, k, col, 0u);
+        stage_frags(S, G_OUT_PAD, col, h3);
+        __syncthreads();
+        if (w < 3) grad_strip<G_HID>(S, 16 * w, G_OUT_PAD, row, G_OFF_W3 + 16 * w * G_HID, G_OFF_B3 + 16 * w, first);
+        __syncthreads();
+        // dz3 = (W3^T dy) [h3 > 0]; dW2 = dz3^T h2
         bf16x8 z[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -599,92 +684,78 @@ __global__ __launch_bounds__(GB_BLOCK) void k_guide_backprop(GuideNet net, Guide
         }
         floatx4 dz[NT][4];
         back_step<NT>(bk.w3, z, h3, dz);
+        stage_units<4>(S, 0, col, dz);
+        stage_frags(S, G_HID, col, h2);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            stage_tiles<4>(st, 16 * t + r, dz[t]);
             z[t][0] = grad_pack(dz[t][0], dz[t][1]);
             z[t][1] = grad_pack(dz[t][2], dz[t][3]);
         }
-        flush_stage<G_HID>(st, T, ld, GT_D3, base);
+        __syncthreads();
+        grad_strip<G_HID>(S, 16 * w, G_HID, row, G_OFF_W2 + 16 * w * G_HID, G_OFF_B2 + 16 * w, first);
+        __syncthreads();
+        // dz2 = (W2^T dz3) [h2 > 0]; dW1 = dz2^T h1
         back_step<NT>(bk.w2, z, h2, dz);
+        stage_units<4>(S, 0, col, dz);
+        stage_frags(S, G_HID, col, h1);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            stage_tiles<4>(st, 16 * t + r, dz[t]);
             z[t][0] = grad_pack(dz[t][0], dz[t][1]);
             z[t][1] = grad_pack(dz[t][2], dz[t][3]);
         }
-        flush_stage<G_HID>(st, T, ld, GT_D2, base);
+        __syncthreads();
+        grad_strip<G_HID>(S, 16 * w, G_HID, row, G_OFF_W1 + 16 * w * G_HID, G_OFF_B1 + 16 * w, first);
+        __syncthreads();
+        // dz1 = (W1^T dz2) [h1 > 0]; dW0 = dz1^T x (features in natural order;
+        // group 3's positions beyond its six direction features are 0)
         back_step<NT>(bk.w1, z, h1, dz);
+        stage_units<4>(S, 0, col, dz);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) stage_tiles<4>(st, 16 * t + r, dz[t]);
-        flush_stage<G_HID>(st, T, ld, GT_D1, base);
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int f = g < 3 ? 24 * g + 8 * c + j : 72 + 8 * c + j;
+                const bool live = 8 * c + j < 6 || g < 3;
+                put_pair(S, G_HID + f, col, live ? bf16_pair(x[0][c][j], x[1][c][j]) : 0u);
+            }
+        __syncthreads();
+        grad_strip<G_IN>(S, 16 * w, G_HID, row, G_OFF_W0 + 16 * w * G_IN, G_OFF_B0 + 16 * w, first);
+        __syncthreads();
+        first = false;
     }
 }
 
-// ---------------------------------------------------------------- training: weight-gradient products
+// ---------------------------------------------------------------- training: Adam
 
-// One 16-row strip of a layer's gradient over samples [k0, k1): dW[m][n] =
-// sum_k D[m][k] A[n][k] (D = dz rows, A = activation rows of T), db[m] =
-// sum_k D[m][k]; written to row[...] of this chunk.
-template <int IN>
-__device__ __forceinline__ void wgrad_strip(const __bf16* T, uint32_t ld, int drow, int arow, uint32_t k0, uint32_t k1,
-                                            float* row, int off_w, int off_b, int rows_valid) {
-    const int l = __lane_id(), g = l >> 4, r = l & 15;
-    floatx4 acc[IN / 16];
-#pragma unroll
-    for (int c = 0; c < IN / 16; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    float bsum = 0.0f;
-    for (uint32_t k = k0; k < k1; k += 32) {
-        const bf16x8 a = frag(T + (size_t)drow * ld, (int)ld, 0, (int)k);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bsum += (float)a[j];
-#pragma unroll
-        for (int c = 0; c < IN / 16; ++c) acc[c] = mfma(a, frag(T + (size_t)(arow + 16 * c) * ld, (int)ld, 0, (int)k), acc[c]);
-    }
-    bsum += __shfl_xor(bsum, 16);
-    bsum += __shfl_xor(bsum, 32);
-#pragma unroll
-    for (int c = 0; c < IN / 16; ++c)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int m = 4 * g + i;
-            if (m < rows_valid) row[off_w + m * IN + 16 * c + r] = acc[c][i];
-        }
-    if (g == 0 && r < rows_valid) row[off_b + r] = bsum;
+// Inverses of the GuideChain / GuideBack layout maps (k_guide_prepare
+// gathers with the maps, k_guide_adam scatters with these).
+__device__ __forceinline__ int kpos_of_unit(int v) {
+    const int c = v >> 5, u = v & 31;
+    return 32 * c + (u < 16 ? 8 * (u >> 2) + (u & 3) : 8 * ((u - 16) >> 2) + 4 + ((u - 16) & 3));
 }
-
-// Weight and bias gradients of one chunk of `chunk` samples (block x) from
-// T: 15 strips of 16 output rows (4 per hidden layer, 3 for the output
-// layer) spread over the block's 4 waves; one partial row per block, summed
-// by k_guide_adam.
-__global__ __launch_bounds__(GW_BLOCK) void k_guide_wgrad(const __bf16* __restrict__ T, uint32_t ld, uint32_t n,
-                                                          uint32_t chunk, float* __restrict__ partial) {
-    const uint32_t npad = (n + 31u) & ~31u;
-    const uint32_t k0 = blockIdx.x * chunk, k1 = min(npad, k0 + chunk);
-    float* row = partial + (size_t)blockIdx.x * G_NPARAMS;
-    const int wave = threadIdx.x >> 6;
-    for (int strip = wave; strip < 15; strip += GW_BLOCK / 64) {
-        const int layer = strip >> 2, sb = strip & 3, m0 = 16 * sb;
-        if (layer == 0)
-            wgrad_strip<G_IN>(T, ld, GT_D1 + m0, GT_X, k0, k1, row, G_OFF_W0 + m0 * G_IN, G_OFF_B0 + m0, 16);
-        else if (layer == 1)
-            wgrad_strip<G_HID>(T, ld, GT_D2 + m0, GT_H1, k0, k1, row, G_OFF_W1 + m0 * G_HID, G_OFF_B1 + m0, 16);
-        else if (layer == 2)
-            wgrad_strip<G_HID>(T, ld, GT_D3 + m0, GT_H2, k0, k1, row, G_OFF_W2 + m0 * G_HID, G_OFF_B2 + m0, 16);
-        else
-            wgrad_strip<G_HID>(T, ld, GT_DY + m0, GT_H3, k0, k1, row, G_OFF_W3 + m0 * G_HID, G_OFF_B3 + m0, 16);
+__device__ __forceinline__ int kpos_of_feature(int f) {   // -1: not an input of the chain
+    if (f < 72) {
+        const int g = f / 24, s = f % 24;
+        return 32 * (s >> 3) + 8 * g + (s & 7);
     }
+    return f < 78 ? 24 + (f - 72) : -1;
+}
+__device__ __forceinline__ int row_of_output(int o) {       // o < G_OUT
+    const int g = o / 10, s = o % 10;
+    return 16 * (s >> 2) + 4 * g + (s & 3);
 }
 
 // Adam (Kingma & Ba) on the fp32 master parameters: the gradient is the sum
 // of nparts partial rows (GA_GROUPS interleaved row groups per parameter,
 // then the groups in order: deterministic) times grad_scale; b1t, b2t =
 // beta^t.  32 parameters per block, one row group per wave half, so a
-// 256-row sum is 8 loads per thread with 16 waves per block in flight.
+// 256-row sum is 8 loads per thread with 16 waves per block in flight.  The
+// updated parameter is also written, rounded, to its places in the bf16
+// inference and backward copies (whose padding k_guide_prepare zeroed).
 __global__ __launch_bounds__(GA_BLOCK) void k_guide_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
                                                          const float* __restrict__ partial, uint32_t nparts, float lr,
                                                          float b1, float b2, float eps, float b1t, float b2t,
-                                                         float grad_scale) {
+                                                         float grad_scale, GuideNet net) {
     constexpr int GA_GROUPS = GA_BLOCK / 32;
     __shared__ float red[GA_GROUPS][32];
     const int pl = threadIdx.x & 31, rg = threadIdx.x >> 5;
@@ -704,7 +775,40 @@ __global__ __launch_bounds__(GA_BLOCK) void k_guide_adam(float* __restrict__ P, 
     m[i] = mi;
     v[i] = vi;
     const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
-    P[i] -= lr * mh / (sqrtf(vh) + eps);
+    const float pn = P[i] - lr * mh / (sqrtf(vh) + eps);
+    P[i] = pn;
+    GuideChain& ch = *net.chain;
+    GuideBack& bk = *net.back;
+    const __bf16 hb = (__bf16)pn;
+    auto hidden = [&](__bf16* cw, __bf16* bw, int k) {   // W1 or W2 [o][in], k = o * G_HID + in
+        const int o = k / G_HID, in = k % G_HID;
+        cw[o * GC_LD + kpos_of_unit(in)] = hb;
+        bw[in * GC_LD + kpos_of_unit(o)] = hb;
+    };
+    if (i < G_OFF_B0) {
+        const int o = i / G_IN, f = i % G_IN, p = kpos_of_feature(f);
+        if (p >= 0) ch.w0[o * GC_LD0 + p] = hb;
+    } else if (i < G_OFF_W1) {
+        ch.b0[i - G_OFF_B0] = pn;
+    } else if (i < G_OFF_B1) {
+        hidden(ch.w1, bk.w1, i - G_OFF_W1);
+    } else if (i < G_OFF_W2) {
+        ch.b1[i - G_OFF_B1] = pn;
+    } else if (i < G_OFF_B2) {
+        hidden(ch.w2, bk.w2, i - G_OFF_W2);
+    } else if (i < G_OFF_W3) {
+        ch.b2[i - G_OFF_B2] = pn;
+    } else if (i < G_OFF_B3) {
+        const int k = i - G_OFF_W3, o = k / G_HID, in = k % G_HID;
+        if (o < G_OUT) {
+            const int rw = row_of_output(o);
+            ch.w3[rw * GC_LD + kpos_of_unit(in)] = hb;
+            bk.w3[in * GC_LD + kpos_of_unit(rw)] = hb;
+        }
+    } else {
+        const int o = i - G_OFF_B3;
+        if (o < G_OUT) ch.b3[row_of_output(o)] = pn;
+    }
 }
 
 // Training records -> samples.  C = (L_path - pre) / W estimates

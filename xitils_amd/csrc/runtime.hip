@@ -70,10 +70,8 @@ struct GuideState {
     float* P = nullptr;              // fp32 master parameters
     float* m = nullptr;              // Adam moments
     float* v = nullptr;
-    float* partial = nullptr;        // per-chunk gradient rows of k_guide_wgrad
+    float* partial = nullptr;        // per-block gradient rows of k_guide_grad
     uint32_t partial_rows = 0;
-    __bf16* T = nullptr;             // per-sample activations / gradients of a minibatch (GT_ROWS x t_ld)
-    uint32_t t_ld = 0;
     void* net_mem = nullptr;         // GuideChain + GuideBack
     GuideNet net{};
     uint64_t step = 0;               // Adam steps taken
@@ -84,7 +82,7 @@ struct GuideState {
     float* d_wsum = nullptr;         // their weight sum
     float* host_io = nullptr;        // pinned: count + weight sum read-back
     void release() {
-        hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem); hipFree(T);
+        hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem);
         hipFree(recs); hipFree(samples); hipFree(d_count); hipFree(d_wsum);
         if (host_io) hipHostFree(host_io);
         *this = GuideState();
@@ -880,30 +878,17 @@ static xt_status guide_need(xt_scene* s) {
 }
 
 // Gradient of the minibatch d_samples[0..n) into g.partial rows [0, *rows):
-// the per-sample chains write T, then one weight-gradient product per chunk
-// of samples (at most partial_rows chunks).
+// one fused kernel, a block per 128 samples (at most partial_rows blocks,
+// each taking every rows-th sample group).
 static xt_status guide_gradient_rows(xt_scene* s, const GuideSample* d_samples, uint32_t n, float weight_scale,
                                      float weight_clip, uint32_t* rows) {
     GuideState& g = s->guide;
-    const uint32_t npad = (n + 31u) & ~31u;
-    if (npad > g.t_ld || !g.T) {
-        hipFree(g.T);
-        g.T = nullptr;
-        g.t_ld = std::max<uint32_t>((npad + 255u) & ~255u, 256u);
-        XT_HIP(dalloc(&g.T, (size_t)GT_ROWS * g.t_ld));
-        XT_HIP(hipMemsetAsync(g.T, 0, (size_t)GT_ROWS * g.t_ld * sizeof(__bf16), s->stream));   // dy rows 40..47 stay 0
-    }
-    const uint32_t waves = std::max(1u, npad / 32u);
-    const uint32_t bp_blocks = std::max(1u, std::min((waves + GB_BLOCK / 64 - 1) / (GB_BLOCK / 64),
-                                                     (uint32_t)s->dev->num_cu));
-    hipLaunchKernelGGL(k_guide_backprop, dim3(bp_blocks), dim3(GB_BLOCK), 0, s->stream, g.net, g.gb, d_samples, n,
-                       weight_scale, weight_clip, g.T, g.t_ld);
-    // 128-sample chunks: a 32768-sample minibatch fills one block per CU
-    uint32_t chunk = std::max(128u, (((npad + g.partial_rows - 1) / g.partial_rows) + 31u) & ~31u);
-    const uint32_t nchunks = std::max(1u, (npad + chunk - 1) / chunk);
-    hipLaunchKernelGGL(k_guide_wgrad, dim3(nchunks), dim3(GW_BLOCK), 0, s->stream, g.T, g.t_ld, n, chunk, g.partial);
+    const uint32_t groups = (n + GG_BLOCK_SAMPLES - 1) / GG_BLOCK_SAMPLES;
+    const uint32_t blocks = std::max(1u, std::min(groups, g.partial_rows));
+    hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, d_samples, n, weight_scale,
+                       weight_clip, g.partial);
     XT_HIP(hipGetLastError());
-    *rows = nchunks;
+    *rows = blocks;
     return XT_OK;
 }
 
@@ -923,11 +908,10 @@ static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t
         g.step++;
         const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
         const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
+        // Adam also re-packs the bf16 copies the next step and the renders read
         hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(GA_BLOCK), 0, s->stream, g.P, g.m, g.v, g.partial,
                            grid, g.cfg.learning_rate, g.cfg.beta1, g.cfg.beta2, g.cfg.adam_eps, b1t, b2t,
-                           1.0f / (float)cnt);
-        xt_status st = guide_prepare(s);
-        if (st != XT_OK) return st;
+                           1.0f / (float)cnt, g.net);
         if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "guide training launch failed");
     }
     if (steps_taken) *steps_taken += steps;

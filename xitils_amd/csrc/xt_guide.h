@@ -53,13 +53,14 @@ XT_HD void guide_encode(const GuideBounds& gb, f3 p, f3 n, f3 wo, T* x) {
     float u[3] = {(p.x - gb.lo.x) * gb.inv_ext.x, (p.y - gb.lo.y) * gb.inv_ext.y, (p.z - gb.lo.z) * gb.inv_ext.z};
     for (int a = 0; a < 3; ++a) {
         float ua = fminf(fmaxf(u[a], 0.0f), 1.0f);
-        float s = sinf(PI * ua), c = cosf(PI * ua);
+        float s, c;
+        xt_sincosf(PI * ua, &s, &c);
         for (int k = 0; k < G_BANDS; ++k) {
             x[a * 2 * G_BANDS + 2 * k] = (T)s;
             x[a * 2 * G_BANDS + 2 * k + 1] = (T)c;
-            float s2 = 2.0f * s * c;          // sin 2x
-            float c2 = c * c - s * s;         // cos 2x
-            s = s2;
+            const float d = s + s;
+            const float c2 = fmaf(-d, s, 1.0f);   // cos 2x = 1 - 2 sin^2 x, one rounding
+            s = d * c;                            // sin 2x
             c = c2;
         }
     }
