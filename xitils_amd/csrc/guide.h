@@ -68,8 +68,13 @@ __global__ void k_guide_finalize(const TrainRec* recs, uint32_t n, const float4*
                                  uint32_t* count, float* weight_sum);
 
 constexpr int GI_BLOCK = 256;   // threads per block of k_guide_forward / k_guide_infer (64 queries per wave)
-constexpr int GG_BLOCK = 256;   // threads per block of k_guide_grad; a 32768-sample minibatch is one
-constexpr int GG_BLOCK_SAMPLES = 128;   // block per CU, each writing one partial row
+#ifndef XT_GUIDE_GRAD_BLOCK
+#define XT_GUIDE_GRAD_BLOCK 512
+#endif
+constexpr int GG_BLOCK = XT_GUIDE_GRAD_BLOCK;   // threads per block of k_guide_grad: 512 (16 samples per
+                                                // wave, 2 waves per SIMD) 23.1 us per 32768-sample step,
+                                                // 256 (32 per wave, 1 wave per SIMD) 27.0 us
+constexpr int GG_BLOCK_SAMPLES = 128;   // a 32768-sample minibatch is one block per CU, each writing one partial row
 constexpr int GF_BLOCK = 1024;  // threads per block of k_guide_finalize
 constexpr int GA_BLOCK = 1024;  // threads per block of k_guide_adam (32 row groups x 32 parameters)
 

@@ -478,43 +478,49 @@ __device__ __forceinline__ void back_step(const __bf16* w, const bf16x8 (&z)[NT]
 template <int NT>
 __device__ __forceinline__ void loss_grad(const floatx4 (&y)[NT][3], const f3 (&dir)[NT], const float (&wgt)[NT],
                                           floatx4 (&dy)[NT][3]) {
+    // hardware exp2 / reciprocal / square root (a few ulp, far below the bf16
+    // rounding the gradient then takes); expm1 stays exact, 1/k - 2/expm1(2k)
+    // cancels for small k
+    const float LOG2E = 1.44269504088896341f, TWO_PI = 6.28318530717958648f;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const float la = out_slot(y[t], 4), lb = out_slot(y[t], 9);
         float mx = fmaxf(la, lb);
         mx = fmaxf(mx, __shfl_xor(mx, 16));
         mx = fmaxf(mx, __shfl_xor(mx, 32));
-        const float ea = expf(la - mx), eb = expf(lb - mx);
+        const float ea = __builtin_amdgcn_exp2f((la - mx) * LOG2E), eb = __builtin_amdgcn_exp2f((lb - mx) * LOG2E);
         float esum = ea + eb;
         esum += __shfl_xor(esum, 16);
         esum += __shfl_xor(esum, 32);
-        VmfLobe lob[2];
-        float len[2], v[2];
+        const float resum = __builtin_amdgcn_rcpf(esum);
+        f3 mu[2];
+        float alpha[2], kap[2], rlen[2], v[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const f3 m = mk(out_slot(y[t], 5 * k), out_slot(y[t], 5 * k + 1), out_slot(y[t], 5 * k + 2));
-            len[k] = length(m);
-            lob[k].mu = len[k] > 1e-12f ? div(m, len[k]) : mk(0.0f, 0.0f, 1.0f);
-            lob[k].kappa = expf(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX));
-            lob[k].alpha = (k == 0 ? ea : eb) / esum;
-            v[k] = vmf_lobe_pdf(lob[k], dir[t]);
+            const float len = __builtin_sqrtf(dot(m, m));
+            rlen[k] = __builtin_amdgcn_rcpf(fmaxf(len, 1e-12f));
+            mu[k] = len > 1e-12f ? m * rlen[k] : mk(0.0f, 0.0f, 1.0f);
+            kap[k] = __builtin_amdgcn_exp2f(fminf(fmaxf(out_slot(y[t], 5 * k + 3), G_KLOG_MIN), G_KLOG_MAX) * LOG2E);
+            alpha[k] = (k == 0 ? ea : eb) * resum;
+            // vMF pdf: k / (2 pi (1 - e^{-2k})) e^{k (mu.dir - 1)}
+            const float norm = kap[k] * __builtin_amdgcn_rcpf(TWO_PI * (1.0f - __builtin_amdgcn_exp2f(-2.0f * LOG2E * kap[k])));
+            v[k] = norm * __builtin_amdgcn_exp2f(kap[k] * (dot(dir[t], mu[k]) - 1.0f) * LOG2E);
         }
-        float q = lob[0].alpha * v[0] + lob[1].alpha * v[1];
+        float q = alpha[0] * v[0] + alpha[1] * v[1];
         q += __shfl_xor(q, 16);
         q += __shfl_xor(q, 32);
-        q = fmaxf(q, 1e-30f);
+        const float rq = __builtin_amdgcn_rcpf(fmaxf(q, 1e-30f));
         float d[12];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const float rsp = lob[k].alpha * v[k] / q;
-            const float kap = lob[k].kappa;
-            const f3 mu = lob[k].mu;
-            const float mw = dot(mu, dir[t]);
-            const f3 gm = (dir[t] - mw * mu) * (rsp * kap / fmaxf(len[k], 1e-12f));
-            const float dk = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (mw - 1.0f);
+            const float rsp = alpha[k] * v[k] * rq;
+            const float mw = dot(mu[k], dir[t]);
+            const f3 gm = (dir[t] - mw * mu[k]) * (rsp * kap[k] * rlen[k]);
+            const float dk = __builtin_amdgcn_rcpf(kap[k]) - 2.0f * __builtin_amdgcn_rcpf(expm1f(2.0f * kap[k])) + (mw - 1.0f);
             const float kl = out_slot(y[t], 5 * k + 3);
-            const float gk = (kl > G_KLOG_MIN && kl < G_KLOG_MAX) ? rsp * dk * kap : 0.0f;
-            const float ga = rsp - lob[k].alpha;
+            const float gk = (kl > G_KLOG_MIN && kl < G_KLOG_MAX) ? rsp * dk * kap[k] : 0.0f;
+            const float ga = rsp - alpha[k];
             d[5 * k] = -wgt[t] * gm.x;
             d[5 * k + 1] = -wgt[t] * gm.y;
             d[5 * k + 2] = -wgt[t] * gm.z;
@@ -537,39 +543,49 @@ __device__ __forceinline__ void loss_grad(const floatx4 (&y)[NT][3], const f3 (&
 // block's samples layer by layer as MFMA products with samples as K: the
 // pair a layer's gradient needs (its pre-activation gradient and its input
 // activations) is staged in LDS unit-major, [unit][sample] bf16, the block
-// synchronises, and each wave multiplies one 16-row strip.  Nothing per
-// sample leaves the chip; each block writes one partial row (G_NPARAMS
-// floats), summed by k_guide_adam in a fixed order.
+// synchronises, and each wave multiplies its share of the 16 x 16 output
+// tiles.  Nothing per sample leaves the chip; each block writes one partial
+// row (G_NPARAMS floats), summed by k_guide_adam in a fixed order.
 constexpr int GG_WAVES = GG_BLOCK / 64;
-constexpr int GG_NT = 2;                            // tiles of 16 samples per wave
-constexpr int GG_SAMPLES = GG_WAVES * 16 * GG_NT;
-static_assert(GG_SAMPLES == GG_BLOCK_SAMPLES, "samples per block as the host sizes the grid");
+constexpr int GG_NT = GG_BLOCK_SAMPLES / (16 * GG_WAVES);   // tiles of 16 samples per wave
+constexpr int GG_SAMPLES = GG_BLOCK_SAMPLES;
+static_assert(GG_NT == 1 || GG_NT == 2, "one or two tiles of 16 samples per wave");
+static_assert(GG_WAVES * 16 * GG_NT == GG_SAMPLES, "the block's samples");
 constexpr int GG_LD = GG_SAMPLES + 8;               // bf16 per staged row: 272-byte rows, 16-byte aligned
 constexpr int GG_ROWS = G_HID + G_IN;               // the largest pair: dz1 and the features
 struct GradTile {
     __bf16 v[GG_ROWS * GG_LD];
 };
-static_assert(GG_NT == 2, "a lane stages its two tiles' values of a unit as one 32-bit word");
 
-// Column of this lane's sample 16t + r of wave w: the two tiles side by side
-// (the column order is the same for every row, so the products over samples
-// do not depend on it).
+// Column of this lane's sample 16t + r of wave w (with two tiles, the lane's
+// two samples side by side: one 32-bit store per unit).  The column order is
+// the same for every row, so the products over samples do not depend on it.
 __device__ __forceinline__ int grad_col(int w, int r) { return w * 16 * GG_NT + GG_NT * r; }
 
-__device__ __forceinline__ void put_pair(GradTile& S, int row, int col, uint32_t pair) {
-    *(uint32_t*)&S.v[row * GG_LD + col] = pair;
+// this lane's values of one unit, one per tile, into row `row`
+__device__ __forceinline__ void put_unit(GradTile& S, int row, int col, const float (&v)[GG_NT]) {
+    if constexpr (GG_NT == 2) *(uint32_t*)&S.v[row * GG_LD + col] = pack_bf16(v[0], v[1]);
+    else S.v[row * GG_LD + col] = (__bf16)v[0];
 }
-__device__ __forceinline__ uint32_t bf16_pair(__bf16 a, __bf16 b) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+__device__ __forceinline__ void put_unit_bf16(GradTile& S, int row, int col, __bf16 a, __bf16 b) {
+    if constexpr (GG_NT == 2)
+        *(uint32_t*)&S.v[row * GG_LD + col] =
+            (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    else S.v[row * GG_LD + col] = a;
 }
-// output tiles (units 16b + 4g + i) of both tiles -> rows row0 + unit
+// output tiles (units 16b + 4g + i) -> rows row0 + unit
 template <int NB>
 __device__ __forceinline__ void stage_units(GradTile& S, int row0, int col, const floatx4 (&v)[GG_NT][NB]) {
     const int g = __lane_id() >> 4;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) put_pair(S, row0 + 16 * b + 4 * g + i, col, pack_bf16(v[0][b][i], v[1][b][i]));
+        for (int i = 0; i < 4; ++i) {
+            float u[GG_NT];
+#pragma unroll
+            for (int t = 0; t < GG_NT; ++t) u[t] = v[t][b][i];
+            put_unit(S, row0 + 16 * b + 4 * g + i, col, u);
+        }
 }
 // input fragments (k position 32c + 8g + j = unit unit_of_kpos) -> rows row0 + unit
 __device__ __forceinline__ void stage_frags(GradTile& S, int row0, int col, const bf16x8 (&h)[GG_NT][2]) {
@@ -578,39 +594,53 @@ __device__ __forceinline__ void stage_frags(GradTile& S, int row0, int col, cons
     for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            put_pair(S, row0 + unit_of_kpos(32 * c + 8 * g + j), col, bf16_pair(h[0][c][j], h[1][c][j]));
+            put_unit_bf16(S, row0 + unit_of_kpos(32 * c + 8 * g + j), col, h[0][c][j], h[GG_NT - 1][c][j]);
 }
 
-// One 16-row strip of a layer's gradient over the block's samples: dW[m][n] =
-// sum_k D[drow + m][k] A[arow + n][k], db[m] = sum_k D[drow + m][k], added to
-// (first pass: written to) the block's partial row.
-template <int IN>
-__device__ __forceinline__ void grad_strip(const GradTile& S, int drow, int arow, float* row, int off_w, int off_b,
-                                           bool first) {
+// This wave's share of one layer's gradient over the block's samples: the
+// layer's (rows / 16) x (IN / 16) output tiles in parts of TP tiles of one
+// 16-row strip, part w for wave w: dW[m][n] = sum_k D[drow + m][k] A[arow +
+// n][k], and for the strip's first part db[m] = sum_k D[drow + m][k]; added
+// to (first pass: written to) the block's partial row.
+template <int IN, int TP>
+__device__ __forceinline__ void grad_part(const GradTile& S, int strips, int drow, int arow, float* row, int off_w,
+                                          int off_b, bool first) {
+    constexpr int PPS = IN / 16 / TP;   // parts per strip
+    static_assert(PPS * TP * 16 == IN, "parts tile the strip");
+    const int w = threadIdx.x >> 6;
+    if (w >= strips * PPS) return;
+    const int sb = w / PPS, c0 = (w % PPS) * TP;
     const int l = __lane_id(), g = l >> 4, r = l & 15;
-    floatx4 acc[IN / 16];
+    floatx4 acc[TP];
 #pragma unroll
-    for (int c = 0; c < IN / 16; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < TP; ++c) acc[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;
 #pragma unroll
     for (int k = 0; k < GG_SAMPLES; k += 32) {
-        const bf16x8 a = frag(S.v, GG_LD, drow, k);
+        const bf16x8 a = frag(S.v, GG_LD, drow + 16 * sb, k);
 #pragma unroll
         for (int j = 0; j < 8; ++j) bsum += (float)a[j];
 #pragma unroll
-        for (int c = 0; c < IN / 16; ++c) acc[c] = mfma(a, frag(S.v, GG_LD, arow + 16 * c, k), acc[c]);
+        for (int c = 0; c < TP; ++c) acc[c] = mfma(a, frag(S.v, GG_LD, arow + 16 * (c0 + c), k), acc[c]);
     }
     bsum += __shfl_xor(bsum, 16);
     bsum += __shfl_xor(bsum, 32);
 #pragma unroll
-    for (int c = 0; c < IN / 16; ++c)
+    for (int c = 0; c < TP; ++c)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            float* d = &row[off_w + (4 * g + i) * IN + 16 * c + r];
+            float* d = &row[off_w + (16 * sb + 4 * g + i) * IN + 16 * (c0 + c) + r];
             *d = first ? acc[c][i] : *d + acc[c][i];
         }
-    if (g == 0) row[off_b + r] = first ? bsum : row[off_b + r] + bsum;
+    if (c0 == 0 && g == 0) {
+        float* d = &row[off_b + 16 * sb + r];
+        *d = first ? bsum : *d + bsum;
+    }
 }
+// tiles per part: a whole strip per wave with 4 waves; with 8 waves, half a
+// strip (2 of 4 tiles, 3 of 6)
+constexpr int GG_TP_HID = GG_WAVES >= 8 ? 2 : 4;
+constexpr int GG_TP_IN = GG_WAVES >= 8 ? 3 : 6;
 
 // Gradient of sum_i -min(w_i * weight_scale, weight_clip) log q(dir_i |
 // query_i) over samples [0, n): block x writes partial row x, taking sample
@@ -667,13 +697,20 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
 
         // layer 3: dW3 = dy^T h3 (dy rows: raw output index, 40..47 zero)
 #pragma unroll
-        for (int k = 0; k < 10; ++k) put_pair(S, 10 * g + k, col, pack_bf16(out_slot(dy[0], k), out_slot(dy[1], k)));
-        if (g == 0)
+        for (int k = 0; k < 10; ++k) {
+            float u[NT];
 #pragma unroll
-            for (int k = G_OUT; k < G_OUT_PAD; ++k) put_pair(S, k, col, 0u);
+            for (int t = 0; t < NT; ++t) u[t] = out_slot(dy[t], k);
+            put_unit(S, 10 * g + k, col, u);
+        }
+        if (g == 0) {
+            const float zero[NT] = {};
+#pragma unroll
+            for (int k = G_OUT; k < G_OUT_PAD; ++k) put_unit(S, k, col, zero);
+        }
         stage_frags(S, G_OUT_PAD, col, h3);
         __syncthreads();
-        if (w < 3) grad_strip<G_HID>(S, 16 * w, G_OUT_PAD, row, G_OFF_W3 + 16 * w * G_HID, G_OFF_B3 + 16 * w, first);
+        grad_part<G_HID, GG_TP_HID>(S, 3, 0, G_OUT_PAD, row, G_OFF_W3, G_OFF_B3, first);
         __syncthreads();
         // dz3 = (W3^T dy) [h3 > 0]; dW2 = dz3^T h2
         bf16x8 z[NT][2];
@@ -692,7 +729,7 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
             z[t][1] = grad_pack(dz[t][2], dz[t][3]);
         }
         __syncthreads();
-        grad_strip<G_HID>(S, 16 * w, G_HID, row, G_OFF_W2 + 16 * w * G_HID, G_OFF_B2 + 16 * w, first);
+        grad_part<G_HID, GG_TP_HID>(S, 4, 0, G_HID, row, G_OFF_W2, G_OFF_B2, first);
         __syncthreads();
         // dz2 = (W2^T dz3) [h2 > 0]; dW1 = dz2^T h1
         back_step<NT>(bk.w2, z, h2, dz);
@@ -704,7 +741,7 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
             z[t][1] = grad_pack(dz[t][2], dz[t][3]);
         }
         __syncthreads();
-        grad_strip<G_HID>(S, 16 * w, G_HID, row, G_OFF_W1 + 16 * w * G_HID, G_OFF_B1 + 16 * w, first);
+        grad_part<G_HID, GG_TP_HID>(S, 4, 0, G_HID, row, G_OFF_W1, G_OFF_B1, first);
         __syncthreads();
         // dz1 = (W1^T dz2) [h1 > 0]; dW0 = dz1^T x (features in natural order;
         // group 3's positions beyond its six direction features are 0)
@@ -716,10 +753,11 @@ __global__ __launch_bounds__(GG_BLOCK) void k_guide_grad(GuideNet net, GuideBoun
             for (int j = 0; j < 8; ++j) {
                 const int f = g < 3 ? 24 * g + 8 * c + j : 72 + 8 * c + j;
                 const bool live = 8 * c + j < 6 || g < 3;
-                put_pair(S, G_HID + f, col, live ? bf16_pair(x[0][c][j], x[1][c][j]) : 0u);
+                const __bf16 z0 = (__bf16)0.0f;
+                put_unit_bf16(S, G_HID + f, col, live ? x[0][c][j] : z0, live ? x[NT - 1][c][j] : z0);
             }
         __syncthreads();
-        grad_strip<G_IN>(S, 16 * w, G_HID, row, G_OFF_W0 + 16 * w * G_IN, G_OFF_B0 + 16 * w, first);
+        grad_part<G_IN, GG_TP_IN>(S, 4, 0, G_HID, row, G_OFF_W0, G_OFF_B0, first);
         __syncthreads();
         first = false;
     }
