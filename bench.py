@@ -565,8 +565,24 @@ def nmap_main(args):
         "build": {"scene_gen_s": round(gen_s, 1), "bvh_build_upload_s": round(upload_s, 1),
                   "bvh_nodes": info["bvh_nodes"], "bvh_depth": info["bvh_depth"]},
     }
+    ref = None
+    if not args.no_ref or args.guided:
+        # converged reference of variant 0 at 960x540 (at 3840x2160 the 16,384-spp
+        # FP64 reference would be 136 G paths)
+        from xitils_amd import quality
+        t0 = time.perf_counter()
+        ref = quality.reference_image(scene, NMAP_QW, NMAP_QH, spp=args.ref_spp, chunk=512, seed=0x5EED, camera=cams[0],
+                                      max_depth=depth, flags=flags)
+        ref_s = time.perf_counter() - t0
+        log(f"[bench] reference of variant 0 at {NMAP_QW}x{NMAP_QH}, {args.ref_spp} spp, FP64: {ref_s:.1f} s")
+        if not args.no_ref:
+            img = scene.render(NMAP_QW, NMAP_QH, NMAP_QSPP, camera=cams[0], seed=1, max_depth=depth, flags=flags) / NMAP_QSPP
+            out["quality"] = {"relmse": quality.rel_mse(img, ref), "image_spp": NMAP_QSPP, "reference_spp": args.ref_spp,
+                              "resolution": f"{NMAP_QW}x{NMAP_QH}", "camera": "variant 0",
+                              "reference": "same renderer, FP64 film, seed 0x5EED",
+                              "reference_seconds": round(ref_s, 2)}
     if args.guided:
-        out.update(nmap_guided(args, scene, cams, W, H, spp, depth, flags, paths / wall / 1e6))
+        out.update(nmap_guided(args, scene, cams, W, H, spp, depth, flags, paths / wall / 1e6, ref))
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         cdesc = desc
@@ -577,12 +593,15 @@ def nmap_main(args):
     print(json.dumps(out), flush=True)
 
 
-def nmap_guided(args, scene, cams, W, H, spp, depth, flags, plain_mpaths):
+NMAP_QW, NMAP_QH, NMAP_QSPP = 960, 540, 64   # the displaced plane's relMSE images
+
+
+def nmap_guided(args, scene, cams, W, H, spp, depth, flags, plain_mpaths, ref):
     """The displaced plane with the guiding network: online training (one
     1-spp training render per camera variant), then timed steps that render
     every variant guided in one xt_render_batch call, then plain and guided
-    at equal spp on variant 0 against a reference (960x540, 4096 spp: the
-    16,384-spp 4K oracle would be 136 G paths per variant)."""
+    at equal spp on variant 0 against the converged reference `ref`
+    (NMAP_QW x NMAP_QH, args.ref_spp spp, FP64)."""
     import numpy as np
     import torch
     from xitils_amd import api, quality
@@ -619,9 +638,7 @@ def nmap_guided(args, scene, cams, W, H, spp, depth, flags, plain_mpaths):
     guided_mpaths = W * H * spp * nv * args.steps / wall / 1e6
     finite = all(bool(torch.isfinite(f).all()) for f in films)
     # quality on variant 0 at reduced resolution, equal spp
-    qw, qh, qspp = 960, 540, 64
-    ref = quality.reference_image(scene, qw, qh, spp=4096, chunk=512, seed=0x5EED, camera=cams[0], max_depth=depth,
-                                  flags=flags)
+    qw, qh, qspp = NMAP_QW, NMAP_QH, NMAP_QSPP
     res = {}
     for name, f in (("plain", flags), ("guided", gflags)):
         img = torch.zeros((qh, qw, 3), dtype=torch.float32, device="cuda:0")
@@ -638,7 +655,7 @@ def nmap_guided(args, scene, cams, W, H, spp, depth, flags, plain_mpaths):
             "mpaths_per_s": round(guided_mpaths, 2), "plain_mpaths_per_s": round(plain_mpaths, 2),
             "calls": "one xt_render_batch call of %d variants per step" % nv, "finite": finite,
             "training": {"renders": nv, "adam_steps": steps, "samples": samples, "seconds": round(train_s, 2)},
-            "quality_variant0": {"resolution": f"{qw}x{qh}", "spp": qspp, "reference_spp": 4096,
+            "quality_variant0": {"resolution": f"{qw}x{qh}", "spp": qspp, "reference_spp": args.ref_spp,
                                  "plain": {k: round(v, 5) for k, v in res["plain"].items()},
                                  "guided": {k: round(v, 5) for k, v in res["guided"].items()},
                                  "efficiency_ratio": round(eff, 3),
