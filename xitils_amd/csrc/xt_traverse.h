@@ -394,12 +394,38 @@ __device__ __forceinline__ bool trav_pop(Trav& t, const uint32_t* lds, int lds_s
     return true;
 }
 
-// One traversal step (one node or one triangle); returns true when the ray
-// is finished (for ANY, *occ tells whether a blocker was found).
+// One traversal step; returns true when the ray is finished (for ANY, *occ
+// tells whether a blocker was found).  A step visits the lane's node, and if
+// that leaves the lane at a leaf (the nearest hit child is one, or the
+// popped entry is), tests the leaf's next triangle in the same step; a lane
+// that starts at a leaf tests one triangle.  Every lane of a wave then meets
+// the triangle code in the same iteration as the lanes that were already in
+// leaves: the iterations a ray takes drop from nodes + triangles toward the
+// node count, and each branch runs with more lanes active.
+#ifndef XT_TRAV_FUSED
+#define XT_TRAV_FUSED 1
+#endif
 template <bool ANY, int LDS_DEPTH, bool COUNT, class SV>
 __device__ __forceinline__ bool trav_step(const SV& sc, const RayPre& r, Trav& t, const float4* top,
                                           uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
                                           VisitCount& vc, bool* occ) {
+#if XT_TRAV_FUSED
+    if (t.node >= 0) {
+        if (!visit_node<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, lds, lds_stride, ovf, ovf_stride, vc) &&
+            !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride))
+            return true;
+    }
+    if (t.node < 0) {
+        const int rest = visit_tri<ANY, COUNT>(sc, r, t, t.node, vc, occ);
+        if (ANY && *occ) return true;
+        if (rest) {
+            t.node = rest;
+            return false;
+        }
+        return !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride);
+    }
+    return false;
+#else
     if (t.node >= 0) {
         if (visit_node<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, lds, lds_stride, ovf, ovf_stride, vc)) return false;
     } else {
@@ -413,6 +439,7 @@ __device__ __forceinline__ bool trav_step(const SV& sc, const RayPre& r, Trav& t
         }
     }
     return !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride);
+#endif
 }
 
 }  // namespace xt
