@@ -135,13 +135,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
             next = min(next + nidle, end);
         }
         if (__ballot(active) == 0) break;
+        bool occ = false;
+        bool done;
+        if constexpr (MULTI)
+            done = trav_step<ANY, LDS_DEPTH, COUNT>(active, view, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
+        else
+            done = trav_step<ANY, LDS_DEPTH, COUNT>(active, sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
         if (active) {
-            bool occ = false;
-            bool done;
-            if constexpr (MULTI)
-                done = trav_step<ANY, LDS_DEPTH, COUNT>(view, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
-            else
-                done = trav_step<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ);
             if (done) {
                 if (ANY) {
                     occluded[rid] = occ ? 1 : 0;
@@ -695,9 +695,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
             next = min(next + nidle, end);
         }
         if (__ballot(active) == 0) break;
+        bool occ = false;
+        const bool done = trav_step<true, TRACE_LDS_DEPTH, COUNT>(active, sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf,
+                                                                   nthreads, vc, &occ);
         if (active) {
-            bool occ = false;
-            if (trav_step<true, TRACE_LDS_DEPTH, COUNT>(sc, r, t, top, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc, &occ)) {
+            if (done) {
                 // unoccluded contributions go straight to the path radiance, or to
                 // pending[] when the path continues (see k_shade)
                 float4 c4 = sq.c[rid];
@@ -974,10 +976,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, co
             next = min(next + nidle, end);
         }
         if (__ballot(active) == 0) break;
+        bool occ = false;
+        const bool done = trav_step<true, TRACE_LDS_DEPTH, false>(active, view, r, t, nullptr, my_lds, TRACE_BLOCK,
+                                                                   my_ovf, nthreads, vc, &occ);
         if (active) {
-            bool occ = false;
-            if (trav_step<true, TRACE_LDS_DEPTH, false>(view, r, t, nullptr, my_lds, TRACE_BLOCK, my_ovf, nthreads, vc,
-                                                        &occ)) {
+            if (done) {
                 const float4 c4 = sq.c[rid];
                 const f3 c = occ ? splat(0.0f) : xyz(c4);
                 if (c4.w != 0.0f) {

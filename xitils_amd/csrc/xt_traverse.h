@@ -394,38 +394,37 @@ __device__ __forceinline__ bool trav_pop(Trav& t, const uint32_t* lds, int lds_s
     return true;
 }
 
-// One traversal step; returns true when the ray is finished (for ANY, *occ
-// tells whether a blocker was found).  A step visits the lane's node, and if
-// that leaves the lane at a leaf (the nearest hit child is one, or the
-// popped entry is), tests the leaf's next triangle in the same step; a lane
-// that starts at a leaf tests one triangle.  Every lane of a wave then meets
-// the triangle code in the same iteration as the lanes that were already in
-// leaves: the iterations a ray takes drop from nodes + triangles toward the
-// node count, and each branch runs with more lanes active.
+// One traversal step of a lane (`active`: the lane holds a ray; the call
+// sites make it for every lane); returns true when the ray is finished (for ANY,
+// *occ tells whether a blocker was found).  A step visits the lane's node,
+// and if that leaves the lane at a leaf (the nearest hit child is one, or
+// the popped entry is), tests the leaf's next triangle in the same step, so
+// the lanes that reach leaves share the triangle code with those already in
+// leaves.  The visits and their order per ray do not change (identical hits
+// and fetch counts), only how they are grouped into wave iterations.
 #ifndef XT_TRAV_FUSED
 #define XT_TRAV_FUSED 1
 #endif
 template <bool ANY, int LDS_DEPTH, bool COUNT, class SV>
-__device__ __forceinline__ bool trav_step(const SV& sc, const RayPre& r, Trav& t, const float4* top,
+__device__ __forceinline__ bool trav_step(bool active, const SV& sc, const RayPre& r, Trav& t, const float4* top,
                                           uint32_t* lds, int lds_stride, uint32_t* ovf, size_t ovf_stride,
                                           VisitCount& vc, bool* occ) {
 #if XT_TRAV_FUSED
-    if (t.node >= 0) {
+    bool done = false;
+    if (active && t.node >= 0) {
         if (!visit_node<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, lds, lds_stride, ovf, ovf_stride, vc) &&
             !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride))
-            return true;
+            done = true;
     }
-    if (t.node < 0) {
+    if (active && !done && t.node < 0) {
         const int rest = visit_tri<ANY, COUNT>(sc, r, t, t.node, vc, occ);
         if (ANY && *occ) return true;
-        if (rest) {
-            t.node = rest;
-            return false;
-        }
-        return !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride);
+        if (rest) t.node = rest;
+        else done = !trav_pop<LDS_DEPTH>(t, lds, lds_stride, ovf, ovf_stride);
     }
-    return false;
+    return done;
 #else
+    if (!active) return false;
     if (t.node >= 0) {
         if (visit_node<ANY, LDS_DEPTH, COUNT>(sc, r, t, top, lds, lds_stride, ovf, ovf_stride, vc)) return false;
     } else {
