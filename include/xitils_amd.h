@@ -257,8 +257,9 @@ typedef struct xt_batch_item {
  * captured graph that traces and shades every path in its own item's scene.
  * Other items of at most 4 Mi paths run their bounce loop as a per-scene HIP
  * graph with no host round trip, concurrently on their scenes' streams;
- * larger items then render one after another through xt_render's bounce loop
- * while those run.  Items that share a film accumulate into it in item order
+ * larger items render through xt_render's bounce loop while those run, items
+ * of different scenes concurrently from a few host threads (a scene's items
+ * in item order; XITILS_AMD_BATCH_THREADS, default 3).  Items that share a film accumulate into it in item order
  * (all of them through the wavefront when none is guided, else each through
  * xt_render in turn).  Needs a bounce cap (1 <= max_depth <= 1024) and no
  * XT_FLAG_GUIDE_TRAIN.  Graphs are cached per scene, camera, parameters and

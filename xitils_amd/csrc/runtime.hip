@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bvh.h"
@@ -1669,6 +1671,14 @@ static void unified_finish(xt_batch_item* items, UnifiedRun& run) {
     }
 }
 
+// Host threads that render a batch's lagged-loop items concurrently
+// (XITILS_AMD_BATCH_THREADS, default 3; 1 renders them one after another).
+static uint32_t batch_loop_threads() {
+    const char* e = getenv("XITILS_AMD_BATCH_THREADS");
+    const int v = e ? std::atoi(e) : 3;
+    return (uint32_t)std::max(1, std::min(v, 16));
+}
+
 // Items with more paths than one graph replay holds render through
 // xt_render's lagged bounce loop: per item on the 64-room batch it is 15-60%
 // faster than replaying the fixed max_depth + 1 rounds per 4 Mi paths, while
@@ -1797,12 +1807,48 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
         flush();
     }
     // items larger than one graph replay: the lagged bounce loop of xt_render,
-    // one after another, while the graphs and the wavefront run
-    for (uint32_t i = 0; i < n && result == XT_OK && !in_order; ++i) {
-        xt_batch_item& it = items[i];
-        if (uni[i] || it.params.spp == 0 || !batch_item_uses_loop(it)) continue;
-        result = xt_render(it.scene, &it.camera, &it.params, d_films[i], it.film_is_f64, 1);
-        if (result == XT_OK) it.ms = (float)it.scene->stats.ms_total;
+    // while the graphs and the wavefront run.  Items on different scenes
+    // render concurrently from a few host threads (a scene's items in item
+    // order, on one thread): a render's last bounces keep few rays in flight,
+    // and another scene's render fills the GPU meanwhile.
+    if (result == XT_OK && !in_order) {
+        std::vector<std::vector<uint32_t>> groups;   // loop items per scene, in item order
+        for (uint32_t i = 0; i < n; ++i) {
+            const xt_batch_item& it = items[i];
+            if (uni[i] || it.params.spp == 0 || !batch_item_uses_loop(it)) continue;
+            size_t gi = 0;
+            while (gi < groups.size() && items[groups[gi][0]].scene != it.scene) ++gi;
+            if (gi == groups.size()) groups.emplace_back();
+            groups[gi].push_back(i);
+        }
+        const uint32_t nthreads = std::min<uint32_t>(batch_loop_threads(), (uint32_t)groups.size());
+        std::vector<xt_status> tst(std::max(1u, nthreads), XT_OK);
+        std::vector<std::string> terr(tst.size());
+        std::atomic<size_t> next_group{0};
+        auto work = [&](uint32_t t) {
+            for (size_t gi = next_group++; gi < groups.size(); gi = next_group++)
+                for (uint32_t i : groups[gi]) {
+                    xt_batch_item& it = items[i];
+                    const xt_status st = xt_render(it.scene, &it.camera, &it.params, d_films[i], it.film_is_f64, 1);
+                    if (st != XT_OK) {
+                        tst[t] = st;
+                        terr[t] = xt_last_error();
+                        next_group = groups.size();   // the other threads stop after their current item
+                        return;
+                    }
+                    it.ms = (float)it.scene->stats.ms_total;
+                }
+        };
+        if (nthreads > 1) {
+            std::vector<std::thread> pool;
+            for (uint32_t t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+            work(0);
+            for (std::thread& th : pool) th.join();
+        } else if (!groups.empty()) {
+            work(0);
+        }
+        for (size_t t = 0; t < tst.size() && result == XT_OK; ++t)
+            if (tst[t] != XT_OK) result = fail(tst[t], terr[t]);
     }
     for (uint32_t i = 0; i < n; ++i) {
         hipError_t e = hipStreamSynchronize(items[i].scene->stream);
