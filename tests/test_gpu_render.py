@@ -96,6 +96,22 @@ def test_furnace_exact_on_gpu(dev):
         assert np.all(is_a | is_1) and is_a.mean() > 0.1
 
 
+def test_render_longer_than_one_batch(dev):
+    # 1024^2 x 80 spp = 84 M paths: more than one batch of the default path
+    # capacity (64 Mi), so the render runs two batches; the same samples in
+    # five 16-spp calls (one batch each) give the same film up to the float
+    # summation order of the per-pixel sums
+    desc = scenes.cornell(1024, 1024)
+    g = api.Scene(dev, desc)
+    print("[gpu] 1024^2 x 80 spp in one call", flush=True)
+    a = g.render(1024, 1024, 80, seed=6, max_depth=4)
+    b = np.zeros_like(a)
+    for k in range(5):
+        g.render(1024, 1024, 16, film=b, seed=6, max_depth=4, spp_offset=16 * k)
+    assert np.isfinite(a).all()
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-4)
+
+
 def test_many_batches_large_image(dev):
     # more paths than one batch holds: the batching over samples/pixels must
     # not change the result

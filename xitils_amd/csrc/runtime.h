@@ -8,13 +8,18 @@
 
 #include "../../include/xitils_amd.h"
 
+#include <mutex>
+
 struct MultiState;   // unified-batch buffers (runtime.hip)
+struct RenderPool;   // xt_render's path buffers, shared by the device's scenes (runtime.hip)
 
 struct xt_device {
     int ordinal = 0;
     int num_cu = 0;
     std::string arch;
     MultiState* multi = nullptr;        // created by the first unified xt_render_batch
+    RenderPool* pool = nullptr;         // created by the first xt_render
+    std::mutex pool_mu;                 // guards creating `pool`
     unsigned char* scratch = nullptr;   // this device's address of the static scratch array (xt_render_static)
 };
 

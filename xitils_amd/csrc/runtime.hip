@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -211,7 +212,12 @@ xt_status xt_device_open(int ordinal, xt_device** out) {
 static void multi_release(xt_device* dev);
 
 xt_status xt_device_close(xt_device* dev) {
-    if (dev) multi_release(dev);
+    if (dev) {
+        multi_release(dev);
+        hipSetDevice(dev->ordinal);
+        hipDeviceSynchronize();
+        delete dev->pool;
+    }
     delete dev;
     return XT_OK;
 }
@@ -761,11 +767,8 @@ static void drop_graphs(xt_scene* s) {
     s->graphs.clear();
 }
 
-static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
-    RenderBuffers& rb = s->rb;
-    if (rb.capacity >= capacity) return XT_OK;
-    drop_graphs(s);   // graphs hold the old buffer addresses
-    hipStreamSynchronize(s->stream);
+// (Re)allocates rb for `capacity` paths; its previous contents must be idle.
+static xt_status alloc_buffers(RenderBuffers& rb, uint32_t capacity) {
     rb.release();
     size_t c = capacity;
     for (int k = 0; k < 2; ++k) {
@@ -787,6 +790,81 @@ static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
     XT_HIP(hipHostGetDevicePointer((void**)&rb.host_counts_dev, rb.host_counts, 0));
     for (hipEvent_t& e : rb.round_ev) XT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rb.capacity = capacity;
+    return XT_OK;
+}
+
+// The scene's own buffers (its captured graphs use them).
+static xt_status ensure_buffers(xt_scene* s, uint32_t capacity) {
+    if (s->rb.capacity >= capacity) return XT_OK;
+    drop_graphs(s);   // graphs hold the old buffer addresses
+    hipStreamSynchronize(s->stream);
+    return alloc_buffers(s->rb, capacity);
+}
+
+// Path buffers of xt_render's bounce loop, shared by the device's scenes: a
+// render takes a free set (grown to the render's capacity if smaller) and
+// gives it back when it ends, so memory follows the renders in flight (a
+// batch's concurrent renders), not the number of scenes.  xt_render is
+// synchronous, so a returned set is idle.
+struct RenderPool {
+    std::mutex mu;
+    std::vector<RenderBuffers*> sets;
+    std::vector<char> busy;
+    ~RenderPool() {
+        for (RenderBuffers* b : sets) {
+            b->release();
+            delete b;
+        }
+    }
+};
+struct PooledBuffers {
+    RenderPool* pool = nullptr;
+    size_t idx = 0;
+    RenderBuffers* rb = nullptr;
+    PooledBuffers() = default;
+    PooledBuffers(const PooledBuffers&) = delete;
+    PooledBuffers& operator=(const PooledBuffers&) = delete;
+    ~PooledBuffers() {
+        if (rb) {
+            std::lock_guard<std::mutex> lock(pool->mu);
+            pool->busy[idx] = 0;
+        }
+    }
+};
+static xt_status acquire_buffers(xt_device* dev, uint32_t capacity, PooledBuffers& out) {
+    RenderPool* pool;
+    {
+        std::lock_guard<std::mutex> lock(dev->pool_mu);
+        if (!dev->pool) dev->pool = new (std::nothrow) RenderPool();
+        pool = dev->pool;
+    }
+    if (!pool) return fail(XT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    {
+        std::lock_guard<std::mutex> lock(pool->mu);
+        // the smallest free set that holds the render, else the largest free set, else a new one
+        size_t pick = pool->sets.size();
+        for (size_t i = 0; i < pool->sets.size(); ++i) {
+            if (pool->busy[i]) continue;
+            if (pick == pool->sets.size()) {
+                pick = i;
+                continue;
+            }
+            const uint32_t ci = pool->sets[i]->capacity, cp = pool->sets[pick]->capacity;
+            const bool fits_i = ci >= capacity, fits_p = cp >= capacity;
+            if ((fits_i && (!fits_p || ci < cp)) || (!fits_i && !fits_p && ci > cp)) pick = i;
+        }
+        if (pick == pool->sets.size()) {
+            RenderBuffers* b = new (std::nothrow) RenderBuffers();
+            if (!b) return fail(XT_ERR_OUT_OF_MEMORY, "host allocation failed");
+            pool->sets.push_back(b);
+            pool->busy.push_back(0);
+        }
+        pool->busy[pick] = 1;
+        out.pool = pool;
+        out.idx = pick;
+        out.rb = pool->sets[pick];
+    }
+    if (out.rb->capacity < capacity) return alloc_buffers(*out.rb, capacity);
     return XT_OK;
 }
 
@@ -825,13 +903,16 @@ struct StageTimer {
     }
 };
 
-// Paths in flight per batch (224 B each, 3.6 GB at the default 16 Mi);
-// XITILS_AMD_PATHS overrides.
+// Paths in flight per batch of xt_render (224 B each, 15 GB at the default
+// 64 Mi); XITILS_AMD_PATHS overrides.  Larger batches spend less of a
+// render in its last, sparse bounce rounds: on the 1080p heightfield at 64
+// spp, 1349 Mpaths/s at 16 Mi paths (8 spp per batch), 1468 at 32 Mi, 1543
+// at 64 Mi.
 static uint32_t path_capacity() {
     static uint32_t cap = [] {
         const char* e = getenv("XITILS_AMD_PATHS");
         long v = e ? atol(e) : 0;
-        return v >= 65536 && v <= (1l << 26) ? (uint32_t)v : (1u << 24);
+        return v >= 65536 && v <= (1l << 26) ? (uint32_t)v : (1u << 26);
     }();
     return cap;
 }
@@ -972,13 +1053,11 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     s->stats = xt_render_stats{};
     if (p->spp == 0) return XT_OK;
     const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix_all * p->spp);
-    xt_status st = ensure_buffers(s, cap);
+    PooledBuffers pooled;
+    xt_status st = acquire_buffers(s->dev, cap, pooled);
     if (st != XT_OK) return st;
-    RenderBuffers& rbg = s->rb;
-    if (guide && !rbg.lobes) {
-        drop_graphs(s);
-        XT_HIP(dalloc(&rbg.lobes, (size_t)G_LOBE_FIELDS * rbg.capacity));
-    }
+    RenderBuffers& rbg = *pooled.rb;
+    if (guide && !rbg.lobes) XT_HIP(dalloc(&rbg.lobes, (size_t)G_LOBE_FIELDS * rbg.capacity));
     if (train) {
         if (!rbg.rec[0]) {
             XT_HIP(dalloc(&rbg.rec[0], rbg.capacity));
@@ -994,7 +1073,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
         XT_HIP(hipMalloc(&d_film, film_bytes));
         XT_HIP(hipMemcpyAsync(d_film, film, film_bytes, hipMemcpyHostToDevice, s->stream));
     }
-    RenderBuffers& rb = s->rb;
+    RenderBuffers& rb = rbg;
     if (s->counting) XT_HIP(hipMemsetAsync(rb.visits, 0, 4 * sizeof(unsigned long long), s->stream));
     WaveParams wp{};
     wp.width = p->width;
