@@ -903,16 +903,17 @@ struct StageTimer {
     }
 };
 
-// Paths in flight per batch of xt_render (224 B each, 15 GB at the default
-// 64 Mi); XITILS_AMD_PATHS overrides.  Larger batches spend less of a
-// render in its last, sparse bounce rounds: on the 1080p heightfield at 64
-// spp, 1349 Mpaths/s at 16 Mi paths (8 spp per batch), 1468 at 32 Mi, 1543
-// at 64 Mi.
+// Paths in flight per batch of xt_render (224 B each, 30 GB at the default
+// 128 Mi; the guiding lobes add 128 B per path to a guided render's set);
+// XITILS_AMD_PATHS overrides (65536 .. 256 Mi).  Larger batches spend less
+// of a render in its last, sparse bounce rounds: on the 1080p heightfield at
+// 64 spp, 1349 Mpaths/s at 16 Mi paths (8 spp per batch), 1468 at 32 Mi,
+// 1543-1559 at 64 Mi, 1605-1608 at 128 Mi (the whole frame in one batch).
 static uint32_t path_capacity() {
     static uint32_t cap = [] {
         const char* e = getenv("XITILS_AMD_PATHS");
         long v = e ? atol(e) : 0;
-        return v >= 65536 && v <= (1l << 26) ? (uint32_t)v : (1u << 26);
+        return v >= 65536 && v <= (1l << 28) ? (uint32_t)v : (1u << 27);
     }();
     return cap;
 }
