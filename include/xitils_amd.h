@@ -252,7 +252,8 @@ typedef struct xt_batch_item {
 /* Many RenderTarget::render calls at once (no reference counterpart; the
  * closest is the tile scheduler of RenderTarget.h:84-127).  Each item gives
  * the film xt_render would give, bit for bit.  Unguided items of at most 4 Mi
- * paths share one wavefront: their paths are numbered one after another and
+ * paths, and larger unguided items that share their scene with another such
+ * item, share one wavefront: their paths are numbered one after another and
  * walked in windows of the path capacity, each window one replay of a
  * captured graph that traces and shades every path in its own item's scene.
  * Other items of at most 4 Mi paths run their bounce loop as a per-scene HIP

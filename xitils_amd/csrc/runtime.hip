@@ -1841,6 +1841,10 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
         result = xt_render(it.scene, &it.camera, &it.params, d_films[i], it.film_is_f64, 1);
         if (result == XT_OK) it.ms = (float)it.scene->stats.ms_total;
     }
+    // larger unguided items that share their scene with another such item
+    // would render one after another (a scene renders on its one stream):
+    // they share the wavefront's bounce rounds instead
+    std::vector<uint32_t> same_scene(n, 0);
     for (uint32_t i = 0; i < n && !in_order; ++i) {
         const xt_batch_item& it = items[i];
         const bool can = one_device && !(it.params.flags & XT_FLAG_GUIDE) && it.params.spp > 0;
@@ -1848,6 +1852,13 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
         const bool small = (uint64_t)it.params.width * it.params.height * it.params.spp <=
                            graph_capacity((uint64_t)it.params.width * it.params.height, it.params.spp);
         uni[i] = mode == BatchMode::Unified || small || (shared_film && !any_guided);
+        if (!uni[i])
+            for (uint32_t j = 0; j < n; ++j) same_scene[j] += items[j].scene == it.scene;
+    }
+    for (uint32_t i = 0; i < n && !in_order && mode == BatchMode::Hybrid; ++i) {
+        const xt_batch_item& it = items[i];
+        const bool can = one_device && !(it.params.flags & XT_FLAG_GUIDE) && it.params.spp > 0;
+        if (can && !uni[i] && same_scene[i] >= 2) uni[i] = 1;
     }
     // per-item graphs first (their scenes' streams), so the unified runs and
     // the lagged-loop items below overlap them
