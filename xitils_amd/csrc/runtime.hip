@@ -1930,13 +1930,17 @@ xt_status xt_render_batch(xt_batch_item* items, uint32_t n) {
                     it.ms = (float)it.scene->stats.ms_total;
                 }
         };
-        if (nthreads > 1) {
+        if (!groups.empty()) {
             std::vector<std::thread> pool;
-            for (uint32_t t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+            for (uint32_t t = 1; t < nthreads; ++t) {
+                try {
+                    pool.emplace_back(work, t);
+                } catch (...) {   // no thread: the groups it would have taken go to the others
+                    break;
+                }
+            }
             work(0);
             for (std::thread& th : pool) th.join();
-        } else if (!groups.empty()) {
-            work(0);
         }
         for (size_t t = 0; t < tst.size() && result == XT_OK; ++t)
             if (tst[t] != XT_OK) result = fail(tst[t], terr[t]);
