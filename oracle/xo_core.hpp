@@ -139,9 +139,32 @@ inline void sinCos(float x, float* s, float* c) {
 }
 // std::pow(float, int) promotes to double: x^5 by double products, one float rounding.
 inline float pow5(float x) { const double d = x; return (float)(d * d * d * d * d); }
-// atan2 and acos in double, rounded to float.
-inline float atan2F(float y, float x) { return (float)std::atan2((double)y, (double)x); }
-inline float acosF(float x) { return (float)std::acos((double)x); }
+// atan2 and acos in float (the device's method, xt_libm.h): atan on [0, 1]
+// as a P(a^2) with a degree-8 polynomial (near-minimax fit, float
+// coefficients, fma Horner), then the octant; acos x = atan2(sqrt((1 - x)
+// (1 + x)), x).  Pinned against float64 (2.4 / 2.8 ulp).
+inline float atanPoly(float t) {
+    float p = 0x1.758192p-9f;
+    p = std::fma(p, t, -0x1.073e90p-6f);
+    p = std::fma(p, t, 0x1.5db3fap-5f);
+    p = std::fma(p, t, -0x1.335c98p-4f);
+    p = std::fma(p, t, 0x1.b3d812p-4f);
+    p = std::fma(p, t, -0x1.22e3a2p-3f);
+    p = std::fma(p, t, 0x1.9972d8p-3f);
+    p = std::fma(p, t, -0x1.5554a6p-2f);
+    return std::fma(p, t, 1.0f);
+}
+inline float atan2F(float y, float x) {
+    const float Pi = 0x1.921fb6p+1f, HalfPi = 0x1.921fb6p+0f;
+    const float ax = std::fabs(x), ay = std::fabs(y);
+    const float mx = std::fmax(ax, ay), mn = std::fmin(ax, ay);
+    const float a = mx > 0.0f ? mn / mx : 0.0f;
+    float r = a * atanPoly(a * a);
+    if (ay > ax) r = HalfPi - r;
+    if (std::signbit(x)) r = Pi - r;
+    return std::copysign(r, y);
+}
+inline float acosF(float x) { return atan2F(std::sqrt((1.0f - x) * (1.0f + x)), x); }
 
 // Vector.h:781-795
 struct Basis {

@@ -1,8 +1,9 @@
 """The path's transcendentals -- the oracle's (xo_core.hpp) and the HIP
 library's (xt_libm.h, built for the host with g++) -- against float64 numpy:
 sin and cos within 2 ulp on [-256, 256] (dense around the angles the path
-uses, [-2 pi, 2 pi]), x^5 equal to the correctly rounded value, atan2 and
-acos within 1 ulp; and the two implementations equal bit for bit, which is
+uses, [-2 pi, 2 pi]), x^5 equal to the correctly rounded value, atan2
+within 2.5 ulp and acos within 3 ulp (including the zeros' signs and the
+ends of acos's domain); and the two implementations equal bit for bit, which is
 what makes GPU and oracle sample the same directions."""
 import os
 import subprocess
@@ -75,13 +76,22 @@ def test_pow5_correctly_rounded_and_identical(product_libm):
     assert np.all(ulp_err(o, x.astype(np.float64) ** 5) <= 0.5000001)
 
 
-def test_atan2_acos_within_1_ulp_and_identical(product_libm):
+def test_atan2_acos_pinned_and_identical(product_libm):
     rng = np.random.default_rng(3)
-    yx = rng.normal(size=(200000, 2)).astype(np.float32)
+    th = np.linspace(-np.pi, np.pi, 400001)
+    yx = np.concatenate([rng.normal(size=(200000, 2)),
+                         np.stack([np.sin(th), np.cos(th)], 1),                 # unit directions, every octant
+                         rng.normal(size=(50000, 2)) * 10.0 ** rng.integers(-6, 6, size=(50000, 1)),
+                         [[0.0, 1.0], [-0.0, 1.0], [0.0, -1.0], [-0.0, -1.0], [1.0, 0.0], [-1.0, 0.0],
+                          [0.0, 0.0], [-0.0, 0.0], [0.0, -0.0], [-0.0, -0.0], [1.0, 1.0], [-1.0, -1.0]]]).astype(np.float32)
     o, p = po.libm("atan2", yx), product_libm("atan2", yx)
-    assert np.array_equal(o, p)
-    assert ulp_err(o, np.arctan2(yx[:, 0].astype(np.float64), yx[:, 1].astype(np.float64))).max() <= 1.0
-    c = np.concatenate([rng.uniform(-1, 1, 200000), [-1.0, 0.0, 1.0]]).astype(np.float32)
+    assert np.array_equal(o.view(np.uint32), p.view(np.uint32))
+    want = np.arctan2(yx[:, 0].astype(np.float64), yx[:, 1].astype(np.float64))
+    assert ulp_err(o, want).max() <= 2.5
+    assert np.array_equal(np.signbit(o), np.signbit(want))                      # IEEE zero and quadrant signs
+    c = np.concatenate([rng.uniform(-1, 1, 200000), np.linspace(-1, 1, 400001),
+                        1.0 - np.float32(2.0 ** -24) * np.arange(200), [-1.0, 0.0, -0.0, 1.0]]).astype(np.float32)
     o, p = po.libm("acos", c), product_libm("acos", c)
-    assert np.array_equal(o, p)
-    assert ulp_err(o, np.arccos(c.astype(np.float64))).max() <= 1.0
+    assert np.array_equal(o.view(np.uint32), p.view(np.uint32))
+    assert ulp_err(o, np.arccos(c.astype(np.float64))).max() <= 3.0
+    assert o[-1] == 0.0 and o[-4] == np.float32(np.pi)
