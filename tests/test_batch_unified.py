@@ -105,6 +105,25 @@ def test_shared_film_with_a_guided_item_renders_in_item_order(dev):
     assert np.array_equal(ref, film)
 
 
+def test_large_guided_items_on_two_scenes_render_concurrently(dev):
+    # guided items of more than one graph replay on different scenes: the
+    # lagged loop, from two host threads at once; films as xt_render's
+    a = api.Scene(dev, scenes.cornell(32, 32))
+    b = api.Scene(dev, scenes.material_zoo(32, 32))
+    for sc, seed in ((a, 3), (b, 4)):
+        sc.guide_init(seed=seed)
+        sc.render(64, 64, 4, seed=seed, max_depth=6, flags=api.XT_FLAG_GUIDE | api.XT_FLAG_GUIDE_TRAIN)
+    kw = dict(max_depth=6, flags=api.XT_FLAG_GUIDE)
+    jobs = [(a, 1024, 1024, 5, dict(seed=21, **kw)), (b, 1024, 1024, 5, dict(seed=22, **kw)),
+            (a, 512, 512, 20, dict(seed=23, **kw))]
+    ref = [sc.render(w, h, spp, **k) for sc, w, h, spp, k in jobs]
+    print("[gpu] guided batch of 3 large items on 2 scenes", flush=True)
+    films, ms = api.render_batch([(sc, w, h, spp, None, k) for sc, w, h, spp, k in jobs])
+    for r, f, t in zip(ref, films, ms):
+        assert np.array_equal(r, f)
+        assert t > 0
+
+
 def test_per_item_graphs_stay_available(dev):
     sc = api.Scene(dev, scenes.cornell(48, 48))
     kw = dict(seed=12, max_depth=6)
