@@ -185,9 +185,13 @@ typedef struct xt_render_stats {
      * default quantised 4-wide nodes) and triangle (48 B) fetches of the
      * traversal kernels; filled only after xt_scene_set_counting(scene, 1). */
     uint64_t ext_node_fetches, ext_tri_fetches, shadow_node_fetches, shadow_tri_fetches;
-    /* guiding: network inference and training kernel time, Adam steps taken,
-     * training samples kept, queue entries run through the network and
-     * samples run through the gradient kernels (all epochs) */
+    /* guiding: network inference kernel time; training time = the durations
+     * of the training bursts (HIP events on the guide's stream) that ended
+     * since the previous render (a training render's own burst runs after it
+     * returns and is counted by a later render; ms_total does not include
+     * it); Adam steps taken, training samples kept, queue entries run
+     * through the network and samples run through the gradient kernels (all
+     * epochs) by this render's bursts */
     double ms_guide_infer, ms_guide_train;
     uint64_t guide_steps, guide_samples, guide_queries, guide_grad_samples;
 } xt_render_stats;
@@ -328,7 +332,17 @@ xt_status xt_vmf_fit(xt_device* dev, uint32_t num_lobes, const float* samples, u
  * normal, view direction) to an 8-lobe vMF mixture over incident directions;
  * raw outputs per lobe j are y[5j..5j+4] = (mu unnormalised, log kappa,
  * alpha logit).  Weights live per scene on the device: fp32 master copy,
- * bf16 copies for the MFMA kernels, Adam state. */
+ * bf16 copies for the MFMA kernels, Adam state.
+ *
+ * Online training (xt_render with XT_FLAG_GUIDE_TRAIN): after each render
+ * batch, its samples go through Adam steps (a burst) on a stream of the
+ * guide's own, which then publishes the network into one of two published
+ * copies; xt_render returns without waiting, so the next render runs while
+ * the burst does.  A training batch samples with the network the burst
+ * before last published, any other guided render with the last one (its
+ * stream waits for that burst on the device).  Which copy a render reads
+ * depends only on the number of bursts, not on timing.  Every xt_guide_*
+ * call below first waits for the bursts in flight. */
 #define XT_GUIDE_NUM_PARAMS 17648   /* W0 64x96, b0 64, W1 64x64, b1, W2 64x64, b2, W3 48x64, b3 48 */
 #define XT_GUIDE_NUM_INPUTS 9       /* query: p.xyz, shading n.xyz, wo.xyz */
 #define XT_GUIDE_NUM_OUTPUTS 40

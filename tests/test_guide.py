@@ -337,3 +337,44 @@ def test_equal_time_compare_reports_the_median_run():
     assert res["relmse_ratio_equal_time"] == pytest.approx(ratios[1], rel=1e-3)
     assert res["guided"]["seed"] in (1, 2, 3) and res["plain"]["spp"] > 0
     assert all(np.isfinite(r["relmse"]) and r["spp"] > 0 for r in res["runs"])
+    # the curve's device-side relMSE is quality.rel_mse of the film
+    curve, _ = equal_time.progressive(sc, 48, 48, 0.05, ref, max_depth=8, seed=7)
+    spp = curve[-1][1]
+    film = sc.render(48, 48, spp, seed=7, max_depth=8)
+    assert curve[-1][2] == pytest.approx(quality.rel_mse(film / spp, ref), rel=1e-4)
+    assert all(t1 > t0 for (t0, _, _), (t1, _, _) in zip(curve, curve[1:]))
+
+
+@pytest.mark.gpu
+def test_training_bursts_read_the_burst_before_last():
+    """A training render's Adam steps (a burst) run on a stream of their own
+    while the next render runs.  A training render samples with the network
+    published by the burst before last, any other render with the last one.
+    (Training itself is not bit-reproducible: samples are appended through
+    atomics, so gradient sums run in varying order; films are compared only
+    where the network they read is fixed.)"""
+    print("[gpu] asynchronous training bursts, cornell 32x32", flush=True)
+    dev = api.Device(0)
+    desc = scenes.cornell(32, 32)
+    tr = dict(max_depth=8, flags=api.XT_FLAG_GUIDE | api.XT_FLAG_GUIDE_TRAIN)
+    a, b, c, d = (api.Scene(dev, desc) for _ in range(4))
+    for sc in (a, b, c, d):
+        sc.guide_init(seed=5)
+    fa = [a.render(32, 32, 8, seed=60 + k, **tr) for k in range(4)]          # back to back
+    fb = [b.render(32, 32, 8, seed=60, **tr)]
+    p1 = b.guide_params()                                                    # waits for the first burst
+    assert b.stats()["guide_steps"] > 0
+    fb += [b.render(32, 32, 8, seed=60 + k, **tr) for k in (1, 2)]
+    # the first two training renders read the initial network, whenever the host waits
+    assert np.array_equal(fa[0], fb[0]) and np.array_equal(fa[1], fb[1])
+    assert np.array_equal(c.render(32, 32, 8, seed=61, **tr), fa[1])
+    # the third reads the first burst's network
+    d.guide_set_params(p1)
+    assert np.array_equal(d.render(32, 32, 8, seed=62, **tr), fb[2])
+    assert not np.array_equal(fb[2], c.render(32, 32, 8, seed=62, **tr))
+    # a render that does not train reads the last burst's network: the same
+    # film as from the final parameters set again (which re-derive every copy)
+    kw = dict(seed=70, max_depth=8, flags=api.XT_FLAG_GUIDE)
+    g1 = a.render(32, 32, 8, **kw)
+    a.guide_set_params(a.guide_params())
+    assert np.array_equal(a.render(32, 32, 8, **kw), g1)

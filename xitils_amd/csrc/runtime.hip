@@ -75,19 +75,34 @@ struct GuideState {
     float* v = nullptr;
     float* partial = nullptr;        // per-block gradient rows of k_guide_grad
     uint32_t partial_rows = 0;
-    void* net_mem = nullptr;         // GuideChain + GuideBack
-    GuideNet net{};
+    void* net_mem = nullptr;         // GuideChain + GuideBack + the two published GuideChains
+    GuideNet net{};                  // the copy training reads and Adam writes
     uint64_t step = 0;               // Adam steps taken
     TrainRec* recs = nullptr;
-    GuideSample* samples = nullptr;
+    GuideSample* samples[2] = {};    // finalised samples of a training render (bursts alternate)
     uint32_t rec_cap = 0;
     uint32_t* d_count = nullptr;     // finalised samples
     float* d_wsum = nullptr;         // their weight sum
     float* host_io = nullptr;        // pinned: count + weight sum read-back
+    // A training render's Adam steps (a burst) run on train_stream while the
+    // next render runs: burst b reads samples[b % 2] and ends by publishing
+    // the network into pub[b % 2].  A training render reads pub[bursts % 2]
+    // (the burst before last: the last one may still run), any other render
+    // pub[(bursts + 1) % 2] (the last burst, waited for).
+    hipStream_t train_stream = nullptr;
+    GuideChain* pub[2] = {};
+    hipEvent_t pub_ev[2] = {}, t0[2] = {}, t1[2] = {};
+    bool pub_pending[2] = {}, unreported[2] = {};
+    uint64_t bursts = 0;
+    double train_ms_carry = 0.0;     // burst time folded into the next render's stats
     void release() {
+        if (train_stream) hipStreamSynchronize(train_stream);
         hipFree(P); hipFree(m); hipFree(v); hipFree(partial); hipFree(net_mem);
-        hipFree(recs); hipFree(samples); hipFree(d_count); hipFree(d_wsum);
+        hipFree(recs); hipFree(samples[0]); hipFree(samples[1]); hipFree(d_count); hipFree(d_wsum);
         if (host_io) hipHostFree(host_io);
+        for (hipEvent_t e : {pub_ev[0], pub_ev[1], t0[0], t0[1], t1[0], t1[1]})
+            if (e) hipEventDestroy(e);
+        if (train_stream) hipStreamDestroy(train_stream);
         *this = GuideState();
     }
 };
@@ -947,29 +962,67 @@ static void guide_init_params(uint32_t seed, std::vector<float>& P) {
     for (int i = 0; i < G_OUT * G_HID; ++i) P[G_OFF_W3 + i] = uni(0.1f * std::sqrt(6.0f / G_HID));
 }
 
+// Both published copies <- the training copy, on s->stream (the train
+// stream must be idle).
+static xt_status guide_publish_all(xt_scene* s) {
+    GuideState& g = s->guide;
+    for (int k = 0; k < 2; ++k) {
+        XT_HIP(hipMemcpyAsync(g.pub[k], g.net.chain, sizeof(GuideChain), hipMemcpyDeviceToDevice, s->stream));
+        g.pub_pending[k] = false;
+    }
+    return XT_OK;
+}
+
+// The network copies from the fp32 parameters, both published copies
+// included (the train stream must be idle).
 static xt_status guide_prepare(xt_scene* s) {
     GuideState& g = s->guide;
     hipLaunchKernelGGL(k_guide_prepare, dim3((G_HID * GC_LD0 + 255) / 256), dim3(256), 0, s->stream, g.P, g.net);
     if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "k_guide_prepare launch failed");
-    return XT_OK;
+    return guide_publish_all(s);
 }
 
+// Folds the times of finished training bursts into the stats (wait: block
+// until they finish; slot >= 0: only that slot's burst).
+static void guide_collect_bursts(xt_scene* s, bool wait, int slot = -1) {
+    GuideState& g = s->guide;
+    for (int k = 0; k < 2; ++k) {
+        if (!g.unreported[k] || (slot >= 0 && k != slot)) continue;
+        if ((wait ? hipEventSynchronize(g.t1[k]) : hipEventQuery(g.t1[k])) != hipSuccess) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, g.t0[k], g.t1[k]) == hipSuccess) g.train_ms_carry += ms;
+        g.unreported[k] = false;
+    }
+}
+
+// Every guide API call first lets the training bursts in flight finish.
 static xt_status guide_need(xt_scene* s) {
     if (!s) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!s->guide.ready) return fail(XT_ERR_INVALID_ARGUMENT, "guiding not initialised (xt_guide_init)");
     XT_HIP(hipSetDevice(s->dev->ordinal));
+    XT_HIP(hipStreamSynchronize(s->guide.train_stream));
+    guide_collect_bursts(s, true);
+    return XT_OK;
+}
+
+// s->stream waits (on the device) for the last training burst: captured
+// guided renders read the training copy, final once that burst has ended.
+static xt_status guide_wait_last_burst(xt_scene* s) {
+    GuideState& g = s->guide;
+    const int k = (int)((g.bursts + 1) % 2);
+    if (g.ready && g.pub_pending[k]) XT_HIP(hipStreamWaitEvent(s->stream, g.pub_ev[k], 0));
     return XT_OK;
 }
 
 // Gradient of the minibatch d_samples[0..n) into g.partial rows [0, *rows):
 // one fused kernel, a block per 128 samples (at most partial_rows blocks,
 // each taking every rows-th sample group).
-static xt_status guide_gradient_rows(xt_scene* s, const GuideSample* d_samples, uint32_t n, float weight_scale,
-                                     float weight_clip, uint32_t* rows) {
+static xt_status guide_gradient_rows(xt_scene* s, hipStream_t stream, const GuideSample* d_samples, uint32_t n,
+                                     float weight_scale, float weight_clip, uint32_t* rows) {
     GuideState& g = s->guide;
     const uint32_t groups = (n + GG_BLOCK_SAMPLES - 1) / GG_BLOCK_SAMPLES;
     const uint32_t blocks = std::max(1u, std::min(groups, g.partial_rows));
-    hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(GG_BLOCK), 0, s->stream, g.net, g.gb, d_samples, n, weight_scale,
+    hipLaunchKernelGGL(k_guide_grad, dim3(blocks), dim3(GG_BLOCK), 0, stream, g.net, g.gb, d_samples, n, weight_scale,
                        weight_clip, g.partial);
     XT_HIP(hipGetLastError());
     *rows = blocks;
@@ -977,8 +1030,8 @@ static xt_status guide_gradient_rows(xt_scene* s, const GuideSample* d_samples, 
 }
 
 // Adam steps over d_samples[0..n) in minibatches, weights scaled by weight_scale.
-static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t n, float weight_scale,
-                             uint32_t max_steps, uint64_t* steps_taken) {
+static xt_status guide_train(xt_scene* s, hipStream_t stream, const GuideSample* d_samples, uint32_t n,
+                             float weight_scale, uint32_t max_steps, uint64_t* steps_taken) {
     GuideState& g = s->guide;
     const uint32_t M = std::max(64u, g.cfg.minibatch);
     const uint32_t per_epoch = (n + M - 1) / M;
@@ -986,14 +1039,14 @@ static xt_status guide_train(xt_scene* s, const GuideSample* d_samples, uint32_t
     for (uint32_t k = 0; k < steps; ++k) {
         const uint32_t off = (k % per_epoch) * M, cnt = std::min(M, n - off);
         uint32_t grid = 0;
-        xt_status gst = guide_gradient_rows(s, d_samples + off, cnt, weight_scale, g.cfg.weight_clip, &grid);
+        xt_status gst = guide_gradient_rows(s, stream, d_samples + off, cnt, weight_scale, g.cfg.weight_clip, &grid);
         if (gst != XT_OK) return gst;
         s->stats.guide_grad_samples += cnt;
         g.step++;
         const float b1t = (float)std::pow((double)g.cfg.beta1, (double)g.step);
         const float b2t = (float)std::pow((double)g.cfg.beta2, (double)g.step);
         // Adam also re-packs the bf16 copies the next step and the renders read
-        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(GA_BLOCK), 0, s->stream, g.P, g.m, g.v, g.partial,
+        hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAMS + 31) / 32), dim3(GA_BLOCK), 0, stream, g.P, g.m, g.v, g.partial,
                            grid, g.cfg.learning_rate, g.cfg.beta1, g.cfg.beta2, g.cfg.adam_eps, b1t, b2t,
                            1.0f / (float)cnt, g.net);
         if (hipGetLastError() != hipSuccess) return fail(XT_ERR_DEVICE, "guide training launch failed");
@@ -1007,7 +1060,8 @@ static xt_status guide_ensure_training(xt_scene* s) {
     if (g.recs) return XT_OK;
     g.rec_cap = std::max(1024u, g.cfg.max_records);
     XT_HIP(dalloc(&g.recs, g.rec_cap));
-    XT_HIP(dalloc(&g.samples, g.rec_cap));
+    XT_HIP(dalloc(&g.samples[0], g.rec_cap));
+    XT_HIP(dalloc(&g.samples[1], g.rec_cap));
     return XT_OK;
 }
 
@@ -1015,14 +1069,16 @@ static xt_status guide_ensure_training(xt_scene* s) {
 static xt_status guide_upload_samples(xt_scene* s, const float* queries, const float* dirs, const float* weights,
                                       uint32_t n, GuideSample** out) {
     GuideState& g = s->guide;
-    if (g.rec_cap < n || !g.samples) {
+    if (g.rec_cap < n || !g.samples[0]) {
         hipFree(g.recs);
-        hipFree(g.samples);
+        hipFree(g.samples[0]);
+        hipFree(g.samples[1]);
         g.recs = nullptr;
-        g.samples = nullptr;
+        g.samples[0] = g.samples[1] = nullptr;
         g.rec_cap = std::max(n, g.cfg.max_records);
         XT_HIP(dalloc(&g.recs, g.rec_cap));
-        XT_HIP(dalloc(&g.samples, g.rec_cap));
+        XT_HIP(dalloc(&g.samples[0], g.rec_cap));
+        XT_HIP(dalloc(&g.samples[1], g.rec_cap));
     }
     std::vector<GuideSample> hs(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -1030,9 +1086,9 @@ static xt_status guide_upload_samples(xt_scene* s, const float* queries, const f
         for (int k = 0; k < 3; ++k) hs[i].dir[k] = dirs[3 * (size_t)i + k];
         hs[i].weight = weights[i];
     }
-    XT_HIP(hipMemcpyAsync(g.samples, hs.data(), n * sizeof(GuideSample), hipMemcpyHostToDevice, s->stream));
+    XT_HIP(hipMemcpyAsync(g.samples[0], hs.data(), n * sizeof(GuideSample), hipMemcpyHostToDevice, s->stream));
     XT_HIP(hipStreamSynchronize(s->stream));
-    *out = g.samples;
+    *out = g.samples[0];
     return XT_OK;
 }
 
@@ -1052,6 +1108,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
     const uint64_t npix_all = (uint64_t)p->width * p->height;
     const size_t film_bytes = npix_all * 3 * (film_f64 ? 8 : 4);
     s->stats = xt_render_stats{};
+    if (s->guide.ready) guide_collect_bursts(s, false);
     if (p->spp == 0) return XT_OK;
     const uint32_t cap = (uint32_t)std::min<uint64_t>(path_capacity(), npix_all * p->spp);
     PooledBuffers pooled;
@@ -1111,6 +1168,20 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             wp.num_pixels = np;
             wp.num_paths = np * Sb;
             wp.sample0 = p->spp_offset + s0;
+            // The network this batch samples with: a training batch the one
+            // published by the burst before last (its own samples and the
+            // burst it starts take that slot next), any other the last one.
+            GuideNet net_r{};
+            int tslot = 0;
+            if (guide) {
+                tslot = (int)(g.bursts % 2);
+                const int rd = train ? tslot : 1 - tslot;
+                if (g.pub_pending[rd] && hipStreamWaitEvent(s->stream, g.pub_ev[rd], 0) != hipSuccess) {
+                    result = fail(XT_ERR_DEVICE, "hipStreamWaitEvent failed");
+                    break;
+                }
+                net_r = GuideNet{g.pub[rd], g.net.back};
+            }
             hipEvent_t a = tm.mark();
             hipLaunchKernelGGL(k_generate, dim3(blocks256(wp.num_paths)), dim3(256), 0, s->stream, wp, rb.q[0], rb.radiance);
             XT_LAUNCHED(result);
@@ -1164,7 +1235,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                                                                   (uint32_t)s->dev->num_cu * 4));
                         uint2* l4 = (uint2*)rb.lobes;
                         float2* l2 = (float2*)(rb.lobes + (size_t)2 * G_LOBES * rb.capacity);
-                        hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, g.net, g.gb, s->ds,
+                        hipLaunchKernelGGL(k_guide_infer, dim3(ib), dim3(GI_BLOCK), 0, s->stream, net_r, g.gb, s->ds,
                                            rb.q[cur], rb.counts + cur, rb.hits, l4, l2, rb.capacity,
                                            g.cfg.max_guided_vertex);
                         XT_LAUNCHED(result);
@@ -1233,13 +1304,15 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
             XT_LAUNCHED(result);
             tm.add(3, a, tm.mark());
             if (train && rec_base > 0 && result == XT_OK) {
-                // finished paths -> training samples -> Adam steps
+                // finished paths -> training samples -> Adam steps on the
+                // train stream, which then publish the network into
+                // pub[tslot]; the next render runs meanwhile
                 a = tm.mark();
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, g.d_count, 1u);
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, (uint32_t*)g.d_wsum, 1u);
                 hipLaunchKernelGGL(k_guide_finalize, dim3((rec_base + GF_BLOCK - 1) / GF_BLOCK), dim3(GF_BLOCK), 0, s->stream,
                                    g.recs, rec_base,
-                                   rb.radiance, g.cfg.train_target, g.samples, g.d_count, g.d_wsum);
+                                   rb.radiance, g.cfg.train_target, g.samples[tslot], g.d_count, g.d_wsum);
                 XT_LAUNCHED(result);
                 hipMemcpyAsync(g.host_io, g.d_count, 4, hipMemcpyDeviceToHost, s->stream);
                 hipMemcpyAsync(g.host_io + 1, g.d_wsum, 4, hipMemcpyDeviceToHost, s->stream);
@@ -1250,13 +1323,32 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 uint32_t ns;
                 std::memcpy(&ns, g.host_io, 4);
                 const float wsum = g.host_io[1];
+                tm.add(3, a, tm.mark());
                 if (ns > 0 && wsum > 0.0f && std::isfinite(wsum)) {
-                    xt_status ts = guide_train(s, g.samples, ns, (float)ns / wsum, g.cfg.max_steps_per_batch,
-                                               &s->stats.guide_steps);
-                    if (ts != XT_OK && result == XT_OK) result = ts;
+                    // the finalisation has ended (synchronised above); so has
+                    // the burst that last used this slot (waited for before
+                    // this batch)
+                    guide_collect_bursts(s, true, tslot);
+                    hipStream_t ts_ = g.train_stream;
+                    xt_status ts = hipEventRecord(g.t0[tslot], ts_) == hipSuccess ? XT_OK
+                                   : fail(XT_ERR_DEVICE, "hipEventRecord failed");
+                    if (ts == XT_OK)
+                        ts = guide_train(s, ts_, g.samples[tslot], ns, (float)ns / wsum, g.cfg.max_steps_per_batch,
+                                         &s->stats.guide_steps);
+                    if (ts == XT_OK && (hipMemcpyAsync(g.pub[tslot], g.net.chain, sizeof(GuideChain),
+                                                       hipMemcpyDeviceToDevice, ts_) != hipSuccess ||
+                                        hipEventRecord(g.pub_ev[tslot], ts_) != hipSuccess ||
+                                        hipEventRecord(g.t1[tslot], ts_) != hipSuccess))
+                        ts = fail(XT_ERR_DEVICE, "guide publication failed");
+                    if (ts != XT_OK) {
+                        if (result == XT_OK) result = ts;
+                        break;
+                    }
+                    g.pub_pending[tslot] = true;
+                    g.unreported[tslot] = true;
+                    g.bursts++;
                     s->stats.guide_samples += ns;
                 }
-                tm.add(5, a, tm.mark());
             }
         }
     }
@@ -1273,7 +1365,11 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
         s->stats.ms_shadow = tm.total(2);
         s->stats.ms_other = tm.total(3);
         s->stats.ms_guide_infer = tm.total(4);
-        s->stats.ms_guide_train = tm.total(5);
+        if (guide) {   // training bursts that have ended
+            guide_collect_bursts(s, false);
+            s->stats.ms_guide_train = s->guide.train_ms_carry;
+            s->guide.train_ms_carry = 0.0;
+        }
         if (s->counting) {
             unsigned long long v[4] = {0, 0, 0, 0};
             if (hipMemcpy(v, rb.visits, sizeof v, hipMemcpyDeviceToHost) == hipSuccess) {
@@ -1488,6 +1584,10 @@ static xt_status find_graph(xt_scene* s, const xt_camera* cam, const xt_render_p
 // Enqueues one render on s->stream: reset the cursor, replay the graph once
 // per batch.
 static xt_status enqueue_graph_render(xt_scene* s, const RenderGraph& g, uint32_t spp_offset) {
+    if (g.key.p.flags & XT_FLAG_GUIDE) {
+        xt_status st = guide_wait_last_burst(s);
+        if (st != XT_OK) return st;
+    }
     BatchPlan plan = g.plan;
     plan.spp_offset = spp_offset;
     hipLaunchKernelGGL(k_cursor_init, dim3(1), dim3(1), 0, s->stream, s->d_cursor, plan);
@@ -2168,14 +2268,21 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     chk(dalloc(&g.d_count, 1));
     chk(dalloc(&g.d_wsum, 1));
     chk(hipHostMalloc((void**)&g.host_io, 2 * sizeof(float), hipHostMallocDefault));
-    // the forward and backward copies in one allocation, 256-byte aligned pieces
-    const size_t sizes[2] = {sizeof(GuideChain), sizeof(GuideBack)};
-    size_t off[2], total = 0;
-    for (int k = 0; k < 2; ++k) {
+    // the forward and backward copies and the two published forward copies
+    // in one allocation, 256-byte aligned pieces
+    const size_t sizes[4] = {sizeof(GuideChain), sizeof(GuideBack), sizeof(GuideChain), sizeof(GuideChain)};
+    size_t off[4], total = 0;
+    for (int k = 0; k < 4; ++k) {
         off[k] = total;
         total += (sizes[k] + 255) / 256 * 256;
     }
     chk(hipMalloc(&g.net_mem, total));
+    chk(hipStreamCreateWithFlags(&g.train_stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        chk(hipEventCreateWithFlags(&g.pub_ev[k], hipEventDisableTiming));
+        chk(hipEventCreate(&g.t0[k]));
+        chk(hipEventCreate(&g.t1[k]));
+    }
     if (st != XT_OK) {
         g.release();
         return st;
@@ -2183,6 +2290,8 @@ xt_status xt_guide_init(xt_scene* s, const xt_guide_config* cfg) {
     char* b = (char*)g.net_mem;
     g.net.chain = (GuideChain*)(b + off[0]);
     g.net.back = (GuideBack*)(b + off[1]);
+    g.pub[0] = (GuideChain*)(b + off[2]);
+    g.pub[1] = (GuideChain*)(b + off[3]);
     g.ready = true;
     std::vector<float> P;
     guide_init_params(c.seed, P);
@@ -2260,7 +2369,7 @@ xt_status xt_guide_gradient(xt_scene* s, const float* queries, const float* dirs
     }
     GuideState& g = s->guide;
     uint32_t grid = 0;
-    st = guide_gradient_rows(s, ds, n, weight_scale, weight_clip, &grid);
+    st = guide_gradient_rows(s, s->stream, ds, n, weight_scale, weight_clip, &grid);
     if (st != XT_OK) return st;
     std::vector<float> rows((size_t)grid * G_NPARAMS);
     XT_HIP(hipMemcpyAsync(rows.data(), g.partial, rows.size() * 4, hipMemcpyDeviceToHost, s->stream));
@@ -2286,8 +2395,10 @@ xt_status xt_guide_train_step(xt_scene* s, const float* queries, const float* di
     if (st != XT_OK) return st;
     const uint32_t saved = s->guide.cfg.minibatch;
     s->guide.cfg.minibatch = std::max(saved, n);   // one step over all n samples
-    st = guide_train(s, ds, n, (float)(n / wsum), 1, nullptr);
+    st = guide_train(s, s->stream, ds, n, (float)(n / wsum), 1, nullptr);
     s->guide.cfg.minibatch = saved;
+    if (st != XT_OK) return st;
+    st = guide_publish_all(s);
     if (st != XT_OK) return st;
     XT_HIP(hipStreamSynchronize(s->stream));
     return XT_OK;
