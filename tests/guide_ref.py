@@ -19,7 +19,7 @@ OFF_B2 = OFF_W2 + G_HID * G_HID
 OFF_W3 = OFF_B2 + G_HID
 OFF_B3 = OFF_W3 + G_OUT_PAD * G_HID
 NPARAMS = OFF_B3 + G_OUT_PAD
-KLOG_MIN, KLOG_MAX = -3.0, 7.0
+KLOG_MIN, KLOG_MAX = -3.0, 9.0
 
 
 def bf16(x):

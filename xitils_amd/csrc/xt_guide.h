@@ -38,7 +38,12 @@ constexpr int G_OFF_B2 = G_OFF_W2 + G_HID * G_HID;
 constexpr int G_OFF_W3 = G_OFF_B2 + G_HID;
 constexpr int G_OFF_B3 = G_OFF_W3 + G_OUT_PAD * G_HID;
 constexpr int G_NPARAMS = G_OFF_B3 + G_OUT_PAD;   // 17,648
-constexpr float G_KLOG_MIN = -3.0f, G_KLOG_MAX = 7.0f;
+#ifndef XT_GUIDE_KLOG_MAX
+#define XT_GUIDE_KLOG_MAX 9.0f
+#endif
+// log kappa range of a lobe: kappa up to e^9 ~ 8100, a lobe about 0.6
+// degrees wide (1 / sqrt(kappa) rad), narrower than a 5 cm gap seen from 3 m
+constexpr float G_KLOG_MIN = -3.0f, G_KLOG_MAX = XT_GUIDE_KLOG_MAX;
 
 struct GuideBounds {
     f3 lo;
