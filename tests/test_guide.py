@@ -38,6 +38,18 @@ def test_encoding_closed_form():
     assert not x[:, 78:].any()
 
 
+def test_time_to_reach_is_log_linear():
+    """equal_time.time_to_reach: the first crossing of the error curve, with
+    log(error) linear in time between checkpoints (closed form here)."""
+    from xitils_amd import equal_time
+    curve = [(1.0, 4, 1.0), (2.0, 8, 0.5), (3.0, 12, 0.25)]
+    assert equal_time.time_to_reach(curve, 2.0) == 1.0        # reached at the first checkpoint
+    assert equal_time.time_to_reach(curve, 0.5) == 2.0        # exactly at a checkpoint
+    # between 2 s (0.5) and 3 s (0.25): 0.35 = 0.5 * 2^-a at a = log2(0.5 / 0.35)
+    assert equal_time.time_to_reach(curve, 0.35) == pytest.approx(2.0 + np.log2(0.5 / 0.35), rel=1e-12)
+    assert equal_time.time_to_reach(curve, 0.1) is None       # never reached
+
+
 def test_vmf_pdf_integrates_to_one():
     # midpoint rule over (cos theta, phi) about a tilted mean direction
     mu = np.array([0.3, -0.5, 0.81])
