@@ -186,7 +186,7 @@ __global__ void k_accumulate_multi(MultiBatch mb, const float4* radiance);
 
 __global__ void k_generate(WaveParams wp, PathQueue q, float4* radiance);
 template <bool GUIDE, bool NMAP, bool SORT>
-__global__ void k_shade(WaveParams wp, DevScene sc, PathQueue qin, const uint32_t* count_in, const float4* hits,
+__global__ void k_shade(WaveParams wp, const DevScene* scene, PathQueue qin, const uint32_t* count_in, const float4* hits,
                         PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
                         float4* radiance, float4* pending, GuideRender gr);
 template <bool COUNT>

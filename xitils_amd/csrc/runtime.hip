@@ -130,6 +130,7 @@ struct xt_scene {
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev_pool;   // StageTimer events, reused across renders
     DevScene ds{};
+    DevScene* d_ds = nullptr;         // device copy of ds (the shade kernel reads it through a pointer)
     // device allocations
     float4* d_nodes = nullptr;
     float4* d_tris = nullptr;
@@ -170,7 +171,7 @@ struct xt_scene {
     GuideState guide;
 };
 
-using ShadeKernel = void (*)(WaveParams, DevScene, PathQueue, const uint32_t*, const float4*, PathQueue, uint32_t*,
+using ShadeKernel = void (*)(WaveParams, const DevScene*, PathQueue, const uint32_t*, const float4*, PathQueue, uint32_t*,
                             ShadowQueue, uint32_t*, float4*, float4*, GuideRender);
 // The shade kernel for a render: guided or not, with the normal-mapping
 // materials or not, and with the block-local material partition
@@ -249,6 +250,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
     for (RenderGraph& g : s->graphs) hipGraphExecDestroy(g.exec);
     hipFree(s->d_cursor);
+    hipFree(s->d_ds);
     s->rb.release();
     s->guide.release();
     for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
@@ -667,6 +669,15 @@ xt_status xt_scene_create(xt_device* dev, const xt_scene_desc* desc, xt_scene** 
                 s->bounds_lo[a] = std::min(s->bounds_lo[a], t.p[k][a]);
                 s->bounds_hi[a] = std::max(s->bounds_hi[a], t.p[k][a]);
             }
+    {
+        hipError_t r = dalloc(&s->d_ds, 1);
+        if (r == hipSuccess) r = hipMemcpy(s->d_ds, &ds, sizeof(DevScene), hipMemcpyHostToDevice);
+        if (r != hipSuccess) {
+            xt_scene_destroy(s);
+            return fail(r == hipErrorOutOfMemory ? XT_ERR_OUT_OF_MEMORY : XT_ERR_DEVICE,
+                        std::string("scene record upload failed: ") + hipGetErrorString(r));
+        }
+    }
     s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     *out = s;
     return XT_OK;
@@ -1253,7 +1264,7 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     }
                     a = tm.mark();
                     hipLaunchKernelGGL(shade_kernel(guide, s->nmap, p->flags), dim3(shade_blocks(bound)), dim3(SHADE_BLOCK), 0,
-                                       s->stream, wp, s->ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
+                                       s->stream, wp, s->d_ds, rb.q[cur], rb.counts + cur, rb.hits, rb.q[1 - cur],
                                        rb.counts + (1 - cur), rb.sq, rb.counts + 2, rb.radiance, rb.pending, gr);
                     XT_LAUNCHED(result);
                     tm.add(1, a, tm.mark());
@@ -1496,7 +1507,7 @@ static void enqueue_cursor_batch(xt_scene* s, const xt_render_params* p, const W
             gr.prob = g.cfg.guide_prob;
             gr.max_vertex = g.cfg.max_guided_vertex;
         }
-        hipLaunchKernelGGL(shade_kernel(guide, s->nmap, p->flags), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->ds,
+        hipLaunchKernelGGL(shade_kernel(guide, s->nmap, p->flags), dim3(sblocks), dim3(SHADE_BLOCK), 0, s->stream, wp, s->d_ds,
                            b.q[cur], b.counts + cur, b.hits, b.q[1 - cur], b.counts + (1 - cur), b.sq,
                            b.counts + 2, b.radiance, b.pending, gr);
         hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, b.sq, b.counts + 2,
