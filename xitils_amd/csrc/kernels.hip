@@ -193,12 +193,12 @@ __global__ __launch_bounds__(256) void k_generate(WaveParams wp, PathQueue q, fl
         f3 dc = normalize(mk(wp.tan_x * (fx - 0.5f) * 2, wp.tan_y * (fy - 0.5f) * 2, 1));
         f3 o = xf_point(wp.cam, mk(0, 0, 0));
         f3 d = xf_vector(wp.cam, dc);
+        // e (the sampling record) and the radiance slot of a camera ray are
+        // implied by its vertex index 1: the shade step reads neither
         q.o[id] = make_float4(o.x, o.y, o.z, 0.0f);
         q.d[id] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
         q.w[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));
-        q.e[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         if (q.rec) q.rec[id] = INVALID;
-        radiance[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -325,14 +325,17 @@ __device__ __forceinline__ void shade_entry(const WaveParams& wp, const DevScene
     f3& sh_c = out.sh_c;
     float& sh_tmax = out.sh_tmax;
     uint32_t& id = out.id;
-        const float4 o4 = qin.o[i], d4 = qin.d[i], w4 = qin.w[i], e4 = qin.e[i], h = hits[i];
+        const float4 o4 = qin.o[i], d4 = qin.d[i], w4 = qin.w[i], h = hits[i];
         id = __float_as_uint(d4.w);
         const uint32_t vf = __float_as_uint(w4.w);
         const uint32_t v = vf & 0xFFFFFFu, qflags = vf >> 24;
+        // a camera ray (vertex 1) has no sampling record and no radiance yet:
+        // the generate step writes neither
+        const float4 e4 = v > 1 ? qin.e[i] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         const float pdf_prev = o4.w, pe_prev = e4.w;
         const f3 d = xyz(d4), meval = xyz(e4);
         f3 weight = xyz(w4);
-        f3 rad = xyz(radiance[id]);
+        f3 rad = v > 1 ? xyz(radiance[id]) : splat(0.0f);
         const f3 pend = (qflags & QF_PENDING) ? xyz(pending[id]) : splat(0.0f);
         if (GUIDE && gr.recs) {
             // the vertex that sampled this ray: radiance up to and including its light sample
@@ -887,12 +890,10 @@ __global__ __launch_bounds__(256) void k_generate_multi(MultiBatch mb, PathQueue
         f3 dc = normalize(mk(wp.tan_x * (fx - 0.5f) * 2, wp.tan_y * (fy - 0.5f) * 2, 1));
         f3 o = xf_point(wp.cam, mk(0, 0, 0));
         f3 d = xf_vector(wp.cam, dc);
-        q.o[id] = make_float4(o.x, o.y, o.z, 0.0f);
+        q.o[id] = make_float4(o.x, o.y, o.z, 0.0f);   // e and radiance: implied by vertex 1, as above
         q.d[id] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
         q.w[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));
-        q.e[id] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         mb.item_of[id] = k;
-        radiance[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
