@@ -113,7 +113,8 @@ def test_autotune_changes_shapes_not_results(dev):
     ref = sc.render(128, 128, 8, **kw)
     tb, sb = sc.autotune(128, 128, 8, **kw)
     print(f"[gpu] autotune: {tb} traversal blocks/CU, {sb} shade blocks/CU", flush=True)
-    assert tb in (2, 4, 8) and sb in (2, 4, 8)
+    # candidates 2, 4, 8 traversal blocks per CU, the last clamped to what stays resident
+    assert (tb in (2, 4) or 5 <= tb <= 8) and sb in (2, 4, 8)
     assert np.array_equal(ref, sc.render(128, 128, 8, **kw))
     films, _ = api.render_batch([(sc, 128, 128, 8, None, kw)])
     assert np.array_equal(ref, films[0])

@@ -123,3 +123,23 @@ def test_many_batches_large_image(dev):
     sub = o.render(2048, 2048, 2, seed=1, max_depth=4)
     assert rel_mse(f / 2, sub / 2) <= 1e-4
     assert np.mean(np.all(f == sub, axis=2)) >= 0.999
+
+
+@pytest.mark.gpu
+def test_large_render_with_chunked_traversal_matches_smaller_batches():
+    """A render whose path queue holds at least 8 Mi paths traces its rays in
+    chunks handed out from a cursor (extension and shadow kernels); the same
+    samples rendered in two calls below that size trace in even per-wave
+    shares.  Every path's radiance is the same either way, so the FP64 films
+    agree to the rounding of summing a pixel's samples in two parts."""
+    desc = scenes.heightfield(cells=256)
+    sc = api.Scene(api.Device(0), desc)
+    W = H = 1024
+    kw = dict(seed=5, max_depth=8, flags=api.XT_FLAG_ENV_NEE)
+    print("[gpu] chunked render: 1024^2 x 8 spp vs 2 x 4 spp", flush=True)
+    big = sc.render(W, H, 8, f64=True, **kw)                  # 8,388,608 paths in one batch
+    parts = np.zeros((H, W, 3), np.float64)
+    sc.render(W, H, 4, film=parts, spp_offset=0, **kw)
+    sc.render(W, H, 4, film=parts, spp_offset=4, **kw)
+    assert np.isfinite(big).all() and big.mean() > 0
+    np.testing.assert_allclose(big, parts, rtol=1e-9, atol=1e-9)

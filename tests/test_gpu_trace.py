@@ -177,3 +177,30 @@ def test_ray_batches_at_full_size_match_oracle(dev):
             assert np.array_equal(hit_prims(got), hit_prims(o.trace_closest(rays, mode=0)))
             assert st["extension_rays"] == len(rays) and st["ms_extend"] > 0 and st["ext_node_fetches"] >= len(rays)
         g.set_counting(False)
+
+
+def test_large_queries_take_rays_in_chunks(dev):
+    """A query of at least 8 Mi rays hands its rays out in chunks from a cursor
+    (a wave whose rays end early takes more), smaller ones in an even share
+    per wave: each ray's hit and occlusion bit are the same either way, bit
+    for bit, and the same large query twice in a row (the cursor resets
+    itself at the end of a launch) answers the same."""
+    desc = scenes.heightfield(cells=256, materials=False)
+    g = api.Scene(dev, desc)
+    rays = camera_rays(desc, 3072, 2816)          # 8,650,752 rays, distinct per pixel
+    assert len(rays) >= 8 << 20
+    print(f"[gpu] chunked query: {len(rays)} rays", flush=True)
+    half = len(rays) // 2                          # each half below the chunked size
+    bits = lambda h: h.view(np.uint32)             # misses carry prim bits 0xFFFFFFFF (a NaN as float)
+    h_big = g.trace_closest(rays)
+    h_parts = np.concatenate([g.trace_closest(rays[:half]), g.trace_closest(rays[half:])])
+    assert np.array_equal(bits(h_big), bits(h_parts))
+    assert np.array_equal(bits(g.trace_closest(rays)), bits(h_big))
+    hit = hit_prims(h_big) != 0xFFFFFFFF
+    assert 0.3 < hit.mean() < 1.0
+    r2 = rays.copy()
+    r2[:, 3] = np.where(hit, h_big[:, 0] * np.float32(0.5), np.float32(np.inf))
+    o_big = g.trace_any(r2)
+    o_parts = np.concatenate([g.trace_any(r2[:half]), g.trace_any(r2[half:])])
+    assert np.array_equal(o_big, o_parts)
+    assert np.array_equal(g.trace_any(r2), o_big)

@@ -19,6 +19,9 @@ namespace xt {
 constexpr int SHADE_BLOCK = XT_SHADE_BLOCK;          // threads per block of the shade kernel
 constexpr int TRACE_BLOCK = XT_TRACE_BLOCK;          // lanes per block of the traversal kernels
 constexpr int TRACE_LDS_DEPTH = XT_TRACE_LDS_DEPTH;  // LDS stack entries per lane
+// queue length (an upper bound the host knows) from which the traversal
+// kernels hand out rays in chunks from a cursor (DYN, xt_traverse.h)
+constexpr uint32_t TRAV_DYN_MIN = 8u << 20;
 
 // Where the current batch lies in the render: pixel range, paths, first
 // sample.  Host-driven renders pass it in WaveParams; graph replays read it
@@ -165,14 +168,16 @@ struct MultiBatch {
     uint32_t* units;        // window plan: accumulation units before each overlapping item (num_overlap + 1)
 };
 
-template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI = false>
+template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI = false, bool DYN = false>
 __global__ void k_trace(DevScene sc, const float4* ray_o, const float4* ray_d, int ray_stride,
                         const uint32_t* count_ptr, uint32_t count_fixed, float4* hits, uint8_t* occluded,
-                        int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, unsigned long long* visit_totals,
+                        int hit_prim_ids, int tmax_from_ray, uint32_t* ovf, uint32_t* dyn,
+                        unsigned long long* visit_totals,
                         uint32_t* clear_a, uint32_t* clear_b, const uint32_t* item_of = nullptr,
                         const MultiItem* items = nullptr);
+template <bool DYN>
 __global__ void k_shadow_multi(ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending, uint32_t* ovf,
-                               const uint32_t* item_of, const MultiItem* items);
+                               uint32_t* dyn, const uint32_t* item_of, const MultiItem* items);
 __global__ void k_window_init(MultiBatch mb);
 __global__ void k_window_plan(MultiBatch mb);
 __global__ void k_window_advance(MultiBatch mb);
@@ -189,9 +194,9 @@ template <bool GUIDE, bool NMAP, bool SORT>
 __global__ void k_shade(WaveParams wp, const DevScene* scene, PathQueue qin, const uint32_t* count_in, const float4* hits,
                         PathQueue qout, uint32_t* count_out, ShadowQueue sq, uint32_t* count_shadow,
                         float4* radiance, float4* pending, GuideRender gr);
-template <bool COUNT>
+template <bool COUNT, bool DYN = false>
 __global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, float4* radiance, float4* pending,
-                         uint32_t* ovf, unsigned long long* visit_totals, const uint32_t* counts3,
+                         uint32_t* ovf, uint32_t* dyn, unsigned long long* visit_totals, const uint32_t* counts3,
                          uint32_t* counts_out);
 __global__ void k_accumulate(WaveParams wp, const float4* radiance, void* film, int film_f64);
 __global__ void k_zero_u32(uint32_t* p, uint32_t n);

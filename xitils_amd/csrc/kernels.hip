@@ -79,12 +79,13 @@ __device__ __forceinline__ void flush_counts(unsigned long long* totals, const V
 // occluded[i].  Persistent waves with in-place lane refill (xt_traverse.h).
 // MULTI: the rays of a unified batch's queue, each traced in its own item's
 // scene (item_of[path id] -> the item's nodes and triangles, per lane).
-template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI>
+template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI, bool DYN>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
                                                         const float4* __restrict__ ray_d, int ray_stride,
                                                         const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
                                                         float4* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                         int hit_prim_ids, int tmax_from_ray, uint32_t* __restrict__ ovf,
+                                                        uint32_t* __restrict__ dyn,
                                                         unsigned long long* __restrict__ visit_totals,
                                                         uint32_t* __restrict__ clear_a, uint32_t* __restrict__ clear_b,
                                                         const uint32_t* __restrict__ item_of,
@@ -104,9 +105,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
     uint32_t* my_lds = lds + threadIdx.x;
     uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
     const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
-    const uint32_t per = (n + nwaves - 1) / nwaves;
-    uint32_t next = min(wave * per, n);
-    const uint32_t end = min(next + per, n);
+    RayChunks ch = chunks_first<DYN>(wave, nwaves, n);
     const uint64_t below = (1ull << __lane_id()) - 1ull;
     VisitCount vc;
     bool active = false;
@@ -117,9 +116,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
     while (true) {
         const uint64_t busy = __ballot(active);
         const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
-        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
-            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
-            if (!active && my < end) {
+        const bool refill = nidle >= (uint32_t)REFILL_IDLE || busy == 0;
+        if (DYN && refill && ch.next >= ch.end && !ch.exhausted) chunks_refill(ch, dyn);
+        if (refill && ch.next < ch.end) {
+            const uint32_t my = ch.next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < ch.end) {
                 rid = my;
                 float4 o4 = ray_o[(size_t)rid * ray_stride];
                 float4 d4 = ray_d[(size_t)rid * ray_stride];
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
                 trav_init(t, tmax_from_ray ? o4.w : INFINITY);   // extension rays: tMax = Infinity (PathTracer.h:167)
                 active = true;
             }
-            next = min(next + nidle, end);
+            ch.next = min(ch.next + nidle, ch.end);
         }
         if (__ballot(active) == 0) break;
         bool occ = false;
@@ -155,18 +156,24 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
             }
         }
     }
+    if constexpr (DYN) chunks_release(dyn);
     if (COUNT) flush_counts(visit_totals, vc);
 }
 
-#define XT_INST_TRACE(ANY, COUNT, MULTI)                                                                           \
-    template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT, MULTI>(                                          \
+#define XT_INST_TRACE(ANY, COUNT, MULTI, DYN)                                                                      \
+    template __global__ void k_trace<ANY, TRACE_LDS_DEPTH, COUNT, MULTI, DYN>(                                     \
         DevScene, const float4*, const float4*, int, const uint32_t*, uint32_t, float4*, uint8_t*, int, int,       \
-        uint32_t*, unsigned long long*, uint32_t*, uint32_t*, const uint32_t*, const MultiItem*);
-XT_INST_TRACE(false, false, false)
-XT_INST_TRACE(true, false, false)
-XT_INST_TRACE(false, true, false)
-XT_INST_TRACE(true, true, false)
-XT_INST_TRACE(false, false, true)
+        uint32_t*, uint32_t*, unsigned long long*, uint32_t*, uint32_t*, const uint32_t*, const MultiItem*);
+XT_INST_TRACE(false, false, false, false)
+XT_INST_TRACE(true, false, false, false)
+XT_INST_TRACE(false, true, false, false)
+XT_INST_TRACE(true, true, false, false)
+XT_INST_TRACE(false, false, true, false)
+XT_INST_TRACE(false, false, true, true)
+XT_INST_TRACE(false, false, false, true)
+XT_INST_TRACE(true, false, false, true)
+XT_INST_TRACE(false, true, false, true)
+XT_INST_TRACE(true, true, false, true)
 #undef XT_INST_TRACE
 
 // ---------------------------------------------------------------- wavefront PT
@@ -659,10 +666,10 @@ XT_INST_SHADE(true, true, true)
 
 // Any-hit for queued shadow rays; unoccluded contributions go straight to the
 // path radiance, or to pending[] when the path continues (see k_shade).
-template <bool COUNT>
+template <bool COUNT, bool DYN>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         float4* __restrict__ radiance, float4* __restrict__ pending,
-                                                        uint32_t* __restrict__ ovf,
+                                                        uint32_t* __restrict__ ovf, uint32_t* __restrict__ dyn,
                                                         unsigned long long* __restrict__ visit_totals,
                                                         const uint32_t* __restrict__ counts3,
                                                         uint32_t* __restrict__ counts_out) {
@@ -678,9 +685,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
     uint32_t* my_lds = lds + threadIdx.x;
     uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
     const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
-    const uint32_t per = (n + nwaves - 1) / nwaves;
-    uint32_t next = min(wave * per, n);
-    const uint32_t end = min(next + per, n);
+    RayChunks ch = chunks_first<DYN>(wave, nwaves, n);
     const uint64_t below = (1ull << __lane_id()) - 1ull;
     VisitCount vc;
     bool active = false;
@@ -690,16 +695,18 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
     while (true) {
         const uint64_t busy = __ballot(active);
         const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
-        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
-            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
-            if (!active && my < end) {
+        const bool refill = nidle >= (uint32_t)REFILL_IDLE || busy == 0;
+        if (DYN && refill && ch.next >= ch.end && !ch.exhausted) chunks_refill(ch, dyn);
+        if (refill && ch.next < ch.end) {
+            const uint32_t my = ch.next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < ch.end) {
                 rid = my;
                 float4 o4 = sq.o[rid], d4 = sq.d[rid];
                 r = ray_setup(xyz(o4), xyz(d4));
                 trav_init(t, o4.w);
                 active = true;
             }
-            next = min(next + nidle, end);
+            ch.next = min(ch.next + nidle, ch.end);
         }
         if (__ballot(active) == 0) break;
         bool occ = false;
@@ -721,12 +728,17 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue
             }
         }
     }
+    if constexpr (DYN) chunks_release(dyn);
     if (COUNT) flush_counts(visit_totals, vc);
 }
-template __global__ void k_shadow<false>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
-                                         unsigned long long*, const uint32_t*, uint32_t*);
-template __global__ void k_shadow<true>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*,
-                                        unsigned long long*, const uint32_t*, uint32_t*);
+#define XT_INST_SHADOW(COUNT, DYN)                                                                               \
+    template __global__ void k_shadow<COUNT, DYN>(DevScene, ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*, \
+                                                  uint32_t*, unsigned long long*, const uint32_t*, uint32_t*);
+XT_INST_SHADOW(false, false)
+XT_INST_SHADOW(true, false)
+XT_INST_SHADOW(false, true)
+XT_INST_SHADOW(true, true)
+#undef XT_INST_SHADOW
 
 // film[pixel] += radiance of each of the batch's samples, in sample order.
 __global__ __launch_bounds__(256) void k_accumulate(WaveParams wp, const float4* __restrict__ radiance, void* film,
@@ -940,9 +952,10 @@ template __global__ void k_shade_multi<true>(MultiBatch, PathQueue, const uint32
                                              uint32_t*, ShadowQueue, uint32_t*, float4*, float4*);
 
 // Any-hit for a unified batch's shadow queue, each ray in its item's scene.
+template <bool DYN>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
                                                               float4* __restrict__ radiance, float4* __restrict__ pending,
-                                                              uint32_t* __restrict__ ovf,
+                                                              uint32_t* __restrict__ ovf, uint32_t* __restrict__ dyn,
                                                               const uint32_t* __restrict__ item_of,
                                                               const MultiItem* __restrict__ items) {
     __shared__ uint32_t lds[TRACE_LDS_DEPTH * TRACE_BLOCK];
@@ -952,9 +965,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, co
     uint32_t* my_lds = lds + threadIdx.x;
     uint32_t* my_ovf = ovf ? ovf + gtid : nullptr;
     const uint32_t nwaves = (uint32_t)(nthreads / 64), wave = (uint32_t)(gtid / 64);
-    const uint32_t per = (n + nwaves - 1) / nwaves;
-    uint32_t next = min(wave * per, n);
-    const uint32_t end = min(next + per, n);
+    RayChunks ch = chunks_first<DYN>(wave, nwaves, n);
     const uint64_t below = (1ull << __lane_id()) - 1ull;
     VisitCount vc;
     bool active = false;
@@ -965,9 +976,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, co
     while (true) {
         const uint64_t busy = __ballot(active);
         const uint32_t nidle = 64u - (uint32_t)__popcll(busy);
-        if (next < end && (nidle >= (uint32_t)REFILL_IDLE || busy == 0)) {
-            const uint32_t my = next + (uint32_t)__popcll(~busy & below);
-            if (!active && my < end) {
+        const bool refill = nidle >= (uint32_t)REFILL_IDLE || busy == 0;
+        if (DYN && refill && ch.next >= ch.end && !ch.exhausted) chunks_refill(ch, dyn);
+        if (refill && ch.next < ch.end) {
+            const uint32_t my = ch.next + (uint32_t)__popcll(~busy & below);
+            if (!active && my < ch.end) {
                 rid = my;
                 float4 o4 = sq.o[rid], d4 = sq.d[rid];
                 id = __float_as_uint(d4.w);
@@ -978,7 +991,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, co
                 trav_init(t, o4.w);
                 active = true;
             }
-            next = min(next + nidle, end);
+            ch.next = min(ch.next + nidle, ch.end);
         }
         if (__ballot(active) == 0) break;
         bool occ = false;
@@ -997,7 +1010,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, co
             }
         }
     }
+    if constexpr (DYN) chunks_release(dyn);
 }
+template __global__ void k_shadow_multi<false>(ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*, uint32_t*,
+                                               const uint32_t*, const MultiItem*);
+template __global__ void k_shadow_multi<true>(ShadowQueue, const uint32_t*, float4*, float4*, uint32_t*, uint32_t*,
+                                              const uint32_t*, const MultiItem*);
 
 // film[pixel] += the window's samples of that pixel, in sample order, for
 // every item the window overlaps (one thread per item pixel).

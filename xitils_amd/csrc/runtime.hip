@@ -50,12 +50,13 @@ struct RenderBuffers {
     uint32_t* host_counts_dev = nullptr;  // the same memory's device address (written by k_shadow)
     hipEvent_t round_ev[RB_RING] = {};
     unsigned long long* visits = nullptr;  // node, tri fetches: extension [0..1], shadow [2..3]
+    uint32_t* dyn = nullptr;               // chunk cursors of the traversal kernels: extension [0..1], shadow [2..3]
     void release() {
         for (int k = 0; k < 2; ++k) {
             hipFree(q[k].o); hipFree(q[k].d); hipFree(q[k].w); hipFree(q[k].e);
         }
         hipFree(sq.o); hipFree(sq.d); hipFree(sq.c);
-        hipFree(hits); hipFree(radiance); hipFree(pending); hipFree(counts); hipFree(visits);
+        hipFree(hits); hipFree(radiance); hipFree(pending); hipFree(counts); hipFree(visits); hipFree(dyn);
         hipFree(rec[0]); hipFree(rec[1]); hipFree(lobes);
         if (host_counts) hipHostFree(host_counts);
         for (hipEvent_t& e : round_ev)
@@ -167,6 +168,7 @@ struct xt_scene {
     uint8_t* d_qocc = nullptr;
     size_t qout_cap = 0;
     unsigned long long* d_qvisits = nullptr;   // node, tri fetches of the last counted ray query
+    uint32_t* d_qdyn = nullptr;                // chunk cursor of the ray-query traversal
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
     GuideState guide;
 };
@@ -246,7 +248,7 @@ xt_status xt_scene_destroy(xt_scene* s) {
     hipFree(s->d_nmap_attr); hipFree(s->d_textures);
     for (float4* t : s->d_texels) hipFree(t);
     hipFree(s->d_light_pdf); hipFree(s->d_light_objs); hipFree(s->d_light_slots); hipFree(s->d_ovf);
-    hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc); hipFree(s->d_qvisits);
+    hipFree(s->d_qrays); hipFree(s->d_qhits); hipFree(s->d_qocc); hipFree(s->d_qvisits); hipFree(s->d_qdyn);
     hipFree(s->d_map); hipFree(s->d_map_pdf); hipFree(s->d_map_marg); hipFree(s->d_map_cond);
     for (RenderGraph& g : s->graphs) hipGraphExecDestroy(g.exec);
     hipFree(s->d_cursor);
@@ -723,6 +725,10 @@ static xt_status query(xt_scene* s, const xt_ray* rays, uint32_t n, void* out, i
         d_out = any ? (void*)s->d_qocc : (void*)s->d_qhits;
     }
     const bool count = s->counting != 0;
+    if (!s->d_qdyn) {
+        XT_HIP(dalloc(&s->d_qdyn, 2));
+        XT_HIP(hipMemset(s->d_qdyn, 0, 2 * sizeof(uint32_t)));
+    }
     if (count) {
         if (!s->d_qvisits) XT_HIP(dalloc(&s->d_qvisits, 2));
         XT_HIP(hipMemsetAsync(s->d_qvisits, 0, 2 * sizeof(unsigned long long), s->stream));
@@ -737,9 +743,13 @@ static xt_status query(xt_scene* s, const xt_ray* rays, uint32_t n, void* out, i
     float4* hits = any ? nullptr : (float4*)d_out;
     uint8_t* occ = any ? (uint8_t*)d_out : nullptr;
     hipEventRecord(ev[0], s->stream);
-#define XT_QUERY(ANY, COUNT)                                                                                     \
-    hipLaunchKernelGGL((k_trace<ANY, TRACE_LDS_DEPTH, COUNT>), dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, \
-                       d_rays, d_rays + 1, 2, nullptr, n, hits, occ, 1, 1, s->d_ovf, s->d_qvisits, nullptr, nullptr)
+    const bool dyn = n >= TRAV_DYN_MIN;
+#define XT_QUERY(ANY, COUNT)                                                                                      \
+    hipLaunchKernelGGL((dyn ? k_trace<ANY, TRACE_LDS_DEPTH, COUNT, false, true>                                    \
+                            : k_trace<ANY, TRACE_LDS_DEPTH, COUNT, false, false>),                                 \
+                       dim3(blocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,                                      \
+                       d_rays, d_rays + 1, 2, nullptr, n, hits, occ, 1, 1, s->d_ovf, s->d_qdyn, s->d_qvisits, nullptr, \
+                       nullptr, nullptr, nullptr)
     if (any && count) {
         XT_QUERY(true, true);
     } else if (any) {
@@ -811,6 +821,8 @@ static xt_status alloc_buffers(RenderBuffers& rb, uint32_t capacity) {
     XT_HIP(dalloc(&rb.pending, c));
     XT_HIP(dalloc(&rb.counts, 4));
     XT_HIP(dalloc(&rb.visits, 4));
+    XT_HIP(dalloc(&rb.dyn, 4));
+    XT_HIP(hipMemset(rb.dyn, 0, 4 * sizeof(uint32_t)));
     XT_HIP(hipHostMalloc((void**)&rb.host_counts, RB_RING * 4 * sizeof(uint32_t),
                          hipHostMallocMapped | hipHostMallocCoherent));
     XT_HIP(hipHostGetDevicePointer((void**)&rb.host_counts_dev, rb.host_counts, 0));
@@ -1227,16 +1239,17 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                 {
                     const int slot = round % RB_RING;
                     a = tm.mark();
-                    if (s->counting)
-                        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, true>), dim3(trace_blocks(s, bound)),
-                                           dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, rb.visits,
-                                           rb.counts + (1 - cur), rb.counts + 2);
-                    else
-                        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(trace_blocks(s, bound)),
-                                           dim3(TRACE_BLOCK), 0, s->stream, s->ds, rb.q[cur].o, rb.q[cur].d, 1,
-                                           rb.counts + cur, 0u, rb.hits, nullptr, 0, 0, s->d_ovf, nullptr,
-                                           rb.counts + (1 - cur), rb.counts + 2);   // also clears the next counters
+                    // large queues: rays handed out in chunks from a cursor (the
+                    // round's shadow queue is no longer than its path queue)
+                    const bool dyn = bound >= TRAV_DYN_MIN;
+                    auto trace = s->counting ? (dyn ? k_trace<false, TRACE_LDS_DEPTH, true, false, true>
+                                                    : k_trace<false, TRACE_LDS_DEPTH, true, false, false>)
+                                             : (dyn ? k_trace<false, TRACE_LDS_DEPTH, false, false, true>
+                                                    : k_trace<false, TRACE_LDS_DEPTH, false, false, false>);
+                    hipLaunchKernelGGL(trace, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
+                                       rb.q[cur].o, rb.q[cur].d, 1, rb.counts + cur, 0u, rb.hits, nullptr, 0, 0,
+                                       s->d_ovf, rb.dyn, s->counting ? rb.visits : nullptr, rb.counts + (1 - cur),
+                                       rb.counts + 2, nullptr, nullptr);   // also clears the next counters
                     XT_LAUNCHED(result);
                     tm.add(0, a, tm.mark());
                     GuideRender gr{};
@@ -1269,14 +1282,12 @@ xt_status xt_render(xt_scene* s, const xt_camera* cam, const xt_render_params* p
                     XT_LAUNCHED(result);
                     tm.add(1, a, tm.mark());
                     a = tm.mark();
-                    if (s->counting)
-                        hipLaunchKernelGGL(k_shadow<true>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
-                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, rb.visits + 2,
-                                           rb.counts, rb.host_counts_dev + 4 * slot);
-                    else
-                        hipLaunchKernelGGL(k_shadow<false>, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream,
-                                           s->ds, rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, nullptr,
-                                           rb.counts, rb.host_counts_dev + 4 * slot);   // counts -> pinned ring
+                    auto shadow = s->counting ? (dyn ? k_shadow<true, true> : k_shadow<true, false>)
+                                              : (dyn ? k_shadow<false, true> : k_shadow<false, false>);
+                    hipLaunchKernelGGL(shadow, dim3(trace_blocks(s, bound)), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
+                                       rb.sq, rb.counts + 2, rb.radiance, rb.pending, s->d_ovf, rb.dyn + 2,
+                                       s->counting ? rb.visits + 2 : nullptr, rb.counts,
+                                       rb.host_counts_dev + 4 * slot);   // counts -> pinned ring
                     XT_LAUNCHED(result);
                     tm.add(2, a, tm.mark());
                     if (hipEventRecord(rb.round_ev[slot], s->stream) != hipSuccess) {
@@ -1495,7 +1506,7 @@ static void enqueue_cursor_batch(xt_scene* s, const xt_render_params* p, const W
     int cur = 0;
     for (int it = 0; it <= p->max_depth; ++it) {
         hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false>), dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds,
-                           b.q[cur].o, b.q[cur].d, 1, b.counts + cur, 0u, b.hits, nullptr, 0, 0, b.ovf, nullptr,
+                           b.q[cur].o, b.q[cur].d, 1, b.counts + cur, 0u, b.hits, nullptr, 0, 0, b.ovf, nullptr, nullptr,
                            b.counts + (1 - cur), b.counts + 2);
         GuideRender gr{};
         if (guide) {
@@ -1511,7 +1522,7 @@ static void enqueue_cursor_batch(xt_scene* s, const xt_render_params* p, const W
                            b.q[cur], b.counts + cur, b.hits, b.q[1 - cur], b.counts + (1 - cur), b.sq,
                            b.counts + 2, b.radiance, b.pending, gr);
         hipLaunchKernelGGL(k_shadow<false>, dim3(tblocks), dim3(TRACE_BLOCK), 0, s->stream, s->ds, b.sq, b.counts + 2,
-                           b.radiance, b.pending, b.ovf, nullptr, b.counts, nullptr);
+                           b.radiance, b.pending, b.ovf, nullptr, nullptr, b.counts, nullptr);
         cur = 1 - cur;
     }
     hipLaunchKernelGGL(k_accumulate, dim3(ablocks), dim3(256), 0, s->stream, wp, b.radiance, d_film, f64);
@@ -1621,6 +1632,7 @@ struct MultiState {
     float4* radiance = nullptr;
     float4* pending = nullptr;
     uint32_t* counts = nullptr;
+    uint32_t* dyn = nullptr;           // traversal chunk cursors: extension [0..1], shadow [2..3]
     uint32_t* item_of = nullptr;
     uint32_t* ovf = nullptr;
     size_t ovf_entries = 0;
@@ -1652,6 +1664,7 @@ static void multi_release(xt_device* dev) {
     if (m->stream) hipStreamSynchronize(m->stream);
     m->release_paths();
     hipFree(m->counts); hipFree(m->ovf); hipFree(m->cursor); hipFree(m->items); hipFree(m->scenes); hipFree(m->units);
+    hipFree(m->dyn);
     for (hipEvent_t e : m->events) hipEventDestroy(e);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
@@ -1703,6 +1716,8 @@ static xt_status render_unified(xt_batch_item* items, const std::vector<uint32_t
         XT_HIP(hipStreamCreateWithFlags(&dev->multi->stream, hipStreamNonBlocking));
         XT_HIP(dalloc(&dev->multi->counts, 4));
         XT_HIP(dalloc(&dev->multi->cursor, 1));
+        XT_HIP(dalloc(&dev->multi->dyn, 4));
+        XT_HIP(hipMemset(dev->multi->dyn, 0, 4 * sizeof(uint32_t)));
     }
     MultiState& m = *dev->multi;
     // item and scene tables (items without samples are left out)
@@ -1804,16 +1819,20 @@ static xt_status render_unified(xt_batch_item* items, const std::vector<uint32_t
     const uint32_t ncu = (uint32_t)dev->num_cu;
     const uint32_t gblocks = std::max(1u, std::min((window + 255u) / 256u, ncu * 8));
     const uint32_t sblocks = std::max(1u, std::min((window + SHADE_BLOCK - 1) / SHADE_BLOCK, s0->shade_grid));
-    // capture one window's bounce loop
+    // capture one window's bounce loop (windows of many paths hand the
+    // traversal kernels' rays out in chunks from a cursor)
+    const bool dyn = window >= TRAV_DYN_MIN;
     XT_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_window_plan, dim3(1), dim3(1), 0, m.stream, mb);
     hipLaunchKernelGGL(k_count_from_window, dim3(1), dim3(1), 0, m.stream, m.counts, m.cursor);
     hipLaunchKernelGGL(k_generate_multi, dim3(gblocks), dim3(256), 0, m.stream, mb, m.q[0], m.radiance);
     int cur = 0;
     for (int it = 0; it <= max_depth; ++it) {
-        hipLaunchKernelGGL((k_trace<false, TRACE_LDS_DEPTH, false, true>), dim3(tblocks), dim3(TRACE_BLOCK), 0,
-                           m.stream, s0->ds, m.q[cur].o, m.q[cur].d, 1, m.counts + cur, 0u, m.hits, nullptr, 0, 0,
-                           m.ovf, nullptr, m.counts + (1 - cur), m.counts + 2, m.item_of, m.items);
+        hipLaunchKernelGGL((dyn ? k_trace<false, TRACE_LDS_DEPTH, false, true, true>
+                                : k_trace<false, TRACE_LDS_DEPTH, false, true, false>),
+                           dim3(tblocks), dim3(TRACE_BLOCK), 0, m.stream, s0->ds, m.q[cur].o, m.q[cur].d, 1,
+                           m.counts + cur, 0u, m.hits, nullptr, 0, 0, m.ovf, m.dyn, nullptr, m.counts + (1 - cur),
+                           m.counts + 2, m.item_of, m.items);
         if (nmap)
             hipLaunchKernelGGL(k_shade_multi<true>, dim3(sblocks), dim3(SHADE_BLOCK), 0, m.stream, mb, m.q[cur],
                                m.counts + cur, m.hits, m.q[1 - cur], m.counts + (1 - cur), m.sq, m.counts + 2,
@@ -1822,8 +1841,8 @@ static xt_status render_unified(xt_batch_item* items, const std::vector<uint32_t
             hipLaunchKernelGGL(k_shade_multi<false>, dim3(sblocks), dim3(SHADE_BLOCK), 0, m.stream, mb, m.q[cur],
                                m.counts + cur, m.hits, m.q[1 - cur], m.counts + (1 - cur), m.sq, m.counts + 2,
                                m.radiance, m.pending);
-        hipLaunchKernelGGL(k_shadow_multi, dim3(tblocks), dim3(TRACE_BLOCK), 0, m.stream, m.sq, m.counts + 2,
-                           m.radiance, m.pending, m.ovf, m.item_of, m.items);
+        hipLaunchKernelGGL((dyn ? k_shadow_multi<true> : k_shadow_multi<false>), dim3(tblocks), dim3(TRACE_BLOCK), 0,
+                           m.stream, m.sq, m.counts + 2, m.radiance, m.pending, m.ovf, m.dyn + 2, m.item_of, m.items);
         cur = 1 - cur;
     }
     hipLaunchKernelGGL(k_accumulate_multi, dim3(gblocks), dim3(256), 0, m.stream, mb, m.radiance);

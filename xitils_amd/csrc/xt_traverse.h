@@ -207,11 +207,77 @@ struct VisitCount {
 
 // ---------------------------------------------------------------- persistent traversal
 //
-// A wave owns the contiguous ray range [begin, end) and keeps its 64 lanes
-// busy: whenever at least REFILL_IDLE lanes have finished their ray, the idle
-// lanes take the next rays of the range (a wave-uniform cursor, no atomics).
+// A wave keeps its 64 lanes busy: whenever at least REFILL_IDLE lanes have
+// finished their ray, the idle lanes take the next rays of the wave's chunk
+// (RayChunks below).
 // Each loop iteration advances every busy lane by one node visit or one leaf.
 constexpr int REFILL_IDLE = 16;   // 8 and 32 measured no better
+
+// Which rays a wave traces.  DYN = false: an even contiguous share per wave.
+// DYN = true (large queues): the queue is cut into chunks of a quarter of an
+// even share per wave (64..TRAV_CHUNK rays); wave w first takes chunk w,
+// then, each time its chunk runs out while lanes wait for rays, the next
+// chunk from a cursor in device memory (one atomic per chunk), so waves whose
+// rays end early take work from the rest instead of idling to the end of the
+// launch.  The last block to finish resets the cursor and its own block
+// counter for the next launch that uses them (dyn[0], dyn[1]; zero before the
+// first).  The cursor costs registers (7 instead of 8 waves per SIMD), which
+// small queues do not win back, so the host picks DYN from the queue size
+// (TRAV_DYN_MIN).  A ray's result does not depend on which wave traces it.
+constexpr uint32_t TRAV_CHUNK = 512;
+struct RayChunks {
+    uint32_t next, end;   // this wave's untaken rays [next, end)
+    uint32_t base;        // first ray handed out by the cursor
+    uint32_t chunk;       // rays per chunk
+    uint32_t n;
+    bool exhausted;
+};
+template <bool DYN>
+__device__ __forceinline__ RayChunks chunks_first(uint32_t wave, uint32_t nwaves, uint32_t n) {
+    RayChunks c;
+    if constexpr (!DYN) {
+        const uint32_t per = (n + nwaves - 1) / nwaves;
+        c.chunk = per;
+        c.next = min(wave * per, n);
+        c.end = min(c.next + per, n);
+        c.base = n;
+        c.n = n;
+        c.exhausted = true;
+        return c;
+    }
+    const uint64_t quarter = ((uint64_t)n + 4ull * nwaves - 1) / (4ull * nwaves);
+    c.chunk = (uint32_t)min<uint64_t>(max<uint64_t>(quarter, 64), TRAV_CHUNK);
+    const uint64_t b = (uint64_t)wave * c.chunk;
+    c.next = (uint32_t)min<uint64_t>(b, n);
+    c.end = (uint32_t)min<uint64_t>(b + c.chunk, n);
+    c.base = (uint32_t)min<uint64_t>((uint64_t)nwaves * c.chunk, n);
+    c.n = n;
+    c.exhausted = c.base >= n;   // no chunk is left for the cursor
+    return c;
+}
+// Wave-uniform: the whole wave calls it, with every lane active.
+__device__ __forceinline__ void chunks_refill(RayChunks& c, uint32_t* dyn) {
+    uint32_t got = 0;
+    if (__lane_id() == 0) got = atomicAdd(&dyn[0], c.chunk);
+    const uint64_t g = (uint64_t)__builtin_amdgcn_readfirstlane(got) + c.base;
+    if (g >= c.n) {
+        c.exhausted = true;
+    } else {
+        c.next = (uint32_t)g;
+        c.end = (uint32_t)min<uint64_t>(g + c.chunk, c.n);
+    }
+}
+// Every thread of the block calls it once its wave has left the loop.
+__device__ __forceinline__ void chunks_release(uint32_t* dyn) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&dyn[1], 1u) == gridDim.x - 1) {
+            atomicExch(&dyn[0], 0u);
+            atomicExch(&dyn[1], 0u);
+        }
+    }
+}
 
 
 struct Trav {
