@@ -102,10 +102,21 @@ def traffic_from_profiles(kernel_tag, name=None):
     return pmc_from_profiles(name or f"{kernel_tag}_pmc.txt").get("hbm_bytes_per_launch")
 
 
-# VALU issue peak: one wave-level VALU instruction per CU per cycle (a wave64
-# instruction occupies one of the four 16-lane SIMDs for 4 cycles), 256 CUs at
-# 2.4 GHz.
-VALU_ISSUE_PEAK = 256 * 2.4e9
+# VALU issue peak (wave-level instructions per second): the published fp32
+# vector peak, 157.3 TFLOP/s = 256 CUs x 2.4 GHz x 128 FMA lanes, is two wave64
+# instructions per CU per cycle.  xt_device_valu_rate's probe (16 independent
+# FMA chains per lane) is reported beside it: it issues about 0.75 of this,
+# well above one instruction per CU per cycle.
+VALU_ISSUE_PEAK = 2 * 256 * 2.4e9
+_valu_probe = None
+
+
+def valu_probe_rate():
+    global _valu_probe
+    if _valu_probe is None:
+        from xitils_amd import api
+        _valu_probe = api.Device(0).valu_rate()
+    return _valu_probe
 
 
 def kernel_roofline(kernel, algo_bytes_per_launch, avg_launch_ms, pmc, extra=None):
@@ -115,9 +126,10 @@ def kernel_roofline(kernel, algo_bytes_per_launch, avg_launch_ms, pmc, extra=Non
     BVH; shade: the path-state records it reads and writes) over the measured
     launch time, against the 8 TB/s HBM peak.  The bytes that really
     cross HBM (rocprof PMC) and the VALU issue rate (SQ_INSTS_VALU) are
-    reported beside it: where the first is well below the algorithmic figure
-    (the fetches hit in L2 / MALL) and the second is near its peak, the
-    kernel is bound by instruction issue, and `bound` says so."""
+    reported beside it, and `bound` names what limits the kernel: "hbm" or
+    "valu-issue" when that measured rate is at 70% of its peak or more, else
+    "latency" (neither unit saturated: the kernel waits on its dependent
+    loads and tests)."""
     sec = avg_launch_ms * 1e-3
     achieved = algo_bytes_per_launch / sec / 1e9
     r = {
@@ -135,12 +147,19 @@ def kernel_roofline(kernel, algo_bytes_per_launch, avg_launch_ms, pmc, extra=Non
         r["hbm_measured_frac"] = round(pmc["hbm_bytes_per_launch"] / sec / 1e9 / HBM_PEAK_GBS, 4)
     if "valu_insts_per_launch" in pmc:
         r["valu_issue_frac"] = round(pmc["valu_insts_per_launch"] / sec / VALU_ISSUE_PEAK, 4)
-    vf, hf = r.get("valu_issue_frac", 0.0), r.get("hbm_measured_frac", 1.0)
-    if vf >= 0.5 and vf > hf:
-        r["bound"] = "valu-issue"
+        r["valu_issue_peak"] = round(VALU_ISSUE_PEAK / 1e9, 1)       # G wave-instructions/s
+        r["valu_probe_rate"] = round(valu_probe_rate() / 1e9, 1)     # measured, same unit
+    vf, hf = r.get("valu_issue_frac"), r.get("hbm_measured_frac")
+    if vf is not None and hf is not None:
+        if vf >= 0.7 and vf > hf:
+            r["bound"] = "valu-issue"
+        elif hf < 0.7:
+            r["bound"] = "latency"
         r["note"] = ("frac counts the kernel's algorithmic bytes at full size; most of them hit in L2/MALL, so the "
-                     "bytes crossing HBM (hbm_measured_frac) are far fewer, while VALU issue (valu_issue_frac) "
-                     "is the higher of the two measured rates: the kernel is bound by instruction issue")
+                     "bytes crossing HBM (hbm_measured_frac) are fewer; with VALU issue (valu_issue_frac) below "
+                     "its peak too, the kernel is bound by the latency of its dependent fetches and tests"
+                     if r["bound"] == "latency" else
+                     "frac counts the kernel's algorithmic bytes at full size; the measured rates name the bound")
     if extra:
         r.update(extra)
     return r

@@ -215,6 +215,11 @@ const char* xt_last_error(void);
 /* Opens HIP device `ordinal`; fails with XT_ERR_NO_DEVICE unless it is gfx950. */
 xt_status xt_device_open(int ordinal, xt_device** out);
 xt_status xt_device_close(xt_device* dev);
+/* No reference counterpart: a measurement helper for the kernels' issue-rate
+ * figures.  Times a kernel of 16 independent fp32 FMA chains per lane (no
+ * memory traffic, 8 waves per SIMD on every CU) and returns the wave-level
+ * VALU instructions it issued per second. */
+xt_status xt_device_valu_rate(xt_device* dev, double* wave_instructions_per_s);
 
 /* Scene::addObject x N + Scene::buildAccelerationStructure (Scene.h:17-28):
  * flattens instances to world space, builds the BVH on the host and uploads

@@ -20,7 +20,8 @@ XT_OK = 0
 STATUS_NAMES = {0: "XT_OK", 1: "XT_ERR_INVALID_ARGUMENT", 2: "XT_ERR_UNSUPPORTED_MATERIAL", 3: "XT_ERR_OUT_OF_MEMORY",
                 4: "XT_ERR_DEVICE", 5: "XT_ERR_NO_DEVICE", 6: "XT_ERR_TOO_LARGE"}
 
-EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", "xt_scene_create", "xt_scene_destroy",
+EXPORTED = ["xt_version", "xt_last_error", "xt_device_open", "xt_device_close", "xt_device_valu_rate",
+            "xt_scene_create", "xt_scene_destroy",
             "xt_scene_info", "xt_trace_closest", "xt_trace_any", "xt_render", "xt_render_get_stats",
             "xt_scene_set_counting", "xt_guide_config_default", "xt_guide_init", "xt_guide_get_params",
             "xt_guide_set_params", "xt_guide_get_bounds", "xt_guide_forward", "xt_guide_gradient",
@@ -117,6 +118,7 @@ def lib():
         L.xt_last_error.restype = C.c_char_p
         L.xt_device_open.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.xt_device_close.argtypes = [C.c_void_p]
+        L.xt_device_valu_rate.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.xt_scene_create.argtypes = [C.c_void_p, C.POINTER(xt_scene_desc), C.POINTER(C.c_void_p)]
         L.xt_scene_destroy.argtypes = [C.c_void_p]
         L.xt_scene_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
@@ -173,6 +175,13 @@ class Device:
         if self.h:
             lib().xt_device_close(self.h)
             self.h = None
+
+    def valu_rate(self):
+        """Wave-level VALU instructions per second that a kernel of 16
+        independent fp32 FMA chains per lane issues on this device."""
+        r = C.c_double()
+        check(lib().xt_device_valu_rate(self.h, C.byref(r)))
+        return r.value
 
     def __del__(self):
         try:

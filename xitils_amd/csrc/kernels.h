@@ -200,6 +200,8 @@ __global__ void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* count, flo
                          uint32_t* counts_out);
 __global__ void k_accumulate(WaveParams wp, const float4* radiance, void* film, int film_f64);
 __global__ void k_zero_u32(uint32_t* p, uint32_t n);
+constexpr int VALU_PROBE_CHAINS = 16;   // FMA chains per lane of k_valu_probe
+__global__ void k_valu_probe(uint32_t iters, float* out);
 __global__ void k_cursor_init(BatchCursor* c, BatchPlan plan);
 __global__ void k_advance_cursor(BatchCursor* c, BatchPlan plan);
 __global__ void k_count_from_cursor(uint32_t* count, const BatchCursor* c);

@@ -54,6 +54,24 @@ __device__ __forceinline__ uint2 block_append2(uint32_t* counter_a, bool pa, uin
     return r;
 }
 
+// VALU issue probe: VALU_PROBE_CHAINS independent fp32 FMA chains per lane
+// (one VALU instruction per step, enough independent work to cover the FMA
+// latency at 8 waves per SIMD) and no memory traffic; `out` is written only
+// if the sums take a value they never take, which keeps the chains live.
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float* out) {
+    float a[VALU_PROBE_CHAINS];
+#pragma unroll
+    for (int k = 0; k < VALU_PROBE_CHAINS; ++k) a[k] = 1.0f + 1e-6f * (float)(threadIdx.x + 64 * k);
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < VALU_PROBE_CHAINS; ++k) a[k] = __builtin_fmaf(a[k], 0.9999999f, 1e-7f);
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < VALU_PROBE_CHAINS; ++k) sum += a[k];
+    if (sum == -1.0f) out[blockIdx.x] = sum;
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;

@@ -240,6 +240,36 @@ xt_status xt_device_close(xt_device* dev) {
     return XT_OK;
 }
 
+xt_status xt_device_valu_rate(xt_device* dev, double* rate) {
+    if (!dev || !rate) return fail(XT_ERR_INVALID_ARGUMENT, "NULL argument");
+    XT_HIP(hipSetDevice(dev->ordinal));
+    const uint32_t blocks = (uint32_t)dev->num_cu * 8, iters = 4096;   // 256 threads: 8 waves per SIMD
+    float* out = nullptr;
+    XT_HIP(dalloc(&out, blocks));
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    float best = INFINITY;
+    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // the first launch warms the clocks
+        hipEventRecord(a, nullptr);
+        hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, nullptr, iters, out);
+        e = hipGetLastError();
+        hipEventRecord(b, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess && rep > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess && ms > 0.0f)
+            best = std::min(best, ms);
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipFree(out);
+    if (e != hipSuccess) return fail(XT_ERR_DEVICE, std::string("VALU probe failed: ") + hipGetErrorString(e));
+    if (!std::isfinite(best)) return fail(XT_ERR_DEVICE, "VALU probe could not be timed");
+    const double waves = (double)blocks * 256.0 / 64.0;
+    *rate = waves * (double)iters * VALU_PROBE_CHAINS / (best * 1e-3);
+    return XT_OK;
+}
+
 xt_status xt_scene_destroy(xt_scene* s) {
     if (!s) return XT_OK;
     hipSetDevice(s->dev->ordinal);
