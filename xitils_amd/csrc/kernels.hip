@@ -684,8 +684,10 @@ XT_INST_SHADE(true, true, true)
 
 // Any-hit for queued shadow rays; unoccluded contributions go straight to the
 // path radiance, or to pending[] when the path continues (see k_shade).
+// Bounded to 8 waves per SIMD: every form fits 64 VGPRs without spilling (the
+// chunked ones take 71 unbounded and ran 6% slower at 7 waves).
 template <bool COUNT, bool DYN>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(TRACE_BLOCK, 8) void k_shadow(DevScene sc, ShadowQueue sq, const uint32_t* __restrict__ count,
                                                         float4* __restrict__ radiance, float4* __restrict__ pending,
                                                         uint32_t* __restrict__ ovf, uint32_t* __restrict__ dyn,
                                                         unsigned long long* __restrict__ visit_totals,
