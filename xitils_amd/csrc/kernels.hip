@@ -97,8 +97,10 @@ __device__ __forceinline__ void flush_counts(unsigned long long* totals, const V
 // occluded[i].  Persistent waves with in-place lane refill (xt_traverse.h).
 // MULTI: the rays of a unified batch's queue, each traced in its own item's
 // scene (item_of[path id] -> the item's nodes and triangles, per lane).
+// Bounded to 8 waves per SIMD, which every form but the chunked multi-scene
+// and counting ones reaches without spilling (those stay unbounded).
 template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI, bool DYN>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
+__global__ __launch_bounds__(TRACE_BLOCK, DYN && (MULTI || COUNT) ? 1 : 8) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
                                                         const float4* __restrict__ ray_d, int ray_stride,
                                                         const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
                                                         float4* __restrict__ hits, uint8_t* __restrict__ occluded,
@@ -166,7 +168,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene sc, const float4
                     occluded[rid] = occ ? 1 : 0;
                 } else {
                     hits[rid] = t.best_slot != INVALID
-                                    ? make_float4(t.best_t, __uint_as_float(hit_prim_ids ? t.best_prim : t.best_slot),
+                                    ? make_float4(t.best_t, __uint_as_float(hit_prim_ids ? prim_of_slot(view, t.best_slot)
+                                                                                            : t.best_slot),
                                                   t.b0, t.b1)
                                     : make_float4(INFINITY, __uint_as_float(INVALID), 0.0f, 0.0f);
                 }
@@ -973,7 +976,7 @@ template __global__ void k_shade_multi<true>(MultiBatch, PathQueue, const uint32
 
 // Any-hit for a unified batch's shadow queue, each ray in its item's scene.
 template <bool DYN>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(TRACE_BLOCK, DYN ? 1 : 8) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
                                                               float4* __restrict__ radiance, float4* __restrict__ pending,
                                                               uint32_t* __restrict__ ovf, uint32_t* __restrict__ dyn,
                                                               const uint32_t* __restrict__ item_of,

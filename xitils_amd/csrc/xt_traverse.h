@@ -283,7 +283,7 @@ __device__ __forceinline__ void chunks_release(uint32_t* dyn) {
 struct Trav {
     int node, sp;
     float best_t, b0, b1, cull, tmax;
-    uint32_t best_prim, best_slot;
+    uint32_t best_slot;   // the hit's primitive id is read back from the slot (prim_of_slot)
 };
 
 __device__ __forceinline__ void trav_init(Trav& t, float tmax) {
@@ -293,7 +293,6 @@ __device__ __forceinline__ void trav_init(Trav& t, float tmax) {
     t.b0 = t.b1 = 0.0f;
     t.cull = tmax;
     t.tmax = tmax;
-    t.best_prim = INVALID;
     t.best_slot = INVALID;
 }
 
@@ -420,6 +419,12 @@ __device__ __forceinline__ bool visit_node(const SV& sc, const RayPre& r, Trav& 
     return false;
 }
 
+// Primitive id of the triangle in sorted slot `slot` (the w of its first vertex).
+template <class SV>
+__device__ __forceinline__ uint32_t prim_of_slot(const SV& sc, uint32_t slot) {
+    return __float_as_uint(sc.tris[(size_t)slot * TRI_F4].w);
+}
+
 // Tests the first triangle of leaf code `leaf` (< 0); returns the code of the
 // rest of the leaf, or 0 when the leaf is done (or, for ANY, when *occ was set).
 template <bool ANY, bool COUNT, class SV>
@@ -436,10 +441,12 @@ __device__ __forceinline__ int visit_tri(const SV& sc, const RayPre& r, Trav& t,
             *occ = true;
             return 0;
         }
-        uint32_t prim = __float_as_uint(v0.w);
-        if (th < t.best_t || (th == t.best_t && prim < t.best_prim)) {
+        // equal distances go to the lower primitive id; the current best's id is
+        // read back from its slot only then (exact ties are rare), which keeps
+        // it out of the registers the traversal loop holds
+        if (th < t.best_t ||
+            (th == t.best_t && (t.best_slot == INVALID || __float_as_uint(v0.w) < prim_of_slot(sc, t.best_slot)))) {
             t.best_t = th;
-            t.best_prim = prim;
             t.best_slot = first;
             t.b0 = b0;
             t.b1 = b1;
