@@ -98,9 +98,10 @@ __device__ __forceinline__ void flush_counts(unsigned long long* totals, const V
 // MULTI: the rays of a unified batch's queue, each traced in its own item's
 // scene (item_of[path id] -> the item's nodes and triangles, per lane).
 // Bounded to 8 waves per SIMD, which every form but the chunked multi-scene
-// and counting ones reaches without spilling (those stay unbounded).
+// and counting ones reaches without spilling; the chunked multi-scene form
+// takes 7, the chunked counting form (run once per scene) is unbounded.
 template <bool ANY, int LDS_DEPTH, bool COUNT, bool MULTI, bool DYN>
-__global__ __launch_bounds__(TRACE_BLOCK, DYN && (MULTI || COUNT) ? 1 : 8) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
+__global__ __launch_bounds__(TRACE_BLOCK, DYN && COUNT ? 1 : DYN && MULTI ? 7 : 8) void k_trace(DevScene sc, const float4* __restrict__ ray_o,
                                                         const float4* __restrict__ ray_d, int ray_stride,
                                                         const uint32_t* __restrict__ count_ptr, uint32_t count_fixed,
                                                         float4* __restrict__ hits, uint8_t* __restrict__ occluded,
@@ -975,8 +976,10 @@ template __global__ void k_shade_multi<true>(MultiBatch, PathQueue, const uint32
                                              uint32_t*, ShadowQueue, uint32_t*, float4*, float4*);
 
 // Any-hit for a unified batch's shadow queue, each ray in its item's scene.
+// 8 waves per SIMD for the split form, 7 for the chunked one (65 VGPRs; 8
+// would spill).
 template <bool DYN>
-__global__ __launch_bounds__(TRACE_BLOCK, DYN ? 1 : 8) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(TRACE_BLOCK, DYN ? 7 : 8) void k_shadow_multi(ShadowQueue sq, const uint32_t* __restrict__ count,
                                                               float4* __restrict__ radiance, float4* __restrict__ pending,
                                                               uint32_t* __restrict__ ovf, uint32_t* __restrict__ dyn,
                                                               const uint32_t* __restrict__ item_of,
