@@ -647,8 +647,10 @@ __device__ __forceinline__ void shade_emit(ShadeOut& o, const PathQueue& qout, u
     }
 }
 
+// The forms without normal mapping (but the guided sorted one, at 144 VGPRs)
+// are bounded to 4 waves per SIMD, which they reach without spilling.
 template <bool GUIDE, bool NMAP, bool SORT>
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(WaveParams wp, const DevScene* __restrict__ scene, PathQueue qin,
+__global__ __launch_bounds__(SHADE_BLOCK, NMAP || (GUIDE && SORT) ? 1 : 4) void k_shade(WaveParams wp, const DevScene* __restrict__ scene, PathQueue qin,
                                                const uint32_t* __restrict__ count_in, const float4* __restrict__ hits,
                                                PathQueue qout, uint32_t* __restrict__ count_out, ShadowQueue sq,
                                                uint32_t* __restrict__ count_shadow, float4* __restrict__ radiance,

@@ -343,11 +343,14 @@ XT_HD f3 tex_bilinear(const TexRef& m, float u, float v) {
 }
 // lat-long coordinates of a unit direction (y up, row 0 at the zenith).  Out
 // of line: the double-precision atan2 and acos would otherwise add their
-// registers to every kernel that can evaluate a sky pdf.
-static __host__ __device__ __noinline__ void env_latlong_uv(f3 d, float* u, float* v) {
+// registers to every kernel that can evaluate a sky pdf.  Returned by value
+// (in registers; output pointers would go through the call's stack).
+struct LatLongUV {
+    float u, v;
+};
+static __host__ __device__ __noinline__ LatLongUV env_latlong_uv(f3 d) {
     const float PI = 3.14159265358979323846f;
-    *u = (xt_atan2f(d.z, d.x) + PI) / (2.0f * PI);
-    *v = xt_acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI;
+    return {(xt_atan2f(d.z, d.x) + PI) / (2.0f * PI), xt_acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) / PI};
 }
 // SkySphereFromImage::getRadiance (SkySphere.h:20-33) or the lat-long map.
 // Kept out of line: inlined, the lookup's registers lower the occupancy of
@@ -361,9 +364,8 @@ static __host__ __device__ __noinline__ f3 env_sky_map(int kind, int w, int h, c
         const float r = (float)((1.0 / 3.14159265358979323846) * (double)xt_acosf(d.z) / (double)sq);
         return tex_bilinear(m, (d.x * r + 1.0f) / 2.0f, (d.y * r + 1.0f) / 2.0f);
     }
-    float u, v;
-    env_latlong_uv(normalize(wi), &u, &v);
-    return tex_bilinear(m, u, v);
+    const LatLongUV uv = env_latlong_uv(normalize(wi));
+    return tex_bilinear(m, uv.u, uv.v);
 }
 // the image sky, or SkySphereUniform::getRadiance (SkySphere.h:40-42)
 XT_HD f3 env_sky(const DevEnv& e, f3 wi) {
@@ -405,9 +407,8 @@ XT_HD f3 env_map_sample(const DevEnv& e, float u1, float u2) {
     return mk(st * cp, ct, st * sp);
 }
 XT_HD float env_map_pdf(const DevEnv& e, f3 du) {
-    float u, v;
-    env_latlong_uv(du, &u, &v);
-    const int x = min((int)(u * e.map_w), e.map_w - 1), y = min((int)(v * e.map_h), e.map_h - 1);
+    const LatLongUV uv = env_latlong_uv(du);
+    const int x = min((int)(uv.u * e.map_w), e.map_w - 1), y = min((int)(uv.v * e.map_h), e.map_h - 1);
     const float st = sqrtf(fmaxf(0.0f, 1.0f - du.y * du.y));
     return st > 0.0f ? e.map_pdf[(size_t)y * e.map_w + x] / st : 0.0f;
 }
