@@ -21,7 +21,10 @@ constexpr int TRACE_BLOCK = XT_TRACE_BLOCK;          // lanes per block of the t
 constexpr int TRACE_LDS_DEPTH = XT_TRACE_LDS_DEPTH;  // LDS stack entries per lane
 // queue length (an upper bound the host knows) from which the traversal
 // kernels hand out rays in chunks from a cursor (DYN, xt_traverse.h)
-constexpr uint32_t TRAV_DYN_MIN = 8u << 20;
+#ifndef XT_TRAV_DYN_MIN
+#define XT_TRAV_DYN_MIN (8u << 20)
+#endif
+constexpr uint32_t TRAV_DYN_MIN = XT_TRAV_DYN_MIN;
 
 // Where the current batch lies in the render: pixel range, paths, first
 // sample.  Host-driven renders pass it in WaveParams; graph replays read it
