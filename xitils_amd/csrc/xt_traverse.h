@@ -211,7 +211,10 @@ struct VisitCount {
 // finished their ray, the idle lanes take the next rays of the wave's chunk
 // (RayChunks below).
 // Each loop iteration advances every busy lane by one node visit or one leaf.
-constexpr int REFILL_IDLE = 16;   // 8 and 32 measured no better
+#ifndef XT_REFILL_IDLE
+#define XT_REFILL_IDLE 16
+#endif
+constexpr int REFILL_IDLE = XT_REFILL_IDLE;   // 8 and 24: 1080p frame 1.2% and 0.3% slower; 32 no better
 
 // Which rays a wave traces.  DYN = false: an even contiguous share per wave.
 // DYN = true (large queues): the queue is cut into chunks of a quarter of an
